@@ -1,0 +1,201 @@
+"""Synthetic scene generation for the Newton-step path.
+
+Regular tet grids (5 tets per cell, alternating parity), block partitions into
+<= bank-vertex parts with part-contiguous vertex numbering (the ordering the
+reference's MAS assumes, produced there by MeshProcess/metis_partition/src/
+metis_sort.cpp:25-383), rest-state precomputation (S/gl_main.cu:557-617), and
+the workload configurations the benchmarks and tests use.
+
+This is host-side input preparation in numpy; it is not on the timed path.
+"""
+from dataclasses import dataclass, field
+import numpy as np
+
+# corner offsets of a cell, index = a + 2 b + 4 c for offset (a, b, c)
+_CORNERS = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=np.int64)
+
+
+def _cid(a, b, c):
+    return a + 2 * b + 4 * c
+
+
+# even parity: central tet on the even corners {000,110,101,011}
+_TETS_EVEN = [
+    (_cid(0, 0, 0), _cid(1, 1, 0), _cid(1, 0, 1), _cid(0, 1, 1)),
+    (_cid(1, 0, 0), _cid(0, 0, 0), _cid(1, 1, 0), _cid(1, 0, 1)),
+    (_cid(0, 1, 0), _cid(0, 0, 0), _cid(1, 1, 0), _cid(0, 1, 1)),
+    (_cid(0, 0, 1), _cid(0, 0, 0), _cid(1, 0, 1), _cid(0, 1, 1)),
+    (_cid(1, 1, 1), _cid(1, 1, 0), _cid(1, 0, 1), _cid(0, 1, 1)),
+]
+# odd parity: central tet on the odd corners {100,010,001,111}
+_TETS_ODD = [
+    (_cid(1, 0, 0), _cid(0, 1, 0), _cid(0, 0, 1), _cid(1, 1, 1)),
+    (_cid(0, 0, 0), _cid(1, 0, 0), _cid(0, 1, 0), _cid(0, 0, 1)),
+    (_cid(1, 1, 0), _cid(1, 0, 0), _cid(0, 1, 0), _cid(1, 1, 1)),
+    (_cid(1, 0, 1), _cid(1, 0, 0), _cid(0, 0, 1), _cid(1, 1, 1)),
+    (_cid(0, 1, 1), _cid(0, 1, 0), _cid(0, 0, 1), _cid(1, 1, 1)),
+]
+
+
+def tet_grid(nx, ny, nz, h=1.0, origin=(0.0, 0.0, 0.0)):
+    """Vertices (grid order, x fastest) and positively oriented tets."""
+    gx, gy, gz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    ijk = np.stack([gx.transpose(2, 1, 0).ravel(), gy.transpose(2, 1, 0).ravel(), gz.transpose(2, 1, 0).ravel()], 1)
+    verts = ijk.astype(np.float64) * h + np.asarray(origin, dtype=np.float64)
+
+    def vid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci = ci.transpose(2, 1, 0).ravel(); cj = cj.transpose(2, 1, 0).ravel(); ck = ck.transpose(2, 1, 0).ravel()
+    corner_ids = np.stack([vid(ci + o[0], cj + o[1], ck + o[2]) for o in _CORNERS], 1)  # (cells, 8)
+    parity = (ci + cj + ck) % 2
+    tets = np.empty((ci.shape[0], 5, 4), dtype=np.int64)
+    even = np.asarray(_TETS_EVEN); odd = np.asarray(_TETS_ODD)
+    tets[parity == 0] = corner_ids[parity == 0][:, even]
+    tets[parity == 1] = corner_ids[parity == 1][:, odd]
+    tets = tets.reshape(-1, 4)
+    # orient positively
+    x = verts
+    d = np.einsum("ij,ij->i", np.cross(x[tets[:, 1]] - x[tets[:, 0]], x[tets[:, 2]] - x[tets[:, 0]]),
+                  x[tets[:, 3]] - x[tets[:, 0]])
+    neg = d < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    return verts, tets.astype(np.uint32), ijk
+
+
+def _morton3(i, j, k):
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+        v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+        v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+        return v
+    return spread(i) | (spread(j) << np.uint64(1)) | (spread(k) << np.uint64(2))
+
+
+def block_partition(ijk, bank=16, block=None):
+    """Part id per vertex: vertex-grid blocks of <= bank vertices (2x2x4 for
+    bank 16, 2x4x4 for 32), blocks numbered in Morton order so consecutive
+    parts are spatially compact.  Returns part ids (not yet sorted)."""
+    if block is None:
+        block = (2, 2, 4) if bank == 16 else (2, 4, 4)
+    b = np.asarray(block)
+    bc = ijk // b
+    code = _morton3(bc[:, 0], bc[:, 1], bc[:, 2])
+    uniq, part = np.unique(code, return_inverse=True)
+    return part.astype(np.int32)
+
+
+def renumber_by_part(part, *arrays_to_permute):
+    """Stable sort of vertices by part id; returns (perm, inv, sorted part)."""
+    perm = np.argsort(part, kind="stable")
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.shape[0])
+    return perm, inv, part[perm]
+
+
+@dataclass
+class Body:
+    """One deformable body of the FEM subsystem (already part-sorted)."""
+    x_rest: np.ndarray
+    tets: np.ndarray
+    part: np.ndarray
+    btype: np.ndarray
+
+
+@dataclass
+class Scene:
+    name: str
+    x_rest: np.ndarray          # (n,3) float64 rest positions
+    x: np.ndarray               # (n,3) current positions (Newton iterate)
+    x_tilde: np.ndarray         # (n,3) inertial target
+    tets: np.ndarray            # (T,4) uint32
+    part: np.ndarray            # (n,) int32 ascending part ids
+    btype: np.ndarray           # (n,) int32 (nonzero = pinned)
+    DmInv: np.ndarray           # (T,9) row-major
+    vol: np.ndarray
+    mass: np.ndarray
+    mu: np.ndarray
+    lam: np.ndarray
+    dt: float
+    pairs: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
+    dhat: float = 0.0           # squared barrier distance (reference convention)
+    kappa: float = 0.0
+    bank: int = 16
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+    @property
+    def T(self):
+        return self.tets.shape[0]
+
+
+def lame_snk(E, nu):
+    """(mu', lambda') of the SNK1 energy from (E, nu) (S/gl_main.cu:609-617)."""
+    lmu = E / (2.0 * (1.0 + nu))
+    llam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return 4.0 * lmu / 3.0, llam + 5.0 * lmu / 6.0
+
+
+def rest_precompute(x_rest, tets, E, nu, density):
+    """DmInv (row-major), volume, lumped mass, mu', lambda' per tet."""
+    x = x_rest
+    t = tets.astype(np.int64)
+    Dm = np.stack([x[t[:, 1]] - x[t[:, 0]], x[t[:, 2]] - x[t[:, 0]], x[t[:, 3]] - x[t[:, 0]]], axis=2)
+    DmInv = np.linalg.inv(Dm).reshape(-1, 9)
+    vol = np.abs(np.linalg.det(Dm)) / 6.0
+    mass = np.zeros(x.shape[0])
+    for a in range(4):
+        np.add.at(mass, t[:, a], vol * density / 4.0)
+    E = np.broadcast_to(np.asarray(E, dtype=np.float64), vol.shape)
+    mu, lam = lame_snk(E, nu)
+    return DmInv, vol, mass, np.ascontiguousarray(mu), np.ascontiguousarray(lam)
+
+
+def make_box(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), bank=16, pin_bottom=False):
+    verts, tets, ijk = tet_grid(nx, ny, nz, h, origin)
+    part = block_partition(ijk, bank)
+    perm, inv, part_sorted = renumber_by_part(part)
+    x = verts[perm]
+    tets = inv[tets.astype(np.int64)].astype(np.uint32)
+    ijk = ijk[perm]
+    btype = np.zeros(x.shape[0], np.int32)
+    if pin_bottom:
+        btype[ijk[:, 1] == 0] = 1
+    return Body(x, tets, part_sorted.astype(np.int32), btype)
+
+
+def concat_bodies(bodies):
+    xs, ts, ps, bs = [], [], [], []
+    voff = 0; poff = 0
+    for b in bodies:
+        xs.append(b.x_rest); ts.append(b.tets.astype(np.int64) + voff)
+        ps.append(b.part.astype(np.int64) + poff); bs.append(b.btype)
+        voff += b.x_rest.shape[0]; poff += int(b.part.max()) + 1
+    return (np.concatenate(xs), np.concatenate(ts).astype(np.uint32), np.concatenate(ps).astype(np.int32),
+            np.concatenate(bs).astype(np.int32))
+
+
+def cube_scene(cells=10, seed=1, E=1e5, nu=0.45, density=1000.0, dt=0.01, gravity=-9.8, sigma_rel=0.01,
+               bank=16, size=1.0):
+    """Unit cube, cells^3 grid, 5 tets/cell, bottom face pinned, seeded
+    Gaussian perturbation of the positions (the no-contact FP64 workload)."""
+    h = size / cells
+    body = make_box(cells, cells, cells, h, bank=bank, pin_bottom=True)
+    rng = np.random.default_rng(seed)
+    x_rest = body.x_rest
+    DmInv, vol, mass, mu, lam = rest_precompute(x_rest, body.tets, E, nu, density)
+    x = x_rest + rng.normal(0.0, sigma_rel * h, size=x_rest.shape)
+    x[body.btype != 0] = x_rest[body.btype != 0]
+    v = np.zeros_like(x)
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = x + dt * v + dt * dt * g
+    x_tilde[body.btype != 0] = x[body.btype != 0]
+    return Scene("cube%d" % cells, x_rest, x, x_tilde, body.tets, body.part, body.btype, DmInv, vol, mass,
+                 mu, lam, dt, bank=bank, meta=dict(cells=cells, E=E, nu=nu, seed=seed))
