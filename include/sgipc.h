@@ -1,0 +1,162 @@
+/*
+ * sgipc.h -- C ABI of the MI355X (gfx950) Newton-step library: FEM + barrier
+ * assembly into block-3x3 triplets, triplet -> BCOO conversion, symmetric
+ * SpMV, MAS / block-Jacobi preconditioners and PCG.
+ *
+ * It replaces the C++ linear-system boundary of Stiff-GIPC
+ * (namespace gipc, StiffGIPC/linear_system/), keeping its data layout so
+ * reference-format triplets can be fed unchanged:
+ *   - block values: 9 doubles per 3x3 block, COLUMN-major (Eigen::Matrix3d);
+ *   - block indices: int32 in 3-DOF units, upper triangle (row <= col);
+ *   - vertices: double3 AoS; tets: uint32 x4; DmInv: 3x3 row-major per tet;
+ *   - contact pairs: int4 in the reference sign encoding.
+ *
+ * Conventions: every function returns SGIPC_OK (0) or an error code and never
+ * aborts; sgipc_last_error() describes the last failure on a context.  Device
+ * pointers ("d_") are caller-owned device memory; host pointers ("h_") are
+ * host memory.  Work is enqueued on the context's stream; functions that
+ * return host values (counts, solver statistics) synchronise that stream.
+ * A context is driven by one host thread at a time.
+ */
+#ifndef SGIPC_H
+#define SGIPC_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGIPC_OK 0
+#define SGIPC_ERR_INVALID_ARG 1
+#define SGIPC_ERR_HIP 2
+#define SGIPC_ERR_CAPACITY 3
+#define SGIPC_ERR_STATE 4
+#define SGIPC_ERR_INTERNAL 5
+
+#define SGIPC_PRECOND_NONE 0
+#define SGIPC_PRECOND_DIAG 1     /* 3x3 block Jacobi (diag_preconditioner.cu) */
+#define SGIPC_PRECOND_MAS 2      /* multilevel additive Schwarz, FP64 apply */
+#define SGIPC_PRECOND_MAS_FP32 3 /* MAS with FP32 inverses/apply (reference precision) */
+
+#define SGIPC_PRECISION_FP64 0
+#define SGIPC_PRECISION_MIXED 1
+
+#define SGIPC_STOP_REF_RZ 0      /* |r.z| <= tol * r0.z0 on the previous rz (pcg_solver.cu:232) */
+#define SGIPC_STOP_TRUE_REL 1    /* ||r||_2 <= tol * ||b||_2 */
+
+typedef struct sgipc_ctx sgipc_ctx;
+
+typedef struct sgipc_solve_opts {
+    int precision;        /* SGIPC_PRECISION_* */
+    int stop_rule;        /* SGIPC_STOP_* */
+    double tol;           /* reference default 1e-4 (parameterSetting.txt:14) */
+    double max_iter_ratio;/* max iterations = ratio * dof (pcg_solver.h:13, default 0.3) */
+    int max_iter;         /* >0 overrides max_iter_ratio */
+    int check_every;      /* host polls the device stop flag every k iterations */
+} sgipc_solve_opts;
+
+typedef struct sgipc_solve_stats {
+    int iterations;       /* reference iteration count (pcg_solver.cu:267) */
+    int converged;
+    int refinements;      /* outer refinement steps (mixed precision) */
+    double rel_residual;  /* true ||b - A x|| / ||b||, recomputed in FP64 */
+    double rz0;
+} sgipc_solve_stats;
+
+/* ---- context -------------------------------------------------------------- */
+int sgipc_version(void);
+/* replaces GlobalLinearSystem construction (global_linear_system.h:24) */
+int sgipc_ctx_create(int device, sgipc_ctx** out);
+int sgipc_ctx_destroy(sgipc_ctx* ctx);
+/* run on a caller stream (hipStream_t); NULL restores the context's own */
+int sgipc_ctx_set_stream(sgipc_ctx* ctx, void* hip_stream);
+int sgipc_ctx_synchronize(sgipc_ctx* ctx);
+const char* sgipc_last_error(const sgipc_ctx* ctx);
+
+/* ---- host-side topology preprocessing ------------------------------------- */
+/* Vertex neighbour CSR from tets; replaces tetrahedra_obj::getVertNeighbors
+ * (StiffGIPC/load_mesh.cpp:934-1040).  list must hold 12*T entries. */
+int sgipc_build_vertex_neighbors(int n, int T, const uint32_t* h_tets, uint32_t* h_start,
+                                 uint32_t* h_num, uint32_t* h_list, int list_capacity,
+                                 int* list_len);
+/* Bank slot maps from ascending part ids; replaces setMAS_partition
+ * (StiffGIPC/gl_main.cu:1066-1093).  part_map needs bank*(last part+1). */
+int sgipc_build_bank_map(int n, const int* h_part_id, int bank, int* h_part_map,
+                         int part_map_capacity, int* map_size, int* h_real_map);
+/* Number of MAS levels for a node count; MASPreconditioner::computeNumLevels
+ * (StiffGIPC/MASPreconditioner.cu:1724-1743). */
+int sgipc_mas_num_levels(int n_nodes, int bank);
+
+/* ---- one-time setup ------------------------------------------------------- */
+/* FEM mesh (host tets) for the deterministic vertex-gradient gather. */
+int sgipc_set_fem_mesh(sgipc_ctx* ctx, int n_fem, int T, const uint32_t* h_tets);
+/* MAS topology; replaces initPreconditioner_Neighbor + the uploads of
+ * StiffGIPC/gl_main.cu:1294-1328 (MASPreconditioner.cu:2264-2302). */
+int sgipc_set_mas_topology(sgipc_ctx* ctx, int n_fem, int node_offset, const uint32_t* h_nstart,
+                           const uint32_t* h_nnum, const uint32_t* h_nlist, int nlist_len,
+                           const int* h_part_map, int map_size, const int* h_real_map, int bank);
+/* Caller-owned triplet store; replaces GIPCTripletMatrix
+ * (linear_system/linear_system/global_matrix.h:18-203). */
+int sgipc_triplets_view(sgipc_ctx* ctx, double* d_vals, int* d_rows, int* d_cols,
+                        long long capacity);
+
+/* ---- assembly ------------------------------------------------------------- */
+/* d_shape_grad = m (x - xtilde); _calKineticGradient (StiffGIPC/GIPC.cu:6339-6350) */
+int sgipc_kinetic_gradient(sgipc_ctx* ctx, int n, const double* d_x, const double* d_xtilde,
+                           const double* d_mass, double* d_shape_grad);
+/* SNK1 tet gradient (+= into d_shape_grad) and 10 upper triplets per tet at
+ * triplet_offset, boundary rows/cols zeroed; _calculate_fem_gradient_hessian
+ * (StiffGIPC/femEnergy.cu:1719-1857) + mask (StiffGIPC/GIPC.cu:10295-10313). */
+int sgipc_assemble_fem(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d_tets,
+                       const double* d_DmInv, const double* d_vol, const double* d_mu,
+                       const double* d_lambda, double dt, const int* d_btype, int index_offset,
+                       long long triplet_offset, double* d_shape_grad);
+/* m_i I3 diagonal triplets (StiffGIPC/GIPC.cu:10317-10336) */
+int sgipc_mass_triplets(sgipc_ctx* ctx, int n, const double* d_mass, int index_offset,
+                        long long triplet_offset);
+/* b = fb + shape (0 on pinned vertices); FEMLinearSubsystem::assemble
+ * (linear_system/subsystem/fem_linear_subsystem.cu:56-84).  d_fb may be NULL. */
+int sgipc_fem_rhs(sgipc_ctx* ctx, int n, const double* d_fb, const double* d_shape_grad,
+                  const int* d_btype, double* d_b);
+
+/* ---- linear system -------------------------------------------------------- */
+/* Triplets [0, n_triplets) -> unique upper BCOO sorted by (row, col), written
+ * back in place at offset 0; Converter::convert
+ * (linear_system/utils/converter.cu:23-196). */
+int sgipc_convert(sgipc_ctx* ctx, long long n_triplets, int n_block_rows, int* n_unique);
+/* y = a A x + b y on the converted system; Spmv::warp_reduce_sym_spmv
+ * (linear_system/utils/spmv.cu:8-133). */
+int sgipc_spmv(sgipc_ctx* ctx, double a, const double* d_x, double b, double* d_y);
+/* Build a preconditioner over block rows [sub_offset, sub_offset+sub_nb):
+ * MAS_Preconditioner::assemble (preconditioner/fem_mas_preconditioner.cu:
+ * 18-30) -> MASPreconditioner::setPreconditioner_bcoo
+ * (MASPreconditioner.cu:2180-2218); DiagPreconditioner::assemble
+ * (preconditioner/diag_preconditioner.cu:53-59).  d_pairs: int4 contact
+ * pairs (may be NULL). */
+int sgipc_precond_build(sgipc_ctx* ctx, int kind, int sub_offset, int sub_nb,
+                        const int* d_pairs, int n_pairs);
+/* z = M^-1 r; MASPreconditioner::preconditioning (MASPreconditioner.cu:2221-2262) */
+int sgipc_precond_apply(sgipc_ctx* ctx, const double* d_r, double* d_z);
+/* PCG solve of A x = b; PCGSolver::solve (linear_system/solver/pcg_solver.cu:158-267) */
+int sgipc_solve(sgipc_ctx* ctx, const double* d_b, double* d_x, const sgipc_solve_opts* opts,
+                sgipc_solve_stats* stats);
+
+/* ---- introspection (parity tests) ----------------------------------------- */
+int sgipc_mas_info(sgipc_ctx* ctx, int* levelnum, int* total_clusters, int* h_level_size);
+/* copies tables to host: coarse_space L*n, going_next (total or n), coarse
+ * table 6*n (itable), fine masks n, prefix per level-0 bank M/bank */
+int sgipc_mas_tables(sgipc_ctx* ctx, int* h_coarse_space, int* h_going_next, int* h_coarse_table,
+                     uint32_t* h_fine_mask, int* h_prefix0);
+/* dense FP64 inverse (symmetric, 3b x 3b row-major) of one bank, to host */
+int sgipc_mas_bank_inverse(sgipc_ctx* ctx, int bank_index, double* h_inv);
+
+/* ---- device memory helpers for callers without a framework ---------------- */
+int sgipc_malloc(sgipc_ctx* ctx, void** d_ptr, size_t bytes);
+int sgipc_free(sgipc_ctx* ctx, void* d_ptr);
+int sgipc_memcpy_h2d(sgipc_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int sgipc_memcpy_d2h(sgipc_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,340 @@
+// extern "C" boundary of libsgipc (declarations: include/sgipc.h).
+#include "../../include/sgipc.h"
+#include "context.hpp"
+
+#include <new>
+
+namespace sgipc {
+int build_vertex_neighbors(int n, int T, const uint32_t* tets, uint32_t* start, uint32_t* num, uint32_t* list,
+                           long long cap);
+int build_bank_map(int n, const int* part, int bank, int* part_map, long long cap, int* real_map);
+int mas_num_levels(int nn, int bank);
+}  // namespace sgipc
+
+struct sgipc_ctx : sgipc::Ctx {};
+
+namespace {
+thread_local std::string g_free_error;
+
+template <typename F>
+int guard(sgipc_ctx* c, F&& f) {
+    try {
+        if (c) c->last_error.clear();
+        f();
+        return SGIPC_OK;
+    } catch (const sgipc::arg_error& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_INVALID_ARG;
+    } catch (const std::invalid_argument& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_INVALID_ARG;
+    } catch (const sgipc::capacity_error& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_CAPACITY;
+    } catch (const std::length_error& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_CAPACITY;
+    } catch (const sgipc::state_error& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_STATE;
+    } catch (const sgipc::hip_error& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_HIP;
+    } catch (const std::exception& e) {
+        if (c) c->last_error = e.what(); else g_free_error = e.what();
+        return SGIPC_ERR_INTERNAL;
+    } catch (...) {
+        if (c) c->last_error = "unknown error"; else g_free_error = "unknown error";
+        return SGIPC_ERR_INTERNAL;
+    }
+}
+
+void set_device(sgipc_ctx* c) { SGIPC_HIP(hipSetDevice(c->device)); }
+
+#define CHECK_CTX(c)                                                     \
+    do {                                                                 \
+        if (!(c)) { g_free_error = "null context"; return SGIPC_ERR_INVALID_ARG; } \
+    } while (0)
+}  // namespace
+
+extern "C" {
+
+int sgipc_version(void) { return 1; }
+
+int sgipc_ctx_create(int device, sgipc_ctx** out) {
+    if (!out) { g_free_error = "null output pointer"; return SGIPC_ERR_INVALID_ARG; }
+    *out = nullptr;
+    sgipc_ctx* c = new (std::nothrow) sgipc_ctx();
+    if (!c) { g_free_error = "out of host memory"; return SGIPC_ERR_INTERNAL; }
+    int rc = guard(nullptr, [&] {
+        int ndev = 0;
+        SGIPC_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw sgipc::arg_error("invalid device index");
+        c->device = device;
+        SGIPC_HIP(hipSetDevice(device));
+        SGIPC_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
+    });
+    if (rc != SGIPC_OK) { delete c; return rc; }
+    *out = c;
+    return SGIPC_OK;
+}
+
+int sgipc_ctx_destroy(sgipc_ctx* c) {
+    if (!c) return SGIPC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->own_stream) {
+        (void)hipStreamSynchronize(c->own_stream);
+        (void)hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+    return SGIPC_OK;
+}
+
+int sgipc_ctx_set_stream(sgipc_ctx* c, void* stream) {
+    CHECK_CTX(c);
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return SGIPC_OK;
+}
+
+int sgipc_ctx_synchronize(sgipc_ctx* c) {
+    CHECK_CTX(c);
+    return guard(c, [&] { set_device(c); SGIPC_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+const char* sgipc_last_error(const sgipc_ctx* c) { return c ? c->last_error.c_str() : g_free_error.c_str(); }
+
+int sgipc_build_vertex_neighbors(int n, int T, const uint32_t* tets, uint32_t* start, uint32_t* num, uint32_t* list,
+                                 int cap, int* len) {
+    return guard(nullptr, [&] {
+        if (n < 0 || T < 0 || (T > 0 && !tets) || !start || !num || !list || !len)
+            throw sgipc::arg_error("build_vertex_neighbors: invalid argument");
+        *len = sgipc::build_vertex_neighbors(n, T, tets, start, num, list, cap);
+    });
+}
+
+int sgipc_build_bank_map(int n, const int* part, int bank, int* part_map, int cap, int* M, int* real_map) {
+    return guard(nullptr, [&] {
+        if (n < 0 || (n > 0 && (!part || !part_map || !real_map)) || !M || (bank != 16 && bank != 32))
+            throw sgipc::arg_error("build_bank_map: invalid argument");
+        *M = sgipc::build_bank_map(n, part, bank, part_map, cap, real_map);
+    });
+}
+
+int sgipc_mas_num_levels(int nn, int bank) { return sgipc::mas_num_levels(nn, bank); }
+
+int sgipc_set_fem_mesh(sgipc_ctx* c, int n, int T, const uint32_t* h_tets) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (T > 0 && !h_tets) throw sgipc::arg_error("set_fem_mesh: null tets");
+        set_device(c);
+        sgipc::fem_set_mesh(*c, n, T, h_tets);
+    });
+}
+
+int sgipc_set_mas_topology(sgipc_ctx* c, int n, int node_offset, const uint32_t* ns, const uint32_t* nn,
+                           const uint32_t* nl, int nlen, const int* pm, int M, const int* rm, int bank) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n > 0 && (!ns || !nn || !rm)) throw sgipc::arg_error("set_mas_topology: null array");
+        if (M > 0 && !pm) throw sgipc::arg_error("set_mas_topology: null part map");
+        if (nlen > 0 && !nl) throw sgipc::arg_error("set_mas_topology: null neighbour list");
+        set_device(c);
+        sgipc::mas_set_topology(*c, n, node_offset, ns, nn, nl, nlen, pm, M, rm, bank);
+    });
+}
+
+int sgipc_triplets_view(sgipc_ctx* c, double* v, int* r, int* col, long long cap) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (cap < 0 || (cap > 0 && (!v || !r || !col))) throw sgipc::arg_error("triplets_view: invalid arrays");
+        c->t_vals = v;
+        c->t_rows = r;
+        c->t_cols = col;
+        c->t_cap = cap;
+        c->have_bcoo = false;
+        c->have_bsr = false;
+    });
+}
+
+int sgipc_kinetic_gradient(sgipc_ctx* c, int n, const double* x, const double* xt, const double* m, double* g) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || (n > 0 && (!x || !xt || !m || !g))) throw sgipc::arg_error("kinetic_gradient: invalid argument");
+        set_device(c);
+        sgipc::fem_kinetic_gradient(*c, n, x, xt, m, g);
+    });
+}
+
+int sgipc_assemble_fem(sgipc_ctx* c, int T, const double* x, const uint32_t* tets, const double* DmInv,
+                       const double* vol, const double* mu, const double* lam, double dt, const int* btype,
+                       int index_offset, long long triplet_offset, double* shape) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (T < 0 || (T > 0 && (!x || !tets || !DmInv || !vol || !mu || !lam || !shape)))
+            throw sgipc::arg_error("assemble_fem: invalid argument");
+        set_device(c);
+        sgipc::fem_assemble(*c, T, x, tets, DmInv, vol, mu, lam, dt, btype, index_offset, triplet_offset, shape);
+    });
+}
+
+int sgipc_mass_triplets(sgipc_ctx* c, int n, const double* m, int index_offset, long long triplet_offset) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || (n > 0 && !m)) throw sgipc::arg_error("mass_triplets: invalid argument");
+        set_device(c);
+        sgipc::fem_mass_triplets(*c, n, m, index_offset, triplet_offset);
+    });
+}
+
+int sgipc_fem_rhs(sgipc_ctx* c, int n, const double* fb, const double* sh, const int* bt, double* b) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || (n > 0 && (!sh || !b))) throw sgipc::arg_error("fem_rhs: invalid argument");
+        set_device(c);
+        sgipc::fem_rhs(*c, n, fb, sh, bt, b);
+    });
+}
+
+int sgipc_convert(sgipc_ctx* c, long long nt, int nbr, int* n_unique) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        set_device(c);
+        int K = sgipc::bcoo_convert(*c, nt, nbr);
+        if (n_unique) *n_unique = K;
+    });
+}
+
+int sgipc_spmv(sgipc_ctx* c, double a, const double* x, double b, double* y) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!x || !y) throw sgipc::arg_error("spmv: null vector");
+        if (!c->have_bcoo) throw sgipc::state_error("spmv: convert first");
+        set_device(c);
+        sgipc::spmv64(*c, a, x, b, y);
+    });
+}
+
+int sgipc_precond_build(sgipc_ctx* c, int kind, int sub_offset, int sub_nb, const int* pairs, int npairs) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (kind < SGIPC_PRECOND_NONE || kind > SGIPC_PRECOND_MAS_FP32) throw sgipc::arg_error("unknown preconditioner");
+        if (npairs < 0 || (npairs > 0 && !pairs)) throw sgipc::arg_error("precond_build: invalid pairs");
+        if (!c->have_bcoo) throw sgipc::state_error("precond_build: convert first");
+        if (sub_offset < 0 || sub_nb < 0 || sub_offset + sub_nb > c->nb)
+            throw sgipc::arg_error("precond_build: subsystem range outside the system");
+        set_device(c);
+        c->precond_kind = kind;
+        c->sub_offset = sub_offset;
+        c->sub_nb = sub_nb;
+        if (kind == SGIPC_PRECOND_DIAG) sgipc::diag_build(*c);
+        if (kind == SGIPC_PRECOND_MAS || kind == SGIPC_PRECOND_MAS_FP32) sgipc::mas_build(*c, sub_offset, sub_nb, pairs, npairs);
+    });
+}
+
+int sgipc_precond_apply(sgipc_ctx* c, const double* r, double* z) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!r || !z) throw sgipc::arg_error("precond_apply: null vector");
+        set_device(c);
+        sgipc::precond_apply(*c, r, z);
+    });
+}
+
+int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts* o, sgipc_solve_stats* st) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!b || !x) throw sgipc::arg_error("solve: null vector");
+        sgipc::SolveOpts so;
+        if (o) {
+            so.precision = o->precision;
+            so.stop_rule = o->stop_rule;
+            so.tol = o->tol;
+            so.max_iter_ratio = o->max_iter_ratio;
+            so.max_iter = o->max_iter;
+            so.check_every = o->check_every;
+        }
+        if (so.stop_rule != 0 && so.stop_rule != 1) throw sgipc::arg_error("solve: unknown stop rule");
+        if (!(so.tol > 0.0)) throw sgipc::arg_error("solve: tolerance must be positive");
+        set_device(c);
+        sgipc::SolveStats s2;
+        sgipc::pcg_solve(*c, b, x, so, s2);
+        if (st) {
+            st->iterations = s2.iterations;
+            st->converged = s2.converged;
+            st->refinements = s2.refinements;
+            st->rel_residual = s2.rel_residual;
+            st->rz0 = s2.rz0;
+        }
+    });
+}
+
+int sgipc_mas_info(sgipc_ctx* c, int* L, int* total, int* ls) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!c->mas.ready) throw sgipc::state_error("mas_info: MAS not built");
+        if (L) *L = c->mas.L;
+        if (total) *total = c->mas.total;
+        if (ls)
+            for (int i = 0; i < 2 * (c->mas.L + 1); ++i) ls[i] = c->mas.level_size_h[i];
+    });
+}
+
+int sgipc_mas_tables(sgipc_ctx* c, int* cs, int* gn, int* ct, uint32_t* fm, int* p0) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        sgipc::MasState& m = c->mas;
+        if (!m.ready) throw sgipc::state_error("mas_tables: MAS not built");
+        set_device(c);
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+        if (cs) SGIPC_HIP(hipMemcpy(cs, m.coarse_space.p, sizeof(int) * (size_t)m.L * m.n, hipMemcpyDeviceToHost));
+        if (gn) SGIPC_HIP(hipMemcpy(gn, m.going_next.p, sizeof(int) * (size_t)std::max(m.total, m.n), hipMemcpyDeviceToHost));
+        if (ct) SGIPC_HIP(hipMemcpy(ct, m.coarse_table.p, sizeof(int) * 6 * (size_t)m.n, hipMemcpyDeviceToHost));
+        if (fm) SGIPC_HIP(hipMemcpy(fm, m.fine_mask.p, sizeof(uint32_t) * (size_t)m.n, hipMemcpyDeviceToHost));
+        if (p0) SGIPC_HIP(hipMemcpy(p0, m.prefix0.p, sizeof(int) * (size_t)(m.M / m.bank), hipMemcpyDeviceToHost));
+    });
+}
+
+int sgipc_mas_bank_inverse(sgipc_ctx* c, int b, double* inv) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        sgipc::MasState& m = c->mas;
+        if (!m.ready) throw sgipc::state_error("mas_bank_inverse: MAS not built");
+        if (b < 0 || b >= m.nbanks || !inv) throw sgipc::arg_error("mas_bank_inverse: bad bank");
+        set_device(c);
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+        size_t D = 3 * (size_t)m.bank;
+        SGIPC_HIP(hipMemcpy(inv, m.inv64.p + (size_t)b * D * D, sizeof(double) * D * D, hipMemcpyDeviceToHost));
+    });
+}
+
+int sgipc_malloc(sgipc_ctx* c, void** p, size_t bytes) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!p) throw sgipc::arg_error("malloc: null output");
+        set_device(c);
+        SGIPC_HIP(hipMalloc(p, bytes ? bytes : 1));
+    });
+}
+int sgipc_free(sgipc_ctx* c, void* p) {
+    CHECK_CTX(c);
+    return guard(c, [&] { set_device(c); if (p) SGIPC_HIP(hipFree(p)); });
+}
+int sgipc_memcpy_h2d(sgipc_ctx* c, void* d, const void* h, size_t bytes) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        set_device(c);
+        SGIPC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int sgipc_memcpy_d2h(sgipc_ctx* c, void* h, const void* d, size_t bytes) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        set_device(c);
+        SGIPC_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

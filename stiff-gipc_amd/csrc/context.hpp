@@ -1,0 +1,141 @@
+// Library context: one HIP stream, caller-owned triplet store, and all
+// device workspaces of the Newton-step path (FEM assembly, BCOO, BSR, MAS,
+// PCG).  Not thread-safe: one host thread drives a context at a time.
+#pragma once
+#include "common.hpp"
+#include <string>
+#include <vector>
+
+namespace sgipc {
+
+// Block-3x3 full symmetric CSR (both triangles) derived from the upper BCOO;
+// values SoA by component (val[c * nnzb + k], c = r + 3 col).
+struct Bsr {
+    int nb = 0;        // block rows
+    int nnzb = 0;      // stored blocks (2K - diag)
+    DBuf<int> rowptr;  // nb + 1
+    DBuf<int> col;     // nnzb
+    DBuf<double> val;  // 9 * nnzb
+    DBuf<float> valf;  // 9 * nnzb (mixed precision copy)
+    DBuf<int> src;     // nnzb: upper BCOO block index, negative-1 if transposed
+};
+
+struct MasState {
+    bool ready = false;
+    int bank = 16;
+    int n = 0;           // FEM vertices covered
+    int node_offset = 0; // subtracted from contact-pair vertex ids
+    int M = 0;           // slot count (bank * parts)
+    int L = 0;           // levels
+    int total = 0;       // clusters over all levels (multiple of bank)
+    int nbanks = 0;      // total / bank
+    std::vector<int> level_size_h;  // 2*(L+1)
+    DBuf<int> part_map, real_map;
+    DBuf<unsigned> nstart, nnum, nlist;
+    DBuf<unsigned> fine_mask, next_mask;
+    DBuf<int> prefix0, psum0, next_prefix, next_psum;
+    DBuf<int> level_size;  // 2*(L+2)
+    DBuf<int> coarse_space;  // L*n
+    DBuf<int> going_next;    // cap
+    size_t going_cap = 0;
+    DBuf<int> coarse_table;  // 6*n (itable layout)
+    DBuf<int> node_of;       // L*n : global node id of vertex at level l (l=0: slot)
+    DBuf<double> bank_upper; // nbanks * npack * 9 (Galerkin, packed upper, col-major blocks)
+    DBuf<double> inv64;      // nbanks * D*D dense symmetric inverse (FP64)
+    DBuf<float> inv32;       // nbanks * D*D dense symmetric inverse (FP32)
+    DBuf<double> R64, Z64;   // 3*total
+    DBuf<float> R32, Z32;
+    DBuf<int> sub_index;     // BCOO block indices in the subsystem range
+};
+
+struct PcgWork {
+    DBuf<double> r, z, p, Ap, xw;   // 3N each
+    DBuf<double> partials;          // per-workgroup partial dot products
+    DBuf<double> scal;              // device scalars (see pcg.hip)
+    DBuf<int> ctrl;                 // done flag, iteration counters, tickets
+    DBuf<float> rf, zf, pf, Apf;    // mixed-precision inner vectors
+    DBuf<double> Dinv;              // 9 * nb block-Jacobi inverses
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // caller-owned triplet store (GIPCTripletMatrix)
+    double* t_vals = nullptr;
+    int* t_rows = nullptr;
+    int* t_cols = nullptr;
+    long long t_cap = 0;
+
+    // FEM mesh incidence for the deterministic gradient gather
+    int fem_n = 0, fem_T = 0;
+    DBuf<int> inc_start;  // n+1
+    DBuf<int> inc_list;   // 4T entries: 4*t + corner, ascending t per vertex
+    DBuf<double> tet_grad; // 12*T
+
+    // converted system
+    int K = 0;            // unique upper blocks
+    int nb = 0;           // block rows of the system
+    bool have_bcoo = false;
+    DBuf<double> bcoo_val; // 9K scratch before copy-back
+    DBuf<int> bcoo_row, bcoo_col;
+    DBuf<int> row_count, row_start, bucket, uniq_count, uniq_start;
+    DBuf<int> dev_int;    // small device ints (totals)
+    Bsr bsr;
+    bool have_bsr = false;
+
+    // preconditioner
+    int precond_kind = 0;
+    int sub_offset = 0, sub_nb = 0;
+    MasState mas;
+
+    PcgWork pcg;
+    DBuf<int> scan_tmp;
+};
+
+void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s);
+
+// component entry points (implemented in the .hip files)
+void fem_set_mesh(Ctx& c, int n, int T, const uint32_t* h_tets);
+void fem_kinetic_gradient(Ctx& c, int n, const double* x, const double* xt, const double* mass,
+                          double* shape);
+void fem_assemble(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv,
+                  const double* vol, const double* mu, const double* lam, double dt,
+                  const int* btype, int index_offset, long long triplet_offset, double* shape);
+void fem_mass_triplets(Ctx& c, int n, const double* mass, int index_offset,
+                       long long triplet_offset);
+void fem_rhs(Ctx& c, int n, const double* fb, const double* shape, const int* btype, double* b);
+
+int bcoo_convert(Ctx& c, long long nt, int nblockrows);
+void bsr_build(Ctx& c);
+void spmv64(Ctx& c, double a, const double* x, double b, double* y);
+
+void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, const unsigned* nnum,
+                      const unsigned* nlist, int nlist_len, const int* part_map, int M,
+                      const int* real_map, int bank);
+void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs);
+void mas_apply64(Ctx& c, const double* r, double* z);
+void mas_apply32(Ctx& c, const double* r, double* z);
+void diag_build(Ctx& c);
+
+struct SolveOpts {
+    int precision = 0;  // 0 fp64, 1 mixed
+    int stop_rule = 0;  // 0 reference |rz|<=tol rz0, 1 ||r||/||b||<=tol
+    double tol = 1e-4;
+    double max_iter_ratio = 0.3;
+    int max_iter = 0;
+    int check_every = 8;
+};
+struct SolveStats {
+    int iterations = 0;
+    double rel_residual = 0;
+    double rz0 = 0;
+    int converged = 0;
+    int refinements = 0;
+};
+void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStats& st);
+void precond_apply(Ctx& c, const double* r, double* z);
+
+}  // namespace sgipc

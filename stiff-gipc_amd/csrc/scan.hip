@@ -1,0 +1,113 @@
+// Device-wide exclusive prefix sum of int32 counts (three passes: tile sums,
+// one-block scan of the tile sums, tile rescan).  Used by the BCOO convert,
+// the full-BSR build and the MAS level numbering.
+#include "context.hpp"
+
+namespace sgipc {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+__device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
+    // sh: kScanThreads/64 + 1 ints
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) sh[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) {
+            int t = sh[w];
+            sh[w] = acc;
+            acc += t;
+        }
+        sh[kScanThreads / 64] = acc;
+    }
+    __syncthreads();
+    total = sh[kScanThreads / 64];
+    int r = sh[wid] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* __restrict__ in, int n,
+                                                            int* __restrict__ tile_sums) {
+    __shared__ int sh[kScanThreads / 64 + 1];
+    long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += in[base + i];
+    int total;
+    (void)block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// single block; ntiles <= kScanTile
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile_sums(int* __restrict__ tile_sums, int ntiles,
+                                                                 int* __restrict__ total_out) {
+    __shared__ int sh[kScanThreads / 64 + 1];
+    int vals[kScanItems];
+    int base = threadIdx.x * kScanItems, s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        vals[i] = (base + i < ntiles) ? tile_sums[base + i] : 0;
+        s += vals[i];
+    }
+    int total;
+    int run = block_exclusive_scan(s, sh, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < ntiles) tile_sums[base + i] = run;
+        run += vals[i];
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_tile_apply(const int* __restrict__ in, int n,
+                                                             const int* __restrict__ tile_offsets,
+                                                             int* __restrict__ out) {
+    __shared__ int sh[kScanThreads / 64 + 1];
+    long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int vals[kScanItems], s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        vals[i] = (base + i < n) ? in[base + i] : 0;
+        s += vals[i];
+    }
+    int total;
+    int run = block_exclusive_scan(s, sh, total) + tile_offsets[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += vals[i];
+    }
+    // out has n+1 entries: out[n] = grand total
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
+}
+
+// out[0..n] (n+1 entries), total_dev optional.  in and out may alias.
+void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s) {
+    if (n <= 0) {
+        fill<int>(out, 0, 1, s);
+        if (total_dev) fill<int>(total_dev, 0, 1, s);
+        return;
+    }
+    int ntiles = div_up(n, kScanTile);
+    if (ntiles > kScanTile) throw arg_error("exclusive_scan: input too large");
+    int* tiles = c.scan_tmp.reserve((size_t)ntiles + 1);
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total_dev);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles, out);
+    SGIPC_LAUNCH_CHECK();
+}
+
+}  // namespace sgipc

@@ -1,0 +1,91 @@
+"""ctypes binding of libsgipc.so (the C ABI declared in include/sgipc.h).
+
+Device memory is passed as raw pointers (torch tensors' data_ptr()).  The
+library must exist: nothing here falls back to a CPU implementation.
+"""
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libsgipc.so")
+
+OK = 0
+ERR = {1: "invalid argument", 2: "HIP error", 3: "capacity exceeded", 4: "invalid state", 5: "internal error"}
+PRECOND_NONE, PRECOND_DIAG, PRECOND_MAS, PRECOND_MAS_FP32 = 0, 1, 2, 3
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+STOP_REF_RZ, STOP_TRUE_REL = 0, 1
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("stop_rule", C.c_int), ("tol", C.c_double),
+                ("max_iter_ratio", C.c_double), ("max_iter", C.c_int), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("refinements", C.c_int),
+                ("rel_residual", C.c_double), ("rz0", C.c_double)]
+
+
+class SgipcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sgipc error %d (%s): %s" % (code, ERR.get(code, "?"), msg))
+        self.code = code
+
+
+_lib = None
+
+# exported symbols and their signatures (must match include/sgipc.h)
+P, I, LL, D, SZ = C.c_void_p, C.c_int, C.c_longlong, C.c_double, C.c_size_t
+SIGNATURES = {
+    "sgipc_version": ([], I),
+    "sgipc_ctx_create": ([I, C.POINTER(P)], I),
+    "sgipc_ctx_destroy": ([P], I),
+    "sgipc_ctx_set_stream": ([P, P], I),
+    "sgipc_ctx_synchronize": ([P], I),
+    "sgipc_last_error": ([P], C.c_char_p),
+    "sgipc_build_vertex_neighbors": ([I, I, P, P, P, P, I, C.POINTER(I)], I),
+    "sgipc_build_bank_map": ([I, P, I, P, I, C.POINTER(I), P], I),
+    "sgipc_mas_num_levels": ([I, I], I),
+    "sgipc_set_fem_mesh": ([P, I, I, P], I),
+    "sgipc_set_mas_topology": ([P, I, I, P, P, P, I, P, I, P, I], I),
+    "sgipc_triplets_view": ([P, P, P, P, LL], I),
+    "sgipc_kinetic_gradient": ([P, I, P, P, P, P], I),
+    "sgipc_assemble_fem": ([P, I, P, P, P, P, P, P, D, P, I, LL, P], I),
+    "sgipc_mass_triplets": ([P, I, P, I, LL], I),
+    "sgipc_fem_rhs": ([P, I, P, P, P, P], I),
+    "sgipc_convert": ([P, LL, I, C.POINTER(I)], I),
+    "sgipc_spmv": ([P, D, P, D, P], I),
+    "sgipc_precond_build": ([P, I, I, I, P, I], I),
+    "sgipc_precond_apply": ([P, P, P], I),
+    "sgipc_solve": ([P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)], I),
+    "sgipc_mas_info": ([P, C.POINTER(I), C.POINTER(I), P], I),
+    "sgipc_mas_tables": ([P, P, P, P, P, P], I),
+    "sgipc_mas_bank_inverse": ([P, I, P], I),
+    "sgipc_malloc": ([P, C.POINTER(P), SZ], I),
+    "sgipc_free": ([P, P], I),
+    "sgipc_memcpy_h2d": ([P, P, P, SZ], I),
+    "sgipc_memcpy_d2h": ([P, P, P, SZ], I),
+}
+
+
+def lib():
+    """Load libsgipc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsgipc.so not found at %s: run __graft_entry__.build() or "
+                               "make -C stiff-gipc_amd" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc != OK:
+        msg = lib().sgipc_last_error(ctx)
+        raise SgipcError(rc, msg.decode() if msg else "")
+    return rc

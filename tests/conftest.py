@@ -1,0 +1,56 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (os.path.join(ROOT, "stiff-gipc_amd"), os.path.join(ROOT, "oracle"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+    O.build()
+    return O
+
+
+@pytest.fixture(scope="session")
+def cube10():
+    from sgipc import scene
+    return scene.cube_scene(10, seed=1)
+
+
+@pytest.fixture(scope="session")
+def cube4():
+    from sgipc import scene
+    return scene.cube_scene(4, seed=7)
+
+
+def oracle_system(O, sc):
+    """Assemble + convert one Newton-step system with the oracle."""
+    shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)
+    grad, vals, rows, cols = O.fem_assemble(sc.x, sc.tets, sc.DmInv, sc.vol, sc.mu, sc.lam, sc.dt, sc.btype, 0,
+                                            shape.copy())
+    mv, mr, mc = O.mass_triplets(sc.mass)
+    R = np.concatenate([rows, mr]); Cc = np.concatenate([cols, mc]); V = np.concatenate([vals, mv])
+    b = O.fem_rhs(np.zeros_like(grad), grad, sc.btype)
+    kr, kc, kv = O.convert(R, Cc, V)
+    return dict(shape=grad, trip=(R, Cc, V), b=b.reshape(-1), bcoo=(kr, kc, kv))
+
+
+def dense_from_bcoo(kr, kc, kv, n):
+    A = np.zeros((3 * n, 3 * n))
+    for k in range(len(kr)):
+        M = kv[k].reshape(3, 3, order="F")
+        i, j = kr[k], kc[k]
+        A[3 * i:3 * i + 3, 3 * j:3 * j + 3] += M
+        if i != j:
+            A[3 * j:3 * j + 3, 3 * i:3 * i + 3] += M.T
+    return A

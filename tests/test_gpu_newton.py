@@ -1,0 +1,153 @@
+"""GPU parity of the HIP Newton-step path against the CPU oracle, through the
+C ABI (libsgipc.so).  Inputs are the seeded synthetic cube scenes."""
+import numpy as np
+import pytest
+
+from conftest import oracle_system, dense_from_bcoo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def make_system(sc, precond=None):
+    from sgipc import capi
+    from sgipc.system import NewtonSystem
+    return NewtonSystem(sc, "cuda:0", precond=capi.PRECOND_MAS if precond is None else precond)
+
+
+@pytest.mark.parametrize("name", ["cube4", "cube10"])
+def test_fem_assembly_matches_oracle(torch_cuda, oracle, request, name):
+    sc = request.getfixturevalue(name)
+    print("case: fem assembly", name, "n", sc.n, "T", sc.T, flush=True)
+    ref = oracle_system(oracle, sc)
+    sys_ = make_system(sc)
+    sys_.assemble()
+    R, Cc, V = sys_.triplets()
+    Ro, Co, Vo = ref["trip"]
+    assert np.array_equal(R, Ro) and np.array_equal(Cc, Co)          # bit-exact indices
+    scale = np.abs(Vo).max()
+    assert np.abs(V - Vo).max() <= 1e-12 * scale                      # FP64 values
+    g = sys_.shape.cpu().numpy().reshape(-1, 3)
+    assert np.abs(g - ref["shape"]).max() <= 1e-12 * np.abs(ref["shape"]).max()
+    b = sys_.b.cpu().numpy()
+    assert np.abs(b - ref["b"]).max() <= 1e-12 * np.abs(ref["b"]).max()
+
+
+@pytest.mark.parametrize("name", ["cube4", "cube10"])
+def test_convert_matches_oracle(torch_cuda, oracle, request, name):
+    sc = request.getfixturevalue(name)
+    print("case: convert", name, flush=True)
+    ref = oracle_system(oracle, sc)
+    sys_ = make_system(sc)
+    sys_.assemble()
+    K = sys_.convert()
+    kr, kc, kv = sys_.bcoo()
+    orow, ocol, oval = ref["bcoo"]
+    assert K == len(orow)
+    assert np.array_equal(kr, orow) and np.array_equal(kc, ocol)     # bit-exact BCOO index set and order
+    assert np.abs(kv - oval).max() <= 1e-12 * np.abs(oval).max()
+
+
+def test_convert_on_oracle_triplets_is_bitexact(torch_cuda, oracle, cube4):
+    """Fed the oracle's own triplets, the GPU convert reproduces the oracle's
+    BCOO bit for bit (same per-block summation order)."""
+    import torch
+    print("case: convert bit-exact on oracle triplets", flush=True)
+    ref = oracle_system(oracle, cube4)
+    sys_ = make_system(cube4)
+    R, Cc, V = ref["trip"]
+    nt = len(R)
+    sys_.tr[:nt] = torch.tensor(R, dtype=torch.int32, device=sys_.dev)
+    sys_.tc[:nt] = torch.tensor(Cc, dtype=torch.int32, device=sys_.dev)
+    sys_.tv[:9 * nt] = torch.tensor(V.reshape(-1), dtype=torch.float64, device=sys_.dev)
+    sys_.nt = nt
+    sys_.convert()
+    kr, kc, kv = sys_.bcoo()
+    orow, ocol, oval = ref["bcoo"]
+    assert np.array_equal(kr, orow) and np.array_equal(kc, ocol)
+    assert np.array_equal(kv, oval)
+
+
+def test_spmv_matches_dense(torch_cuda, oracle, cube10):
+    print("case: spmv cube10", flush=True)
+    sys_ = make_system(cube10)
+    sys_.assemble(); sys_.convert()
+    kr, kc, kv = sys_.bcoo()
+    A = dense_from_bcoo(kr, kc, kv, cube10.n)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=3 * cube10.n); y0 = rng.normal(size=3 * cube10.n)
+    y = sys_.spmv(x, a=0.7, b=-0.3, y=y0)
+    ref = 0.7 * A @ x - 0.3 * y0
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_mas_tables_bitexact(torch_cuda, oracle, cube10):
+    print("case: mas tables cube10", flush=True)
+    ref = oracle_system(oracle, cube10)
+    sys_ = make_system(cube10)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    t = sys_.mas_tables()
+    kr, kc, kv = ref["bcoo"]
+    mas = oracle.Mas(cube10.n, sys_.part_map, sys_.real_map, sys_.ns, sys_.nn, sys_.nl, cube10.bank)
+    tot = mas.build(kr, kc, kv)
+    to = mas.tables()
+    assert t["levelnum"] == mas.levelnum and t["total"] == tot
+    assert np.array_equal(t["level_size"], to["level_size"])
+    assert np.array_equal(t["coarse_space"], to["coarse_space"])
+    assert np.array_equal(t["coarse_table"][:, :mas.levelnum - 1], to["coarse_table"][:, :mas.levelnum - 1])
+    assert np.array_equal(t["fine_mask"], to["fine_mask"])
+    assert np.array_equal(t["prefix0"], to["prefix0"])
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_mas_apply_matches_oracle(torch_cuda, oracle, cube10, fp32):
+    from sgipc import capi
+    print("case: mas apply fp32=%s" % fp32, flush=True)
+    ref = oracle_system(oracle, cube10)
+    sys_ = make_system(cube10, capi.PRECOND_MAS_FP32 if fp32 else capi.PRECOND_MAS)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    kr, kc, kv = ref["bcoo"]
+    mas = oracle.Mas(cube10.n, sys_.part_map, sys_.real_map, sys_.ns, sys_.nn, sys_.nl, cube10.bank)
+    mas.build(kr, kc, kv)
+    r = np.random.default_rng(5).normal(size=3 * cube10.n)
+    z = sys_.precond_apply(r)
+    zo = mas.apply(r, fp32=False)
+    tol = 1e-5 if fp32 else 1e-11
+    assert np.abs(z - zo).max() <= tol * np.abs(zo).max()
+    b0 = sys_.bank_inverse(0)
+    _, Ainv = mas.bank_matrix(0)
+    assert np.abs(b0 - Ainv).max() <= 1e-10 * np.abs(Ainv).max()
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_pcg_matches_oracle(torch_cuda, oracle, cube10, precond):
+    print("case: pcg precond=%d" % precond, flush=True)
+    ref = oracle_system(oracle, cube10)
+    sys_ = make_system(cube10, precond)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    kr, kc, kv = ref["bcoo"]
+    mas = None
+    if precond == 2:
+        mas = oracle.Mas(cube10.n, sys_.part_map, sys_.real_map, sys_.ns, sys_.nn, sys_.nl, cube10.bank)
+        mas.build(kr, kc, kv)
+    # reference stopping rule: identical iteration count, direction within 1e-10
+    st = sys_.solve(tol=1e-4, stop_rule=0)
+    xo, ito, _ = oracle.pcg(kr, kc, kv, cube10.n, ref["b"], precond=precond, mas=mas, tol=1e-4, rule=0)
+    x = sys_.dx.cpu().numpy()
+    assert st.iterations == ito
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    # true relative residual 1e-6 against a direct solve
+    st = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000)
+    x = sys_.dx.cpu().numpy()
+    A = dense_from_bcoo(kr, kc, kv, cube10.n)
+    xs = np.linalg.solve(A, ref["b"])
+    assert st.converged and st.rel_residual <= 1.5e-6
+    assert np.linalg.norm(A @ x - ref["b"]) <= 1.5e-6 * np.linalg.norm(ref["b"])
+    assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(xs)
