@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Newton-step benchmark on one MI355X.
+
+Workload (the contact configuration): two soft boxes of 40^3 cells (5 tets per
+cell, 68,921 vertices each) stacked with a gap of 0.5 dhat, E=1e6, nu=0.4,
+kappa=1e4, dt=0.01, seeded velocities in [-0.5, 0.5].  One step = one Newton
+iteration's hot path on a fixed iterate: kinetic + FEM (+ barrier) gradient
+and Hessian triplets, mass diagonal, triplet -> BCOO, MAS build, MAS-PCG to a
+1e-6 true relative residual.  Inputs are resident on the GPU before timing.
+
+Prints one JSON line (metric: Newton solves/s) with the roofline of the
+dominant kernel (the PCG SpMV) measured live with HIP events, and the CPU
+baseline (the oracle's FP64 restatement of the same step, one thread).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "stiff-gipc_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=40)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def spmv_alg_bytes(K, nb):
+    # upper BCOO: 72 B values + 8 B (row, col) per block; x gathered once
+    # (24 B per block row) and y written once (24 B per block row)
+    return 80.0 * K + 48.0 * nb
+
+
+def cpu_baseline(sc, pairs, mat_index, counts):
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.perf_counter()
+    if pairs is not None:
+        fb, bv, br, bc = O.barrier_assemble(pairs, mat_index, counts, sc.x, sc.x_rest, sc.dhat, sc.kappa, sc.btype)
+    else:
+        fb = np.zeros(3 * sc.n); bv = np.zeros((0, 9)); br = np.zeros(0, np.int32); bc = np.zeros(0, np.int32)
+    shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)
+    grad, vals, rows, cols = O.fem_assemble(sc.x, sc.tets, sc.DmInv, sc.vol, sc.mu, sc.lam, sc.dt, sc.btype, 0,
+                                            shape.copy())
+    mv, mr, mc = O.mass_triplets(sc.mass)
+    kr, kc, kv = O.convert(np.concatenate([br, rows, mr]), np.concatenate([bc, cols, mc]),
+                           np.concatenate([bv, vals, mv]))
+    b = O.fem_rhs(fb.reshape(-1, 3), grad, sc.btype).reshape(-1)
+    t1 = time.perf_counter()
+    ns, nn, nl = O.vertex_neighbors(sc.n, sc.tets)
+    pm, rm = O.bank_map(sc.part, sc.bank)
+    mas = O.Mas(sc.n, pm, rm, ns, nn, nl, sc.bank)
+    t2 = time.perf_counter()
+    mas.build(kr, kc, kv, pairs=pairs, fp32_store=False)
+    x, it, rr = O.pcg(kr, kc, kv, sc.n, b, precond=2, mas=mas, tol=1e-6, rule=1, max_iter=100000)
+    t3 = time.perf_counter()
+    secs = (t1 - t0) + (t3 - t2)  # topology setup is one-time, excluded as on the GPU
+    return secs, it
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (run with --gpus 1)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from sgipc import scene, capi
+    from sgipc.system import NewtonSystem
+
+    sc = scene.two_box_scene(a.cells)
+    pt, ee = scene.facing_contact_candidates(sc)
+    sysm = NewtonSystem(sc, "cuda:0", precond=capi.PRECOND_MAS)
+    sysm.setup_contacts(pt, ee)  # narrow phase once per Newton iterate, outside the timed hot path
+    prec = capi.PRECISION_FP64 if a.precision == "fp64" else capi.PRECISION_MIXED
+    solve_kw = dict(tol=1e-6, stop_rule=capi.STOP_TRUE_REL, precision=prec, max_iter=100000)
+
+    for _ in range(a.warmup):
+        sysm.newton_step(**solve_kw)
+    torch.cuda.synchronize()
+
+    iters = 0
+    pcg_s = 0.0
+    rel = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sysm.assemble()
+        sysm.convert()
+        sysm.build_precond()
+        ts = time.perf_counter()
+        st = sysm.solve(**solve_kw)
+        pcg_s += time.perf_counter() - ts
+        iters += st.iterations
+        rel.append(st.rel_residual)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = a.steps / secs
+
+    # live roofline of the dominant kernel (PCG SpMV), HIP events on the
+    # library stream over two profiled steps
+    import ctypes as C
+    ctx = sysm.ctx
+    ctx.call("sgipc_profile_enable", 1)
+    for slot in range(7):
+        ctx.call("sgipc_profile_read", slot, None, None, 1)
+    for _ in range(2):
+        sysm.newton_step(**solve_kw)
+    calls = C.c_longlong(0); ms = C.c_double(0.0)
+    ctx.call("sgipc_profile_read", capi.PROF_SPMV, C.byref(calls), C.byref(ms), 0)
+    phase = {}
+    for name, slot in (("fem_kernel", capi.PROF_FEM), ("convert", capi.PROF_CONVERT),
+                       ("precond_build", capi.PROF_PRECOND_BUILD), ("pcg", capi.PROF_SOLVE),
+                       ("mas_bank_solve_kernel", capi.PROF_MAS_SOLVE)):
+        c2 = C.c_longlong(0); m2 = C.c_double(0.0)
+        ctx.call("sgipc_profile_read", slot, C.byref(c2), C.byref(m2), 0)
+        phase[name] = round(m2.value / max(c2.value, 1), 4)
+    ctx.call("sgipc_profile_enable", 0)
+    spmv_ms = ms.value / max(calls.value, 1)
+    alg = spmv_alg_bytes(sysm.K, sysm.n)
+    achieved = alg / (spmv_ms * 1e-3) / 1e9
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_traffic.json")
+    if os.path.exists(tf):
+        try:
+            with open(tf) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        csecs, cit = cpu_baseline(sc, sysm.pairs.cpu().numpy().reshape(-1, 4) if sysm.n_pairs else None,
+                                  sysm.mat_index.cpu().numpy() if sysm.n_pairs else None, sysm.counts)
+        cpu = {"value": round(1.0 / csecs, 5), "unit": "Newton solves/s", "cores": 1, "kind": "port",
+               "sample": "one full Newton step of the same workload (barrier + FEM assembly, convert, MAS build, "
+                         "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, "
+                         "1 thread, %.1f s" % (cit, csecs)}
+
+    out = {
+        "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 rel. residual)",
+        "value": round(value, 4),
+        "unit": "Newton solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * secs / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32/f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "two stacked soft boxes, 2 x %d^3 cells, 5 tets/cell, %d vertices, %d tets, gap 0.5 dhat"
+                        % (a.cells, sc.n, sc.T),
+            "contact_pairs": int(sysm.n_pairs),
+            "precision": a.precision,
+            "preconditioner": "MAS, 16-vertex banks, %d levels" % sysm.mas_levels(),
+            "stop": "true relative residual 1e-6",
+            "parallelism": "single GPU",
+        },
+        "pcg_iters_per_step": iters / a.steps,
+        "pcg_iters_per_s": round(iters / pcg_s, 2),
+        "rel_residual_max": max(rel),
+        "phase_ms": phase,
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_pap (PCG SpMV)", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "avg_ms": round(spmv_ms, 5), "alg_bytes": alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

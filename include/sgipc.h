@@ -119,6 +119,29 @@ int sgipc_mass_triplets(sgipc_ctx* ctx, int n, const double* d_mass, int index_o
 int sgipc_fem_rhs(sgipc_ctx* ctx, int n, const double* d_fb, const double* d_shape_grad,
                   const int* d_btype, double* d_b);
 
+/* Narrow phase: classify candidate point-triangle (p, t0, t1, t2) and
+ * edge-edge (a0, a1, b0, b1) vertex quadruples into the reference's int4 pair
+ * encoding, keeping pairs with squared distance < dhat; PT candidates first,
+ * each in candidate order; d_mat_index = rank within the size class;
+ * counts[3] = {6x6, 9x9, 12x12} pair counts (host).  Replaces
+ * _checkPTintersection / _checkEEintersection with _dType_PT / _dType_EE
+ * (StiffGIPC/mlbvh.cu:288-1060); the broad phase (LBVH) is not part of this
+ * library.  For near-parallel edge pairs degenerating to PP/PE the unused
+ * fourth entry carries -(candidate index)-2. */
+int sgipc_classify_contacts(sgipc_ctx* ctx, int n_pt, const int* d_pt, int n_ee, const int* d_ee,
+                            const double* d_x, const double* d_xrest, double dhat, int* d_pairs,
+                            int* d_mat_index, int capacity, int* n_pairs, int* counts);
+/* IPC barrier gradient (+= into d_fb, FP64 atomics) and Gauss-Newton Hessian
+ * triplets of all pairs at triplet_offset (12x12 pairs first, then 9x9, then
+ * 6x6, each at its mat_index slot), lower blocks stored transposed-as-zero and
+ * pinned rows zeroed; _calBarrierGradientAndHessian (StiffGIPC/GIPC.cu:
+ * 3106-4943) + the fem-fem contact remap (StiffGIPC/GIPC.cu:10183-10214).
+ * dhat is the squared barrier distance (as GIPC::dHat); counts on the host. */
+int sgipc_assemble_barrier(sgipc_ctx* ctx, int n_pairs, const int* d_pairs, const int* d_mat_index,
+                           const int* counts, const double* d_x, const double* d_xrest, double dhat,
+                           double kappa, const int* d_btype, int index_offset, long long triplet_offset,
+                           double* d_fb);
+
 /* ---- linear system -------------------------------------------------------- */
 /* Triplets [0, n_triplets) -> unique upper BCOO sorted by (row, col), written
  * back in place at offset 0; Converter::convert
@@ -140,6 +163,20 @@ int sgipc_precond_apply(sgipc_ctx* ctx, const double* d_r, double* d_z);
 /* PCG solve of A x = b; PCGSolver::solve (linear_system/solver/pcg_solver.cu:158-267) */
 int sgipc_solve(sgipc_ctx* ctx, const double* d_b, double* d_x, const sgipc_solve_opts* opts,
                 sgipc_solve_stats* stats);
+
+/* ---- measurement ----------------------------------------------------------- */
+/* HIP-event timing of selected launches on the context stream (no reference
+ * counterpart; the reference times phases with cudaEvents in GIPC::solve_subIP,
+ * StiffGIPC/GIPC.cu:10854-10964).  Slots: */
+#define SGIPC_PROF_SPMV 0          /* PCG SpMV kernel (one launch per iteration) */
+#define SGIPC_PROF_MAS_SOLVE 1     /* MAS bank solve kernel (one per apply) */
+#define SGIPC_PROF_FEM 2           /* per-tet SNK1 kernel */
+#define SGIPC_PROF_CONVERT 3       /* whole triplet -> BCOO convert */
+#define SGIPC_PROF_PRECOND_BUILD 4 /* whole preconditioner build */
+#define SGIPC_PROF_SOLVE 5         /* whole PCG solve */
+#define SGIPC_PROF_BARRIER 6       /* barrier assembly kernel */
+int sgipc_profile_enable(sgipc_ctx* ctx, int on);
+int sgipc_profile_read(sgipc_ctx* ctx, int slot, long long* calls, double* total_ms, int reset);
 
 /* ---- introspection (parity tests) ----------------------------------------- */
 int sgipc_mas_info(sgipc_ctx* ctx, int* levelnum, int* total_clusters, int* h_level_size);

@@ -66,6 +66,15 @@ def lib():
         L.or_pcg.argtypes = [C.c_int, i32p, i32p, f64p, C.c_int, f64p, f64p, C.c_int, C.c_void_p, C.c_int,
                              C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.or_pcg.restype = C.c_int
+        L.or_barrier_pair.argtypes = [f64p, f64p, i32p, C.c_double, C.c_double, i32p, C.POINTER(C.c_int), f64p, f64p,
+                                      C.POINTER(C.c_double)]
+        L.or_barrier_pair.restype = C.c_int
+        L.or_barrier_assemble.argtypes = [C.c_int, i32p, i32p, i32p, f64p, f64p, C.c_double, C.c_double, C.c_void_p,
+                                          f64p, f64p, i32p, i32p]
+        L.or_barrier_energy.argtypes = [C.c_int, i32p, f64p, f64p, C.c_double, C.c_double]
+        L.or_barrier_energy.restype = C.c_double
+        L.or_classify.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double, i32p, i32p, i32p]
+        L.or_classify.restype = C.c_int
         _lib = L
     return _lib
 
@@ -241,3 +250,41 @@ def pcg(rows, cols, vals, nb, b, precond=2, mas=None, mas_fp32=False, tol=1e-4, 
                       nb, _c(b, np.float64).reshape(-1), x, precond, None if mas is None else mas.h,
                       int(mas_fp32), tol, rule, max_iter, C.byref(rr))
     return x, it, rr.value
+
+
+def barrier_pair(x, xrest, q, dhat, kappa):
+    vid = np.zeros(4, np.int32); cls = C.c_int(0); g = np.zeros(12); H = np.zeros(144); e = C.c_double(0)
+    nv = lib().or_barrier_pair(_c(x, np.float64).reshape(-1), _c(xrest, np.float64).reshape(-1),
+                               _c(q, np.int32).reshape(4), dhat, kappa, vid, C.byref(cls), g, H, C.byref(e))
+    nd = 3 * nv
+    return vid[:nv].copy(), cls.value, g[:nd].copy(), H[:nd * nd].reshape(nd, nd).copy(), e.value
+
+
+def barrier_assemble(pairs, mat_index, counts, x, xrest, dhat, kappa, btype=None, fb=None):
+    pairs = _c(pairs, np.int32).reshape(-1, 4)
+    np_ = pairs.shape[0]
+    counts = _c(counts, np.int32)
+    nt = 16 * int(counts[2]) + 9 * int(counts[1]) + 4 * int(counts[0])
+    fb = np.zeros(x.size) if fb is None else fb.reshape(-1).copy()
+    vals = np.zeros((max(nt, 1), 9)); rows = np.zeros(max(nt, 1), np.int32); cols = np.zeros(max(nt, 1), np.int32)
+    bt = None if btype is None else _c(btype, np.int32)
+    lib().or_barrier_assemble(np_, pairs.reshape(-1), _c(mat_index, np.int32), counts,
+                              _c(x, np.float64).reshape(-1), _c(xrest, np.float64).reshape(-1), dhat, kappa,
+                              None if bt is None else bt.ctypes.data, fb, vals.reshape(-1), rows, cols)
+    return fb, vals[:nt], rows[:nt], cols[:nt]
+
+
+def barrier_energy(pairs, x, xrest, dhat, kappa):
+    pairs = _c(pairs, np.int32).reshape(-1)
+    return lib().or_barrier_energy(pairs.shape[0] // 4, pairs, _c(x, np.float64).reshape(-1),
+                                   _c(xrest, np.float64).reshape(-1), dhat, kappa)
+
+
+def classify(pt, ee, x, xrest, dhat):
+    pt = _c(pt, np.int32).reshape(-1, 4); ee = _c(ee, np.int32).reshape(-1, 4)
+    m = pt.shape[0] + ee.shape[0]
+    pairs = np.zeros((max(m, 1), 4), np.int32); mi = np.zeros(max(m, 1), np.int32); counts = np.zeros(3, np.int32)
+    npair = lib().or_classify(pt.shape[0], pt.reshape(-1), ee.shape[0], ee.reshape(-1),
+                              _c(x, np.float64).reshape(-1), _c(xrest, np.float64).reshape(-1), dhat,
+                              pairs.reshape(-1), mi, counts)
+    return pairs[:npair].copy(), mi[:npair].copy(), counts

@@ -121,6 +121,34 @@ int or_pcg(int K, const int* rows, const int* cols, const double* vals, int nb,
            const double* b, double* x, int precond, or_mas* mas, int mas_fp32,
            double tol, int rule, int max_iter, double* rel_res_out);
 
+/* ---- IPC barrier (S/GIPC.cu:3106-4943, energy :701-957) ------------------
+ * Pair p is an int4 in the reference sign encoding.  dhat is the squared
+ * barrier distance, kappa the stiffness.  Per pair: gradient (12 doubles,
+ * vertex order of the decoded pair) and dense Hessian (row-major nv3 x nv3),
+ * returns the number of vertices (2, 3 or 4), their ids in vid, the size
+ * class (2: 6x6, 3: 9x9, 4: 12x12) in *cls, and 0 gradient/Hessian when a
+ * mollified pair has zero edge cross product. */
+int or_barrier_pair(const double* x, const double* xrest, const int q[4], double dhat, double kappa,
+                    int vid[4], int* cls, double grad[12], double H[144], double* energy);
+/* Assemble all pairs: fb += gradient, triplets at offset: class-4 pairs at
+ * 16*matIndex, class-3 at 16*c4 + 9*matIndex, class-2 at 16*c4 + 9*c3 +
+ * 4*matIndex (write_triplet, S/GIPC.cu:80-137), then the fem-fem contact
+ * remap (row<=col kept, zeroed on pinned vertices; lower blocks swapped and
+ * zeroed, S/GIPC.cu:10183-10214). counts[3] = {c2, c3, c4}. */
+void or_barrier_assemble(int npairs, const int* pairs, const int* mat_index, const int counts[3],
+                         const double* x, const double* xrest, double dhat, double kappa,
+                         const int* btype, double* fb, double* vals, int* rows, int* cols);
+/* Total barrier energy (sum over pairs, S/GIPC.cu:701-957). */
+double or_barrier_energy(int npairs, const int* pairs, const double* x, const double* xrest,
+                         double dhat, double kappa);
+/* Narrow-phase classification and int4 encoding (S/mlbvh.cu:288-1060).
+ * pt candidates (p, t0, t1, t2) and ee candidates (a0, a1, b0, b1); output
+ * pairs of candidates with d < dhat, PT candidates first, each in candidate
+ * order; mat_index = rank within the size class; counts = {c2, c3, c4}.
+ * Returns the number of pairs. */
+int or_classify(int npt, const int* pt, int nee, const int* ee, const double* x, const double* xrest,
+                double dhat, int* pairs, int* mat_index, int counts[3]);
+
 #ifdef __cplusplus
 }
 #endif

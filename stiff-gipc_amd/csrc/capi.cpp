@@ -83,6 +83,7 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
 int sgipc_ctx_destroy(sgipc_ctx* c) {
     if (!c) return SGIPC_OK;
     (void)hipSetDevice(c->device);
+    sgipc::prof_release(*c);
     if (c->own_stream) {
         (void)hipStreamSynchronize(c->own_stream);
         (void)hipStreamDestroy(c->own_stream);
@@ -196,11 +197,42 @@ int sgipc_fem_rhs(sgipc_ctx* c, int n, const double* fb, const double* sh, const
     });
 }
 
+int sgipc_classify_contacts(sgipc_ctx* c, int npt, const int* pt, int nee, const int* ee, const double* x,
+                            const double* xr, double dhat, int* pairs, int* mi, int capacity, int* n_pairs,
+                            int* counts) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (npt < 0 || nee < 0 || (npt > 0 && !pt) || (nee > 0 && !ee) || !x || !xr || !pairs || !mi || !n_pairs ||
+            !counts)
+            throw sgipc::arg_error("classify_contacts: invalid argument");
+        if (!(dhat > 0.0)) throw sgipc::arg_error("classify_contacts: dhat must be positive");
+        set_device(c);
+        *n_pairs = sgipc::classify_contacts(*c, npt, pt, nee, ee, x, xr, dhat, pairs, mi, capacity, counts);
+    });
+}
+
+int sgipc_assemble_barrier(sgipc_ctx* c, int np, const int* pairs, const int* mi, const int* counts, const double* x,
+                           const double* xr, double dhat, double kappa, const int* btype, int index_offset,
+                           long long triplet_offset, double* fb) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (np < 0 || !counts || (np > 0 && (!pairs || !mi || !x || !xr || !fb)))
+            throw sgipc::arg_error("assemble_barrier: invalid argument");
+        if (counts[0] < 0 || counts[1] < 0 || counts[2] < 0 || counts[0] + counts[1] + counts[2] != np)
+            throw sgipc::arg_error("assemble_barrier: class counts do not add up to the pair count");
+        if (!(dhat > 0.0)) throw sgipc::arg_error("assemble_barrier: dhat must be positive");
+        set_device(c);
+        sgipc::barrier_assemble(*c, np, pairs, mi, counts, x, xr, dhat, kappa, btype, index_offset, triplet_offset, fb);
+    });
+}
+
 int sgipc_convert(sgipc_ctx* c, long long nt, int nbr, int* n_unique) {
     CHECK_CTX(c);
     return guard(c, [&] {
         set_device(c);
+        sgipc::prof_begin(*c, sgipc::PROF_CONVERT);
         int K = sgipc::bcoo_convert(*c, nt, nbr);
+        sgipc::prof_end(*c, sgipc::PROF_CONVERT);
         if (n_unique) *n_unique = K;
     });
 }
@@ -227,8 +259,10 @@ int sgipc_precond_build(sgipc_ctx* c, int kind, int sub_offset, int sub_nb, cons
         c->precond_kind = kind;
         c->sub_offset = sub_offset;
         c->sub_nb = sub_nb;
+        sgipc::prof_begin(*c, sgipc::PROF_PRECOND_BUILD);
         if (kind == SGIPC_PRECOND_DIAG) sgipc::diag_build(*c);
         if (kind == SGIPC_PRECOND_MAS || kind == SGIPC_PRECOND_MAS_FP32) sgipc::mas_build(*c, sub_offset, sub_nb, pairs, npairs);
+        sgipc::prof_end(*c, sgipc::PROF_PRECOND_BUILD);
     });
 }
 
@@ -258,7 +292,10 @@ int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts
         if (!(so.tol > 0.0)) throw sgipc::arg_error("solve: tolerance must be positive");
         set_device(c);
         sgipc::SolveStats s2;
+        sgipc::prof_begin(*c, sgipc::PROF_SOLVE);
         sgipc::pcg_solve(*c, b, x, so, s2);
+        sgipc::prof_end(*c, sgipc::PROF_SOLVE);
+        sgipc::prof_flush(*c);
         if (st) {
             st->iterations = s2.iterations;
             st->converged = s2.converged;
@@ -266,6 +303,27 @@ int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts
             st->rel_residual = s2.rel_residual;
             st->rz0 = s2.rz0;
         }
+    });
+}
+
+int sgipc_profile_enable(sgipc_ctx* c, int on) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        set_device(c);
+        sgipc::prof_flush(*c);
+        c->prof.on = on != 0;
+    });
+}
+
+int sgipc_profile_read(sgipc_ctx* c, int slot, long long* calls, double* total_ms, int reset) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (slot < 0 || slot >= sgipc::PROF_NSLOTS) throw sgipc::arg_error("profile_read: bad slot");
+        set_device(c);
+        sgipc::prof_flush(*c);
+        if (calls) *calls = c->prof.calls[slot];
+        if (total_ms) *total_ms = c->prof.ms[slot];
+        if (reset) { c->prof.calls[slot] = 0; c->prof.ms[slot] = 0.0; }
     });
 }
 

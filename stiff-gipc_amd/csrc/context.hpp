@@ -57,8 +57,21 @@ struct PcgWork {
     DBuf<double> Dinv;              // 9 * nb block-Jacobi inverses
 };
 
+// Optional HIP-event timing of selected launches (enabled by the caller for
+// measurement runs; off by default).
+enum ProfSlot { PROF_SPMV = 0, PROF_MAS_SOLVE, PROF_FEM, PROF_CONVERT, PROF_PRECOND_BUILD, PROF_SOLVE, PROF_BARRIER,
+                PROF_NSLOTS };
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[PROF_NSLOTS];
+    double ms[PROF_NSLOTS] = {0};
+    long long calls[PROF_NSLOTS] = {0};
+};
+
 struct Ctx {
     int device = 0;
+    Prof prof;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string last_error;
@@ -93,7 +106,14 @@ struct Ctx {
 
     PcgWork pcg;
     DBuf<int> scan_tmp;
+    // contact classification workspaces
+    DBuf<int> cls_buf, cls_flag, cls_scan, cls_pos;
 };
+
+void prof_begin(Ctx& c, int slot);
+void prof_end(Ctx& c, int slot);
+void prof_flush(Ctx& c);
+void prof_release(Ctx& c);
 
 void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s);
 
@@ -119,6 +139,12 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
 void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
 void diag_build(Ctx& c);
+
+void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,
+                      const double* xrest, double dhat, double kappa, const int* btype, int index_offset,
+                      long long triplet_offset, double* fb);
+int classify_contacts(Ctx& c, int npt, const int* pt, int nee, const int* ee, const double* x, const double* xrest,
+                      double dhat, int* pairs, int* mat_index, int capacity, int counts[3]);
 
 struct SolveOpts {
     int precision = 0;  // 0 fp64, 1 mixed

@@ -469,10 +469,12 @@ void fem_assemble(Ctx& c, int T, const double* x, const uint32_t* tets, const do
     if (triplet_offset < 0 || triplet_offset + 10LL * T > c.t_cap)
         throw capacity_error("fem_assemble: triplet capacity exceeded");
     if (T <= 0) return;
+    prof_begin(c, PROF_FEM);
     hipLaunchKernelGGL(k_fem_snk1, dim3(div_up(T, 128)), dim3(128), 0, c.stream, T, x, (const uint4*)tets, DmInv,
                        vol, mu, lam, dt * dt, btype, index_offset, c.t_vals + 9 * triplet_offset,
                        c.t_rows + triplet_offset, c.t_cols + triplet_offset, c.tet_grad.p);
     SGIPC_LAUNCH_CHECK();
+    prof_end(c, PROF_FEM);
     hipLaunchKernelGGL(k_gather_tet_grad, dim3(div_up(c.fem_n, 256)), dim3(256), 0, c.stream, c.fem_n,
                        c.inc_start.p, c.inc_list.p, c.tet_grad.p, shape);
     SGIPC_LAUNCH_CHECK();

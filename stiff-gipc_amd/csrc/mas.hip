@@ -161,20 +161,38 @@ __global__ void k_set_level_size(int* __restrict__ ls, int level, const int* __r
     ls[2 * (level + 1) + 1] = (level == 0) ? first_begin : beg_prev + (cnt_prev + B - 1) / B * B;
 }
 
-// coarse connectivity at `level` (nodes are level-local ids in coarse_space row level-1)
+// coarse connectivity at `level` (nodes are level-local ids in coarse_space
+// row level-1).  Masks are OR-reduced across the wavefront when every lane
+// targets the same coarse node (the common case on coarse levels, where many
+// vertices share one node), and an atomic is skipped when it would add no bit.
 __global__ void k_lx_masks(int n, int B, int level, const unsigned* __restrict__ nstart,
                            const unsigned* __restrict__ nnum, const unsigned* __restrict__ nlist,
                            const int* __restrict__ coarse_space, unsigned* __restrict__ next_mask) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
+    bool valid = v < n;
     const int* cs = coarse_space + (size_t)(level - 1) * n;
-    int ci = cs[v];
-    unsigned msk = 0u, st = nstart[v], nn = nnum[v];
-    for (unsigned i = 0; i < nn; ++i) {
-        int cu = cs[nlist[st + i]];
-        if (cu / B == ci / B) msk |= 1u << (cu % B);
+    int ci = valid ? cs[v] : -1;
+    unsigned msk = 0u;
+    if (valid) {
+        unsigned st = nstart[v], nn = nnum[v];
+        for (unsigned i = 0; i < nn; ++i) {
+            int cu = cs[nlist[st + i]];
+            if (cu / B == ci / B) msk |= 1u << (cu % B);
+        }
     }
-    if (msk) atomicOr(next_mask + ci, msk);
+    int first = __shfl(ci, __ffsll((long long)__ballot(valid)) - 1);
+    bool uniform = __all(!valid || ci == first);
+    if (uniform) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) msk |= __shfl_xor(msk, off);
+        if ((threadIdx.x & 63) == __ffsll((long long)__ballot(valid)) - 1 && valid && msk) {
+            unsigned cur = __hip_atomic_load(next_mask + ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((cur | msk) != cur) atomicOr(next_mask + ci, msk);
+        }
+    } else if (valid && msk) {
+        unsigned cur = __hip_atomic_load(next_mask + ci, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((cur | msk) != cur) atomicOr(next_mask + ci, msk);
+    }
 }
 
 __global__ void __launch_bounds__(kTB) k_lx_closure(int nmax, int B, int level, const int* __restrict__ ls,
@@ -246,51 +264,72 @@ __global__ void k_aggregate(int n, int L, const int* __restrict__ real_map, cons
 // ---- Galerkin projection ----------------------------------------------------
 __device__ inline int pack_index(int B, int r, int c) { return B * r - r * (r + 1) / 2 + c; }  // r <= c
 
-__global__ void k_project(int K, const int* __restrict__ rows, const int* __restrict__ cols,
-                          const double* __restrict__ vals, int off, int n, int B, int L,
-                          const int* __restrict__ node_of, double* __restrict__ bank_upper) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    int r = rows[k] - off, c = cols[k] - off;
-    if (r < 0 || c < 0 || r >= n || c >= n) return;
-    double H[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];  // column-major
+// Galerkin projection of every BCOO block into the bank matrices of all
+// levels.  Level 0 targets are unique (plain stores); coarse levels whose
+// banks all fit in LDS (the top levels, where many fine blocks land on few
+// entries) are pre-summed per workgroup with LDS atomics and flushed once;
+// the remaining coarse levels use global FP64 atomics.
+constexpr int kProjLdsDoubles = 6 * 1224;  // 6 banks of 16 (136 blocks x 9) or 1 bank of 32 + margin
+__global__ void __launch_bounds__(256) k_project(int K, const int* __restrict__ rows, const int* __restrict__ cols,
+                                                 const double* __restrict__ vals, int off, int n, int B, int L,
+                                                 const int* __restrict__ node_of, double* __restrict__ bank_upper,
+                                                 int lds_bank_lo, int lds_nbanks) {
+    __shared__ double acc[kProjLdsDoubles];
     const int npack = B * (B + 1) / 2;
-    for (int l = 0; l < L; ++l) {
-        int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
-        if (I / B != J / B) continue;
-        int li = I % B, lj = J % B;
-        double* dst;
-        double blk[9];
-        if (li == lj) {
-            dst = bank_upper + ((size_t)(I / B) * npack + pack_index(B, li, li)) * 9;
+    const int lds_len = lds_nbanks * npack * 9;
+    for (int i = threadIdx.x; i < lds_len; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
+        int r = rows[k] - off, c = cols[k] - off;
+        if (r < 0 || c < 0 || r >= n || c >= n) continue;
+        double H[9];
 #pragma unroll
-            for (int p = 0; p < 3; ++p)
+        for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];  // column-major
+        for (int l = 0; l < L; ++l) {
+            int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
+            if (I / B != J / B) continue;
+            int li = I % B, lj = J % B, bank = I / B;
+            int pidx;
+            double blk[9];
+            if (li == lj) {
+                pidx = pack_index(B, li, li);
 #pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    blk[p + 3 * q] = (r == c) ? H[p + 3 * q] : H[p + 3 * q] + H[q + 3 * p];
-        } else if (li < lj) {
-            dst = bank_upper + ((size_t)(I / B) * npack + pack_index(B, li, lj)) * 9;
+                for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int e = 0; e < 9; ++e) blk[e] = H[e];
-        } else {
-            dst = bank_upper + ((size_t)(I / B) * npack + pack_index(B, lj, li)) * 9;
+                    for (int q = 0; q < 3; ++q)
+                        blk[p + 3 * q] = (r == c) ? H[p + 3 * q] : H[p + 3 * q] + H[q + 3 * p];
+            } else if (li < lj) {
+                pidx = pack_index(B, li, lj);
 #pragma unroll
-            for (int p = 0; p < 3; ++p)
+                for (int e = 0; e < 9; ++e) blk[e] = H[e];
+            } else {
+                pidx = pack_index(B, lj, li);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) blk[p + 3 * q] = H[q + 3 * p];
-        }
-        if (l == 0) {
-            // level-0 targets are unique per BCOO block
+                for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int e = 0; e < 9; ++e) dst[e] += blk[e];
-        } else {
+                    for (int q = 0; q < 3; ++q) blk[p + 3 * q] = H[q + 3 * p];
+            }
+            if (l == 0) {
+                double* dst = bank_upper + ((size_t)bank * npack + pidx) * 9;  // unique per BCOO block
 #pragma unroll
-            for (int e = 0; e < 9; ++e)
-                if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
+                for (int e = 0; e < 9; ++e) dst[e] += blk[e];
+            } else if (bank >= lds_bank_lo && bank < lds_bank_lo + lds_nbanks) {
+                double* dst = acc + ((size_t)(bank - lds_bank_lo) * npack + pidx) * 9;
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
+            } else {
+                double* dst = bank_upper + ((size_t)bank * npack + pidx) * 9;
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
+            }
         }
     }
+    __syncthreads();
+    double* g = bank_upper + (size_t)lds_bank_lo * npack * 9;
+    for (int i = threadIdx.x; i < lds_len; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(g + i, acc[i]);
 }
 
 // ---- bank inverse: unpivoted in-place Gauss-Jordan in LDS -------------------
@@ -453,11 +492,13 @@ void mas_apply_t(Ctx& c, const double* r, double* z, DBuf<T>& Rb, DBuf<T>& Zb, c
                            m.part_map.p, m.going_next.p, R);
         SGIPC_LAUNCH_CHECK();
     }
+    prof_begin(c, PROF_MAS_SOLVE);
     if (m.bank == 16)
         hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, s, inv, R, Z);
     else
         hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, s, inv, R, Z);
     SGIPC_LAUNCH_CHECK();
+    prof_end(c, PROF_MAS_SOLVE);
     hipLaunchKernelGGL(k_collect<T>, dim3(div_up(m.n, 256)), dim3(256), 0, s, m.n, m.L, m.node_of.p, Z, z);
     SGIPC_LAUNCH_CHECK();
 }
@@ -587,8 +628,19 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
     m.inv32.reserve((size_t)m.nbanks * D * D);
     fill<double>(m.bank_upper.p, 0.0, (long long)m.nbanks * npack * 9, s);
     if (c.K > 0) {
-        hipLaunchKernelGGL(k_project, dim3(div_up(c.K, 256)), dim3(256), 0, s, c.K, c.t_rows, c.t_cols, c.t_vals,
-                           sub_offset, n, B, L, m.node_of.p, m.bank_upper.p);
+        // coarse levels whose banks fit the LDS accumulator (top levels first)
+        int lds_lo = m.nbanks, lds_nb = 0;
+        for (int l = L - 1; l >= 1; --l) {
+            int beg = m.level_size_h[2 * l + 1] / B;
+            int end = (l + 1 < L + 1) ? m.level_size_h[2 * (l + 1) + 1] / B : m.nbanks;
+            int nb_l = end - beg;
+            if ((lds_nb + nb_l) * npack * 9 > kProjLdsDoubles) break;
+            lds_nb += nb_l;
+            lds_lo = beg;
+        }
+        int grid = std::min(div_up(c.K, 256), 1024);
+        hipLaunchKernelGGL(k_project, dim3(grid), dim3(256), 0, s, c.K, c.t_rows, c.t_cols, c.t_vals, sub_offset, n, B,
+                           L, m.node_of.p, m.bank_upper.p, lds_lo, lds_nb);
         SGIPC_LAUNCH_CHECK();
     }
     if (B == 16)

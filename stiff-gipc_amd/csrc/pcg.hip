@@ -17,81 +17,56 @@
 namespace sgipc {
 namespace {
 
-enum { S_RZ = 0, S_RZ0, S_ALPHA, S_BETA, S_PAP, S_RR, S_BB, S_TOL, S_NUM };
-enum { C_DONE = 0, C_ITERS, C_T0, C_T1, C_T2, C_STOP_AFTER, C_NUM };
+enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
+enum { C_STOP = 0, C_NUM };  // C_STOP: iteration at which the loop stopped (0 = running)
 
 constexpr int kThreads = 256;
-constexpr int kMaxBlocks = 2048;
+constexpr int kRedBlocks = 512;   // producers of dot partials use at most this many blocks
+constexpr int kPart = 1024;       // partial slots per dot
 
-__device__ inline double block_sum(double v, double* sh) {
+__device__ inline double block_sum_all(double v, double* sh) {
+    // result broadcast to every thread; fixed reduction tree
     v = wave_sum(v);
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
     if (lane == 0) sh[w] = v;
     __syncthreads();
     double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
-    return s;  // valid in thread 0
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    return s;
 }
 
-// last-workgroup finalisation: returns true in thread 0 of the last block,
-// with the fixed-order total of partials[] in *total.
-__device__ inline bool last_block(double my_partial, double* partials, int* ticket, double* total) {
-    __shared__ bool is_last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = my_partial;
-        __threadfence();
-        int t = atomicAdd(ticket, 1);
-        is_last = (t == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < (int)gridDim.x; ++i) s += ((volatile double*)partials)[i];
-        *total = s;
-        *ticket = 0;
-    }
-    return threadIdx.x == 0;
+// sum of np partials, computed redundantly (same order) by every block
+__device__ inline double sum_partials(const double* __restrict__ part, int np, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+    return block_sum_all(v, sh);
 }
 
-enum { OP_INIT = 0, OP_BETA = 1, OP_BNORM = 2 };
+__device__ inline void write_partial(double v, double* __restrict__ part, double* sh) {
+    double s = block_sum_all(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
 
-// dot(a, b) with a finishing operation on the device scalars
-template <int OP>
-__global__ void __launch_bounds__(kThreads) k_dot(long long N, const double* __restrict__ a, const double* __restrict__ b,
-                                                  double* __restrict__ partials, double* __restrict__ sc,
-                                                  int* __restrict__ ctrl) {
-    if (OP == OP_BETA && ctrl[C_DONE]) return;
+// partial dot products of a.b, one per block
+__global__ void __launch_bounds__(kThreads) k_dot_part(long long N, const double* __restrict__ a,
+                                                       const double* __restrict__ b, double* __restrict__ part,
+                                                       const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
         v += a[i] * b[i];
-    double bs = block_sum(v, sh);
-    double tot;
-    if (last_block(bs, partials, ctrl + C_T1, &tot)) {
-        if (OP == OP_INIT) {
-            sc[S_RZ] = tot;
-            sc[S_RZ0] = tot;
-            if (tot == 0.0) { ctrl[C_DONE] = 1; ctrl[C_ITERS] = 0; }
-        } else if (OP == OP_BETA) {
-            sc[S_BETA] = tot / sc[S_RZ];
-            sc[S_RZ] = tot;
-        } else {
-            sc[S_BB] = tot;
-        }
-    }
+    write_partial(v, part, sh);
 }
 
-// Ap = A p (full block-CSR) and alpha = rz / (p.Ap); reference stop test.
+// Ap = A p (full block-CSR) with per-block partials of p.Ap
 template <int G>
 __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                        const double* __restrict__ val, int nnzb,
                                                        const double* __restrict__ p, double* __restrict__ Ap,
-                                                       double* __restrict__ partials, double* __restrict__ sc,
-                                                       int* __restrict__ ctrl, int iter, int rule) {
-    if (ctrl[C_DONE]) return;
+                                                       double* __restrict__ part, const int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double contrib = 0.0;
     for (int base = blockIdx.x * (kThreads / G); base < nb; base += gridDim.x * (kThreads / G)) {
@@ -119,26 +94,23 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
             contrib += p[o] * s0 + p[o + 1] * s1 + p[o + 2] * s2;
         }
     }
-    double bs = block_sum(contrib, sh);
-    double tot;
-    if (last_block(bs, partials, ctrl + C_T0, &tot)) {
-        sc[S_PAP] = tot;
-        sc[S_ALPHA] = sc[S_RZ] / tot;
-        if (rule == 0 && fabs(sc[S_RZ]) <= sc[S_TOL] * sc[S_RZ0]) {
-            ctrl[C_STOP_AFTER] = 1;
-            ctrl[C_ITERS] = iter;
-        }
-    }
+    write_partial(contrib, part, sh);
 }
 
-// x += alpha p ; r -= alpha Ap ; optional true-relative stop test on ||r||
+// alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; partials of r.r.
+// Reference stop test (|rz| <= tol rz0 on the pre-update rz) decided here.
 __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __restrict__ x, double* __restrict__ r,
                                                         const double* __restrict__ p, const double* __restrict__ Ap,
-                                                        double* __restrict__ partials, double* __restrict__ sc,
+                                                        const double* __restrict__ part_pap, int np_pap,
+                                                        const double* __restrict__ part_rz, int np_rz,
+                                                        double* __restrict__ part_rr, double* __restrict__ sc,
                                                         int* __restrict__ ctrl, int iter, int rule) {
-    if (ctrl[C_DONE]) return;
+    int stop = ctrl[C_STOP];
+    if (stop && stop < iter) return;
     __shared__ double sh[kThreads / 64];
-    double al = sc[S_ALPHA];
+    double rz = sum_partials(part_rz, np_rz, sh);
+    double pap = sum_partials(part_pap, np_pap, sh);
+    double al = rz / pap;
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         x[i] += al * p[i];
@@ -146,30 +118,47 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
         r[i] = ri;
         v += ri * ri;
     }
-    double bs = block_sum(v, sh);
-    double tot;
-    if (last_block(bs, partials, ctrl + C_T2, &tot)) {
-        sc[S_RR] = tot;
-        if (rule == 0) {
-            if (ctrl[C_STOP_AFTER]) ctrl[C_DONE] = 1;
-        } else if (sqrt(tot) <= sc[S_TOL] * sqrt(sc[S_BB])) {
-            ctrl[C_DONE] = 1;
-            ctrl[C_ITERS] = iter;
-        }
+    write_partial(v, part_rr, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double rz0 = (iter == 1) ? rz : sc[S_RZ0];
+        if (iter == 1) sc[S_RZ0] = rz;
+        if (rule == 0 && fabs(rz) <= sc[S_TOL] * rz0) ctrl[C_STOP] = iter;
     }
 }
 
-__global__ void k_update_p(long long N, double* __restrict__ p, const double* __restrict__ z,
-                           const double* __restrict__ sc, const int* __restrict__ ctrl) {
-    if (ctrl[C_DONE]) return;
-    double be = sc[S_BETA];
+// true-relative stop test ||r|| <= tol ||b|| (one block)
+__global__ void __launch_bounds__(kThreads) k_check_rr(const double* __restrict__ part_rr, int np, const double* __restrict__ sc,
+                                                       int* __restrict__ ctrl, int iter) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double sh[kThreads / 64];
+    double rr = sum_partials(part_rr, np, sh);
+    if (threadIdx.x == 0 && sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB])) ctrl[C_STOP] = iter;
+}
+
+// beta = rz_new / rz_old ; p = z + beta p
+__global__ void __launch_bounds__(kThreads) k_update_p(long long N, double* __restrict__ p, const double* __restrict__ z,
+                                                       const double* __restrict__ part_new, const double* __restrict__ part_old,
+                                                       int np, const int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double sh[kThreads / 64];
+    double rn = sum_partials(part_new, np, sh);
+    double ro = sum_partials(part_old, np, sh);
+    double be = rn / ro;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
         p[i] = z[i] + be * p[i];
 }
 
+// scalar = sum of partials into sc[idx] (one block)
+__global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict__ part, int np, double* __restrict__ sc,
+                                                        int idx) {
+    __shared__ double sh[kThreads / 64];
+    double s = sum_partials(part, np, sh);
+    if (threadIdx.x == 0) sc[idx] = s;
+}
+
 __global__ void k_copy_skip(long long N, double* __restrict__ dst, const double* __restrict__ src,
                             const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[C_DONE]) return;
+    if (ctrl && ctrl[C_STOP]) return;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
@@ -190,7 +179,7 @@ __global__ void k_diag_inv(int K, const int* __restrict__ rows, const int* __res
 
 __global__ void k_diag_apply(int nb, const double* __restrict__ Dinv, const double* __restrict__ r,
                              double* __restrict__ z, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[C_DONE]) return;
+    if (ctrl && ctrl[C_STOP]) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
     const double* D = Dinv + 9 * (size_t)i;
@@ -202,7 +191,7 @@ __global__ void k_diag_apply(int nb, const double* __restrict__ Dinv, const doub
 
 int grid_for(long long N) {
     int g = div_up(N, kThreads);
-    return g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g);
+    return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
 }
 
 }  // namespace
@@ -254,68 +243,89 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
     double* z = w.z.reserve(N + 1);
     double* p = w.p.reserve(N + 1);
     double* Ap = w.Ap.reserve(N + 1);
-    double* parts = w.partials.reserve(kMaxBlocks + 8);
+    double* parts = w.partials.reserve(5 * kPart);
+    double* part_rz[2] = {parts, parts + kPart};
+    double* part_pap = parts + 2 * kPart;
+    double* part_rr = parts + 3 * kPart;
+    double* part_bb = parts + 4 * kPart;
     double* sc = w.scal.reserve(S_NUM + 8);
     int* ctrl = w.ctrl.reserve(C_NUM + 8);
     fill<int>(ctrl, 0, C_NUM, s);
     double host_sc[S_NUM] = {0};
     host_sc[S_TOL] = o.tol;
     SGIPC_HIP(hipMemcpyAsync(sc, host_sc, sizeof(host_sc), hipMemcpyHostToDevice, s));
-    int max_iter = o.max_iter > 0 ? o.max_iter : (int)(o.max_iter_ratio * (double)N);
+    const int max_iter = o.max_iter > 0 ? o.max_iter : (int)(o.max_iter_ratio * (double)N);
     const int gN = grid_for(N);
+    const Bsr& B = c.bsr;
+    constexpr int G = 8;
+    const int gS = std::max(1, std::min(kPart, div_up((long long)B.nb * G, kThreads)));
     fill<double>(x, 0.0, N, s);
     hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, r, b, (const int*)nullptr);
     SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dot<OP_BNORM>, dim3(gN), dim3(kThreads), 0, s, N, b, b, parts, sc, ctrl);
+    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, b, b, part_bb, (const int*)nullptr);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
     SGIPC_LAUNCH_CHECK();
     precond_apply_ctrl(c, r, z, nullptr);
-    hipLaunchKernelGGL(k_dot<OP_INIT>, dim3(gN), dim3(kThreads), 0, s, N, r, z, parts, sc, ctrl);
+    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, r, z, part_rz[0], (const int*)nullptr);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
     SGIPC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, p, z, (const int*)nullptr);
     SGIPC_LAUNCH_CHECK();
-    const Bsr& B = c.bsr;
-    constexpr int G = 8;
-    const int gS = std::max(1, std::min(kMaxBlocks, div_up((long long)B.nb * G, kThreads)));
-    int check_every = o.check_every > 0 ? o.check_every : 4;
-    int k = 1;
-    int hctrl[C_NUM] = {0};
-    bool done = false;
-    for (; k < max_iter; ++k) {
-        hipLaunchKernelGGL(k_spmv_pap<G>, dim3(gS), dim3(kThreads), 0, s, B.nb, B.rowptr.p, B.col.p, B.val.p, B.nnzb, p,
-                           Ap, parts, sc, ctrl, k, o.stop_rule);
-        SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_update_xr, dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, parts, sc, ctrl, k, o.stop_rule);
-        SGIPC_LAUNCH_CHECK();
-        precond_apply_ctrl(c, r, z, ctrl);
-        hipLaunchKernelGGL(k_dot<OP_BETA>, dim3(gN), dim3(kThreads), 0, s, N, r, z, parts, sc, ctrl);
-        SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_update_p, dim3(gN), dim3(kThreads), 0, s, N, p, z, sc, ctrl);
-        SGIPC_LAUNCH_CHECK();
-        if ((k % check_every) == 0 || k + 1 >= max_iter) {
-            SGIPC_HIP(hipMemcpyAsync(hctrl, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
-            SGIPC_HIP(hipStreamSynchronize(s));
-            if (hctrl[C_DONE]) { done = true; break; }
-        }
-    }
-    SGIPC_HIP(hipMemcpyAsync(hctrl, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, s));
+    double rz0 = 0.0;
+    SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
-    done = hctrl[C_DONE] != 0;
-    st.converged = done ? 1 : 0;
-    st.iterations = done ? hctrl[C_ITERS] : max_iter;
-    st.rz0 = host_sc[S_RZ0];
+    st.rz0 = rz0;
+    int stop_iter = 0;
+    if (rz0 == 0.0) {
+        // zero right-hand side: x = 0 is exact (the reference would divide 0/0)
+        st.iterations = 0;
+        st.converged = 1;
+    } else {
+        const int check_every = o.check_every > 0 ? o.check_every : 8;
+        int k = 1;
+        for (; k < max_iter; ++k) {
+            prof_begin(c, PROF_SPMV);
+            hipLaunchKernelGGL(k_spmv_pap<G>, dim3(gS), dim3(kThreads), 0, s, B.nb, B.rowptr.p, B.col.p, B.val.p,
+                               B.nnzb, p, Ap, part_pap, ctrl);
+            SGIPC_LAUNCH_CHECK();
+            prof_end(c, PROF_SPMV);
+            hipLaunchKernelGGL(k_update_xr, dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
+                               part_rz[(k - 1) & 1], gN, part_rr, sc, ctrl, k, o.stop_rule);
+            SGIPC_LAUNCH_CHECK();
+            if (o.stop_rule == 1) {
+                hipLaunchKernelGGL(k_check_rr, dim3(1), dim3(kThreads), 0, s, part_rr, gN, sc, ctrl, k);
+                SGIPC_LAUNCH_CHECK();
+            }
+            precond_apply_ctrl(c, r, z, ctrl);
+            hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, r, z, part_rz[k & 1], ctrl);
+            SGIPC_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_update_p, dim3(gN), dim3(kThreads), 0, s, N, p, z, part_rz[k & 1],
+                               part_rz[(k - 1) & 1], gN, ctrl);
+            SGIPC_LAUNCH_CHECK();
+            if ((k % check_every) == 0 || k + 1 >= max_iter) {
+                SGIPC_HIP(hipMemcpyAsync(&stop_iter, ctrl + C_STOP, sizeof(int), hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipStreamSynchronize(s));
+                if (stop_iter) break;
+            }
+        }
+        SGIPC_HIP(hipMemcpyAsync(&stop_iter, ctrl + C_STOP, sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
+        st.converged = stop_iter ? 1 : 0;
+        st.iterations = stop_iter ? stop_iter : max_iter;
+    }
     // true residual ||b - A x|| / ||b||, FP64 (tmp = b - A x)
     double* tmp = w.xw.reserve(N + 1);
     hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, tmp, b, (const int*)nullptr);
     SGIPC_LAUNCH_CHECK();
     spmv64(c, -1.0, x, 1.0, tmp);
-    fill<int>(ctrl, 0, C_NUM, s);
-    hipLaunchKernelGGL(k_dot<OP_BNORM>, dim3(gN), dim3(kThreads), 0, s, N, tmp, tmp, parts, sc, ctrl);
+    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, tmp, tmp, part_rr, (const int*)nullptr);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rr, gN, sc, (int)S_RZ0);
     SGIPC_LAUNCH_CHECK();
     double rr = 0.0, bb = 0.0;
-    SGIPC_HIP(hipMemcpyAsync(&rr, sc + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
-    hipLaunchKernelGGL(k_dot<OP_BNORM>, dim3(gN), dim3(kThreads), 0, s, N, b, b, parts, sc, ctrl);
-    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(hipMemcpyAsync(&rr, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipMemcpyAsync(&bb, sc + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
     st.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);

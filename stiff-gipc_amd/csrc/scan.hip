@@ -110,4 +110,54 @@ void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipS
     SGIPC_LAUNCH_CHECK();
 }
 
+static hipEvent_t pool_get(Ctx& c) {
+    if (!c.prof.pool.empty()) {
+        hipEvent_t e = c.prof.pool.back();
+        c.prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    SGIPC_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void prof_begin(Ctx& c, int slot) {
+    if (!c.prof.on) return;
+    hipEvent_t a = pool_get(c), b = pool_get(c);
+    SGIPC_HIP(hipEventRecord(a, c.stream));
+    c.prof.pend[slot].push_back({a, b});
+}
+
+void prof_end(Ctx& c, int slot) {
+    if (!c.prof.on || c.prof.pend[slot].empty()) return;
+    SGIPC_HIP(hipEventRecord(c.prof.pend[slot].back().second, c.stream));
+}
+
+void prof_flush(Ctx& c) {
+    bool any = false;
+    for (int s = 0; s < PROF_NSLOTS; ++s) any |= !c.prof.pend[s].empty();
+    if (!any) return;
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < PROF_NSLOTS; ++s) {
+        for (auto& pr : c.prof.pend[s]) {
+            float ms = 0.f;
+            SGIPC_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            c.prof.ms[s] += ms;
+            c.prof.calls[s] += 1;
+            c.prof.pool.push_back(pr.first);
+            c.prof.pool.push_back(pr.second);
+        }
+        c.prof.pend[s].clear();
+    }
+}
+
+void prof_release(Ctx& c) {
+    for (int s = 0; s < PROF_NSLOTS; ++s) {
+        for (auto& pr : c.prof.pend[s]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+        c.prof.pend[s].clear();
+    }
+    for (auto e : c.prof.pool) (void)hipEventDestroy(e);
+    c.prof.pool.clear();
+}
+
 }  // namespace sgipc

@@ -14,6 +14,7 @@ ERR = {1: "invalid argument", 2: "HIP error", 3: "capacity exceeded", 4: "invali
 PRECOND_NONE, PRECOND_DIAG, PRECOND_MAS, PRECOND_MAS_FP32 = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 STOP_REF_RZ, STOP_TRUE_REL = 0, 1
+PROF_SPMV, PROF_MAS_SOLVE, PROF_FEM, PROF_CONVERT, PROF_PRECOND_BUILD, PROF_SOLVE, PROF_BARRIER = range(7)
 
 
 class SolveOpts(C.Structure):
@@ -53,11 +54,15 @@ SIGNATURES = {
     "sgipc_assemble_fem": ([P, I, P, P, P, P, P, P, D, P, I, LL, P], I),
     "sgipc_mass_triplets": ([P, I, P, I, LL], I),
     "sgipc_fem_rhs": ([P, I, P, P, P, P], I),
+    "sgipc_classify_contacts": ([P, I, P, I, P, P, P, D, P, P, I, C.POINTER(I), P], I),
+    "sgipc_assemble_barrier": ([P, I, P, P, P, P, P, D, D, P, I, LL, P], I),
     "sgipc_convert": ([P, LL, I, C.POINTER(I)], I),
     "sgipc_spmv": ([P, D, P, D, P], I),
     "sgipc_precond_build": ([P, I, I, I, P, I], I),
     "sgipc_precond_apply": ([P, P, P], I),
     "sgipc_solve": ([P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)], I),
+    "sgipc_profile_enable": ([P, I], I),
+    "sgipc_profile_read": ([P, I, C.POINTER(LL), C.POINTER(D), I], I),
     "sgipc_mas_info": ([P, C.POINTER(I), C.POINTER(I), P], I),
     "sgipc_mas_tables": ([P, P, P, P, P, P], I),
     "sgipc_mas_bank_inverse": ([P, I, P], I),

@@ -199,3 +199,109 @@ def cube_scene(cells=10, seed=1, E=1e5, nu=0.45, density=1000.0, dt=0.01, gravit
     x_tilde[body.btype != 0] = x[body.btype != 0]
     return Scene("cube%d" % cells, x_rest, x, x_tilde, body.tets, body.part, body.btype, DmInv, vol, mass,
                  mu, lam, dt, bank=bank, meta=dict(cells=cells, E=E, nu=nu, seed=seed))
+
+
+def two_box_scene(cells=40, seed=2, E=1e6, nu=0.4, density=1000.0, dt=0.01, gravity=-9.8, size=1.0,
+                  rel_dhat=1e-3, kappa=1e4, vel=0.5, sigma_rel=0.01, bank=16):
+    """Two soft boxes (cells^3 each, 5 tets/cell) stacked along y with a gap
+    of 0.5 dhat, dhat = rel_dhat * box size (a distance); the lower box's
+    bottom face is pinned.  Seeded uniform initial velocities in [-vel, vel]
+    and a seeded Gaussian perturbation (sigma_rel of the edge) of the
+    positions.  Returns the Scene with dhat stored squared (reference
+    convention, S/GIPC.cu:8602)."""
+    h = size / cells
+    dhat = rel_dhat * size
+    gap = 0.5 * dhat
+    lower = make_box(cells, cells, cells, h, origin=(0.0, 0.0, 0.0), bank=bank, pin_bottom=True)
+    upper = make_box(cells, cells, cells, h, origin=(0.0, size + gap, 0.0), bank=bank, pin_bottom=False)
+    x_rest, tets, part, btype = concat_bodies([lower, upper])
+    rng = np.random.default_rng(seed)
+    DmInv, vol, mass, mu, lam = rest_precompute(x_rest, tets, E, nu, density)
+    # perturb tangentially only on the facing faces so the gap stays 0.5 dhat
+    x = x_rest.copy()
+    noise = rng.normal(0.0, sigma_rel * h, size=x.shape)
+    facing = (np.abs(x_rest[:, 1] - size) < 1e-12) | (np.abs(x_rest[:, 1] - (size + gap)) < 1e-12)
+    noise[facing, 1] = 0.0
+    x += noise
+    x[btype != 0] = x_rest[btype != 0]
+    v = rng.uniform(-vel, vel, size=x.shape)
+    v[btype != 0] = 0.0
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = x + dt * v + dt * dt * g
+    x_tilde[btype != 0] = x[btype != 0]
+    return Scene("twobox%d" % cells, x_rest, x, x_tilde, tets, part, btype, DmInv, vol, mass, mu, lam, dt,
+                 dhat=dhat * dhat, kappa=kappa, bank=bank,
+                 meta=dict(cells=cells, E=E, nu=nu, seed=seed, dhat_dist=dhat, gap=gap,
+                           n_lower=lower.x_rest.shape[0]))
+
+
+def boundary_triangles(tets):
+    """Faces that belong to exactly one tet (unoriented vertex triples)."""
+    t = tets.astype(np.int64)
+    faces = np.concatenate([t[:, [0, 1, 2]], t[:, [0, 1, 3]], t[:, [0, 2, 3]], t[:, [1, 2, 3]]])
+    key = np.sort(faces, axis=1)
+    uniq, idx, cnt = np.unique(key, axis=0, return_index=True, return_counts=True)
+    return faces[idx[cnt == 1]]
+
+
+def _grid_cells(pts_xz, h):
+    return np.floor(pts_xz / h + 1e-9).astype(np.int64)
+
+
+def facing_contact_candidates(sc, h=None):
+    """Point-triangle and edge-edge candidate quadruples between the two
+    facing faces of a two-box scene (the broad-phase stand-in: primitives
+    whose grid cells are within one cell of each other).  Returns
+    (pt (m,4) int32, ee (k,4) int32)."""
+    size = 1.0
+    cells = sc.meta["cells"]
+    h = size / cells if h is None else h
+    gap = sc.meta["gap"]
+    n_lower = sc.meta["n_lower"]
+    tris = boundary_triangles(sc.tets)
+    xr = sc.x_rest
+    ylo = np.abs(xr[:, 1] - size) < 1e-9
+    yhi = np.abs(xr[:, 1] - (size + gap)) < 1e-9
+    faces = []
+    for sel in (ylo, yhi):
+        fm = sel[tris].all(axis=1)
+        faces.append(tris[fm])
+    out_pt, out_ee = [], []
+
+    def cell_of(idx):
+        return _grid_cells(xr[idx][..., [0, 2]], h)
+
+    # edges of each face
+    edges = []
+    for f in faces:
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [0, 2]]])
+        e = np.unique(np.sort(e, axis=1), axis=0)
+        edges.append(e)
+    for a, b in ((0, 1), (1, 0)):
+        # points of face a vs triangles of face b
+        pts = np.unique(faces[a].reshape(-1))
+        tri = faces[b]
+        tcell = cell_of(tri).min(axis=1)
+        buckets = {}
+        for i, c in enumerate(map(tuple, tcell)):
+            buckets.setdefault(c, []).append(i)
+        pcell = cell_of(pts)
+        for p, c in zip(pts, map(tuple, pcell)):
+            for di in (-1, 0):
+                for dk in (-1, 0):
+                    for ti in buckets.get((c[0] + di, c[1] + dk), ()):
+                        out_pt.append((p, tri[ti, 0], tri[ti, 1], tri[ti, 2]))
+    ea, eb = edges
+    bcell = cell_of(eb).min(axis=1)
+    buckets = {}
+    for i, c in enumerate(map(tuple, bcell)):
+        buckets.setdefault(c, []).append(i)
+    acell = cell_of(ea).min(axis=1)
+    for i, c in enumerate(map(tuple, acell)):
+        for di in (-1, 0, 1):
+            for dk in (-1, 0, 1):
+                for j in buckets.get((c[0] + di, c[1] + dk), ()):
+                    out_ee.append((ea[i, 0], ea[i, 1], eb[j, 0], eb[j, 1]))
+    pt = np.asarray(out_pt, np.int32).reshape(-1, 4)
+    ee = np.asarray(out_ee, np.int32).reshape(-1, 4)
+    return pt, ee

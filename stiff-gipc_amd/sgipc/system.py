@@ -102,15 +102,53 @@ class NewtonSystem:
                       self.nl.ctypes.data, self.nl.shape[0], self.part_map.ctypes.data, self.part_map.shape[0],
                       self.real_map.ctypes.data, s.bank)
         self.pairs = None
+        self.mat_index = None
+        self.counts = np.zeros(3, np.int32)
         self.n_pairs = 0
+        self.n_contact_triplets = 0
         self.nt = 0
         self.K = 0
 
+    def _set_capacity(self, cap):
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.cap = int(cap)
+        self.tv = torch.zeros(9 * self.cap, **f64)
+        self.tr = torch.zeros(self.cap, dtype=torch.int32, device=self.dev)
+        self.tc = torch.zeros(self.cap, dtype=torch.int32, device=self.dev)
+        self.ctx.call("sgipc_triplets_view", _p(self.tv), _p(self.tr), _p(self.tc), self.cap)
+
+    def setup_contacts(self, pt, ee):
+        """Classify candidate quadruples on the GPU (narrow phase) at the
+        current iterate and keep the resulting pairs for the Newton step."""
+        pt = np.ascontiguousarray(pt, np.int32).reshape(-1, 4)
+        ee = np.ascontiguousarray(ee, np.int32).reshape(-1, 4)
+        m = pt.shape[0] + ee.shape[0]
+        dpt = torch.tensor(pt.reshape(-1), dtype=torch.int32, device=self.dev) if len(pt) else None
+        dee = torch.tensor(ee.reshape(-1), dtype=torch.int32, device=self.dev) if len(ee) else None
+        pairs = torch.zeros(4 * max(m, 1), dtype=torch.int32, device=self.dev)
+        mi = torch.zeros(max(m, 1), dtype=torch.int32, device=self.dev)
+        npair = C.c_int(0)
+        counts = np.zeros(3, np.int32)
+        self.ctx.call("sgipc_classify_contacts", len(pt), _p(dpt), len(ee), _p(dee), _p(self.x), _p(self.x_rest),
+                      float(self.sc.dhat), _p(pairs), _p(mi), m, C.byref(npair), counts.ctypes.data)
+        self.n_pairs = npair.value
+        self.pairs = pairs[:4 * self.n_pairs].clone() if self.n_pairs else None
+        self.mat_index = mi[:self.n_pairs].clone() if self.n_pairs else None
+        self.counts = counts
+        self.n_contact_triplets = 4 * int(counts[0]) + 9 * int(counts[1]) + 16 * int(counts[2])
+        self._set_capacity(10 * self.T + self.n + self.n_contact_triplets)
+        return self.n_pairs
+
     # ---- steps --------------------------------------------------------------
-    def assemble(self, contact_triplets=0):
+    def assemble(self):
         s = self.sc
         self.ctx.call("sgipc_kinetic_gradient", self.n, _p(self.x), _p(self.x_tilde), _p(self.mass), _p(self.shape))
-        off = int(contact_triplets)
+        self.fb.zero_()
+        if self.n_pairs:
+            self.ctx.call("sgipc_assemble_barrier", self.n_pairs, _p(self.pairs), _p(self.mat_index),
+                          self.counts.ctypes.data, _p(self.x), _p(self.x_rest), float(s.dhat), float(s.kappa),
+                          _p(self.btype), 0, 0, _p(self.fb))
+        off = self.n_contact_triplets
         self.ctx.call("sgipc_assemble_fem", self.T, _p(self.x), _p(self.tets), _p(self.DmInv), _p(self.vol),
                       _p(self.mu), _p(self.lam), float(s.dt), _p(self.btype), 0, off, _p(self.shape))
         off += 10 * self.T
@@ -179,6 +217,14 @@ class NewtonSystem:
                       p0.ctypes.data)
         return dict(levelnum=L, total=tot, level_size=ls[:2 * (L + 1)].reshape(-1, 2), coarse_space=cs.reshape(L, n),
                     going_next=gn, coarse_table=ct.reshape(n, 6), fine_mask=fm, prefix0=p0)
+
+    def mas_levels(self):
+        L = C.c_int(0)
+        try:
+            self.ctx.call("sgipc_mas_info", C.byref(L), None, None)
+        except capi.SgipcError:
+            return 0
+        return L.value
 
     def bank_inverse(self, b):
         D = 3 * self.sc.bank
