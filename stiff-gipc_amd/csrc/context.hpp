@@ -48,10 +48,17 @@ struct MasState {
     DBuf<int> sub_index;     // BCOO block indices in the subsystem range
 };
 
+struct PcgGraph {
+    hipGraphExec_t exec = nullptr;
+    size_t key = 0;
+};
+
 struct PcgWork {
     DBuf<double> r, z, p, Ap, xw;   // 3N each
+    DBuf<double> r64b, d64;         // mixed precision: FP64 residual and correction
+    PcgGraph graph64, graph32, graph_mixed;
     DBuf<double> partials;          // per-workgroup partial dot products
-    DBuf<double> scal;              // device scalars (see pcg.hip)
+    DBuf<double> scal;              // device scalars (see pcg.hip); S_NUM+1 entries used
     DBuf<int> ctrl;                 // done flag, iteration counters, tickets
     DBuf<float> rf, zf, pf, Apf;    // mixed-precision inner vectors
     DBuf<double> Dinv;              // 9 * nb block-Jacobi inverses
@@ -138,6 +145,8 @@ void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, co
 void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs);
 void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
+void mas_up_solve64(Ctx& c, const int* ctrl);
+void mas_up_solve32(Ctx& c, const int* ctrl);
 void diag_build(Ctx& c);
 
 void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,
@@ -153,6 +162,7 @@ struct SolveOpts {
     double max_iter_ratio = 0.3;
     int max_iter = 0;
     int check_every = 8;
+    double inner_tol = 1e-4;  // mixed precision: floor of the inner relative tolerance
 };
 struct SolveStats {
     int iterations = 0;

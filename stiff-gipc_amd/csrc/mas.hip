@@ -401,7 +401,9 @@ __global__ void k_restrict0(int M, const int* __restrict__ part_map, const doubl
 template <typename T>
 __global__ void __launch_bounds__(kTB) k_restrict_up(int level_from, int B, const int* __restrict__ ls, int M,
                                                       const int* __restrict__ part_map,
-                                                      const int* __restrict__ going_next, T* __restrict__ R) {
+                                                      const int* __restrict__ going_next, T* __restrict__ R,
+                                                      const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
     __shared__ int par[kTB];
     __shared__ T sv[3][kTB];
     int c = blockIdx.x * blockDim.x + threadIdx.x;  // local index within level
@@ -435,7 +437,8 @@ __global__ void __launch_bounds__(kTB) k_restrict_up(int level_from, int B, cons
 
 template <int B, typename T>
 __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_solve(const T* __restrict__ inv, const T* __restrict__ R,
-                                                                    T* __restrict__ Z) {
+                                                                    T* __restrict__ Z, const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
     constexpr int D = 3 * B;
     __shared__ T rs[D];
     int b = blockIdx.x, i = threadIdx.x;
@@ -474,31 +477,37 @@ int num_levels(int nn, int B) {
     return nLevel > 6 ? 6 : nLevel;
 }
 
+// restriction to the coarse levels + bank solves on all levels; R level 0
+// must already hold the slot-ordered residual.  `ctrl` (device, may be
+// null) skips the work once the PCG loop has stopped.
 template <typename T>
-void mas_apply_t(Ctx& c, const double* r, double* z, DBuf<T>& Rb, DBuf<T>& Zb, const T* inv) {
+void mas_up_solve_t(Ctx& c, T* R, T* Z, const T* inv, const int* ctrl) {
     MasState& m = c.mas;
     hipStream_t s = c.stream;
-    if (m.n <= 0) return;
-    T* R = Rb.reserve(3 * (size_t)m.total + 3);
-    T* Z = Zb.reserve(3 * (size_t)m.total + 3);
-    int Mpad = (m.M + m.bank - 1) / m.bank * m.bank;
-    fill<T>(R + 3 * (size_t)Mpad, (T)0, 3LL * (m.total - Mpad), s);
-    hipLaunchKernelGGL(k_restrict0<T>, dim3(div_up(m.M, 256)), dim3(256), 0, s, m.M, m.part_map.p, r, R);
-    SGIPC_LAUNCH_CHECK();
     for (int l = 0; l + 1 < m.L; ++l) {
         int cnt = (l == 0) ? m.M : m.level_size_h[2 * l];
         if (cnt <= 0) break;
         hipLaunchKernelGGL(k_restrict_up<T>, dim3(div_up(cnt, kTB)), dim3(kTB), 0, s, l, m.bank, m.level_size.p, m.M,
-                           m.part_map.p, m.going_next.p, R);
+                           m.part_map.p, m.going_next.p, R, ctrl);
         SGIPC_LAUNCH_CHECK();
     }
     prof_begin(c, PROF_MAS_SOLVE);
     if (m.bank == 16)
-        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, s, inv, R, Z);
+        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, s, inv, R, Z, ctrl);
     else
-        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, s, inv, R, Z);
+        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, s, inv, R, Z, ctrl);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_MAS_SOLVE);
+}
+
+template <typename T>
+void mas_apply_t(Ctx& c, const double* r, double* z, T* R, T* Z, const T* inv) {
+    MasState& m = c.mas;
+    hipStream_t s = c.stream;
+    if (m.n <= 0) return;
+    hipLaunchKernelGGL(k_restrict0<T>, dim3(div_up(m.M, 256)), dim3(256), 0, s, m.M, m.part_map.p, r, R);
+    SGIPC_LAUNCH_CHECK();
+    mas_up_solve_t<T>(c, R, Z, inv, nullptr);
     hipLaunchKernelGGL(k_collect<T>, dim3(div_up(m.n, 256)), dim3(256), 0, s, m.n, m.L, m.node_of.p, Z, z);
     SGIPC_LAUNCH_CHECK();
 }
@@ -650,16 +659,26 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         hipLaunchKernelGGL(k_bank_inverse<32>, dim3(m.nbanks), dim3(128), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,
                            m.inv32.p);
     SGIPC_LAUNCH_CHECK();
+    // apply work vectors: allocated here so their addresses are stable for
+    // captured PCG graphs; padding entries stay zero (never written)
+    m.R64.reserve(3 * (size_t)m.total + 3);
+    m.Z64.reserve(3 * (size_t)m.total + 3);
+    m.R32.reserve(3 * (size_t)m.total + 3);
+    m.Z32.reserve(3 * (size_t)m.total + 3);
+    fill<double>(m.R64.p, 0.0, 3LL * m.total + 3, s);
+    fill<float>(m.R32.p, 0.0f, 3LL * m.total + 3, s);
     m.ready = true;
 }
 
 void mas_apply64(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
-    mas_apply_t<double>(c, r, z, c.mas.R64, c.mas.Z64, c.mas.inv64.p);
+    mas_apply_t<double>(c, r, z, c.mas.R64.p, c.mas.Z64.p, c.mas.inv64.p);
 }
 void mas_apply32(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
-    mas_apply_t<float>(c, r, z, c.mas.R32, c.mas.Z32, c.mas.inv32.p);
+    mas_apply_t<float>(c, r, z, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p);
 }
+void mas_up_solve64(Ctx& c, const int* ctrl) { mas_up_solve_t<double>(c, c.mas.R64.p, c.mas.Z64.p, c.mas.inv64.p, ctrl); }
+void mas_up_solve32(Ctx& c, const int* ctrl) { mas_up_solve_t<float>(c, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p, ctrl); }
 
 }  // namespace sgipc

@@ -18,7 +18,7 @@ namespace sgipc {
 namespace {
 
 enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
-enum { C_STOP = 0, C_NUM };  // C_STOP: iteration at which the loop stopped (0 = running)
+enum { C_STOP = 0, C_ITER, C_MAXIT, C_NOCONV, C_NUM };  // C_STOP: iteration at which the loop stopped (0 = running)
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 512;   // producers of dot partials use at most this many blocks
@@ -48,35 +48,48 @@ __device__ inline void write_partial(double v, double* __restrict__ part, double
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// partial dot products of a.b, one per block
-__global__ void __launch_bounds__(kThreads) k_dot_part(long long N, const double* __restrict__ a,
-                                                       const double* __restrict__ b, double* __restrict__ part,
-                                                       const int* __restrict__ ctrl) {
+// partial dot products of a.b, one per block (FP64 accumulation)
+template <typename VA, typename VB>
+__global__ void __launch_bounds__(kThreads) k_dot_part(long long N, const VA* __restrict__ a, const VB* __restrict__ b,
+                                                       double* __restrict__ part, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-        v += a[i] * b[i];
+        v += (double)a[i] * (double)b[i];
     write_partial(v, part, sh);
 }
 
-// Ap = A p (full block-CSR) with per-block partials of p.Ap
-template <int G>
+// scalar = sum of partials into sc[idx] (one block)
+__global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict__ part, int np, double* __restrict__ sc,
+                                                        int idx) {
+    __shared__ double sh[kThreads / 64];
+    double s = sum_partials(part, np, sh);
+    if (threadIdx.x == 0) sc[idx] = s;
+}
+
+// Ap = A p (full block-CSR) with per-block partials of p.Ap.  First kernel of
+// an iteration: enforces the iteration budget.
+template <int G, typename MV, typename V>
 __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                       const double* __restrict__ val, int nnzb,
-                                                       const double* __restrict__ p, double* __restrict__ Ap,
-                                                       double* __restrict__ part, const int* __restrict__ ctrl) {
+                                                       const MV* __restrict__ val, int nnzb,
+                                                       const V* __restrict__ p, V* __restrict__ Ap,
+                                                       double* __restrict__ part, int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
+    if (ctrl[C_ITER] >= ctrl[C_MAXIT]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
+        return;
+    }
     __shared__ double sh[kThreads / 64];
     double contrib = 0.0;
     for (int base = blockIdx.x * (kThreads / G); base < nb; base += gridDim.x * (kThreads / G)) {
         int row = base + threadIdx.x / G, sub = threadIdx.x % G;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        MV s0 = 0, s1 = 0, s2 = 0;
         if (row < nb) {
             int e1 = rowptr[row + 1];
             for (int e = rowptr[row] + sub; e < e1; e += G) {
                 int j = col[e];
-                double x0 = p[3 * (size_t)j], x1 = p[3 * (size_t)j + 1], x2 = p[3 * (size_t)j + 2];
+                MV x0 = (MV)p[3 * (size_t)j], x1 = (MV)p[3 * (size_t)j + 1], x2 = (MV)p[3 * (size_t)j + 2];
                 s0 += val[(size_t)0 * nnzb + e] * x0 + val[(size_t)3 * nnzb + e] * x1 + val[(size_t)6 * nnzb + e] * x2;
                 s1 += val[(size_t)1 * nnzb + e] * x0 + val[(size_t)4 * nnzb + e] * x1 + val[(size_t)7 * nnzb + e] * x2;
                 s2 += val[(size_t)2 * nnzb + e] * x0 + val[(size_t)5 * nnzb + e] * x1 + val[(size_t)8 * nnzb + e] * x2;
@@ -90,78 +103,163 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
         }
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;
-            Ap[o] = s0; Ap[o + 1] = s1; Ap[o + 2] = s2;
-            contrib += p[o] * s0 + p[o + 1] * s1 + p[o + 2] * s2;
+            Ap[o] = (V)s0; Ap[o + 1] = (V)s1; Ap[o + 2] = (V)s2;
+            contrib += (double)p[o] * (double)(V)s0 + (double)p[o + 1] * (double)(V)s1 + (double)p[o + 2] * (double)(V)s2;
         }
     }
     write_partial(contrib, part, sh);
 }
 
-// alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; partials of r.r.
-// Reference stop test (|rz| <= tol rz0 on the pre-update rz) decided here.
-__global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __restrict__ x, double* __restrict__ r,
-                                                        const double* __restrict__ p, const double* __restrict__ Ap,
+// alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; partials of r.r ;
+// optionally scatter r into the MAS level-0 slots (fused restriction).
+template <typename V, typename TR, bool FUSE>
+__global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __restrict__ x, V* __restrict__ r,
+                                                        const V* __restrict__ p, const V* __restrict__ Ap,
                                                         const double* __restrict__ part_pap, int np_pap,
                                                         const double* __restrict__ part_rz, int np_rz,
-                                                        double* __restrict__ part_rr, double* __restrict__ sc,
-                                                        int* __restrict__ ctrl, int iter, int rule) {
-    int stop = ctrl[C_STOP];
-    if (stop && stop < iter) return;
+                                                        double* __restrict__ part_rr, const int* __restrict__ ctrl,
+                                                        long long sub3, int n_mas, const int* __restrict__ real_map,
+                                                        TR* __restrict__ R0) {
+    if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double rz = sum_partials(part_rz, np_rz, sh);
     double pap = sum_partials(part_pap, np_pap, sh);
     double al = rz / pap;
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-        x[i] += al * p[i];
-        double ri = r[i] - al * Ap[i];
+        x[i] += al * (double)p[i];
+        V ri = (V)((double)r[i] - al * (double)Ap[i]);
         r[i] = ri;
-        v += ri * ri;
+        v += (double)ri * (double)ri;
+        if (FUSE) {
+            long long j = i - sub3;
+            if (j >= 0 && j < 3LL * n_mas) R0[3 * (size_t)real_map[j / 3] + (j % 3)] = (TR)ri;
+        }
     }
     write_partial(v, part_rr, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double rz0 = (iter == 1) ? rz : sc[S_RZ0];
-        if (iter == 1) sc[S_RZ0] = rz;
-        if (rule == 0 && fabs(rz) <= sc[S_TOL] * rz0) ctrl[C_STOP] = iter;
-    }
 }
 
-// true-relative stop test ||r|| <= tol ||b|| (one block)
-__global__ void __launch_bounds__(kThreads) k_check_rr(const double* __restrict__ part_rr, int np, const double* __restrict__ sc,
-                                                       int* __restrict__ ctrl, int iter) {
+// Stop tests, evaluated identically by every block of the kernel that
+// follows the x/r update (so no block depends on another's flag write):
+// rule 0 (reference): |rz_prev| <= tol * rz0 on the pre-update rz;
+// rule 1: ||r|| <= tol * ||b|| on the updated residual.  rule < 0: none.
+__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, const double* __restrict__ part_rr,
+                                 int np, const double* __restrict__ sc, int* __restrict__ ctrl, double* sh) {
+    bool stop = false;
+    if (rule == 0) {
+        double rz = sum_partials(part_rz_prev, np, sh);
+        stop = fabs(rz) <= sc[S_TOL] * sc[S_RZ0];
+    } else if (rule == 1) {
+        double rr = sum_partials(part_rr, np, sh);
+        stop = sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB]);
+    }
+    if (stop && blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_STOP] = ctrl[C_ITER];
+    return stop;
+}
+
+// Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range (identity
+// elsewhere) and partials of r.z.
+template <typename V, typename T>
+__global__ void __launch_bounds__(kThreads) k_collect_dot(long long N, const V* __restrict__ r, V* __restrict__ z,
+                                                          long long sub3, int n_mas, int L, const int* __restrict__ node_of,
+                                                          const T* __restrict__ Z, double* __restrict__ part_rz,
+                                                          const double* __restrict__ part_rz_prev,
+                                                          const double* __restrict__ part_rr, int np,
+                                                          const double* __restrict__ sc, int* __restrict__ ctrl,
+                                                          int rule) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
-    double rr = sum_partials(part_rr, np, sh);
-    if (threadIdx.x == 0 && sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB])) ctrl[C_STOP] = iter;
+    if (stop_test(rule, part_rz_prev, part_rr, np, sc, ctrl, sh)) return;
+    double v = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        long long j = i - sub3;
+        V zi;
+        if (j >= 0 && j < 3LL * n_mas) {
+            int vt = (int)(j / 3), k = (int)(j % 3);
+            T s = 0;
+            for (int l = 0; l < L; ++l) s += Z[3 * (size_t)node_of[(size_t)l * n_mas + vt] + k];
+            zi = (V)s;
+        } else {
+            zi = r[i];
+        }
+        z[i] = zi;
+        v += (double)r[i] * (double)zi;
+    }
+    write_partial(v, part_rz, sh);
 }
 
-// beta = rz_new / rz_old ; p = z + beta p
-__global__ void __launch_bounds__(kThreads) k_update_p(long long N, double* __restrict__ p, const double* __restrict__ z,
+// Stop test, then z = D^-1 r (block Jacobi) or z = r, and partials of r.z
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, const V* __restrict__ r, V* __restrict__ z,
+                                                                 const double* __restrict__ Dinv, double* __restrict__ part_rz,
+                                                                 const double* __restrict__ part_rz_prev,
+                                                                 const double* __restrict__ part_rr, int np,
+                                                                 const double* __restrict__ sc, int* __restrict__ ctrl,
+                                                                 int rule) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double sh[kThreads / 64];
+    if (stop_test(rule, part_rz_prev, part_rr, np, sc, ctrl, sh)) return;
+    double v = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        V zi;
+        if (Dinv) {
+            long long b = i / 3;
+            int p = (int)(i % 3);
+            const double* D = Dinv + 9 * (size_t)b;
+            zi = (V)(D[p] * (double)r[3 * b] + D[p + 3] * (double)r[3 * b + 1] + D[p + 6] * (double)r[3 * b + 2]);
+        } else {
+            zi = r[i];
+        }
+        z[i] = zi;
+        v += (double)r[i] * (double)zi;
+    }
+    write_partial(v, part_rz, sh);
+}
+
+// beta = rz_new / rz_old ; p = z + beta p ; next iteration
+template <typename V>
+__global__ void __launch_bounds__(kThreads) k_update_p(long long N, V* __restrict__ p, const V* __restrict__ z,
                                                        const double* __restrict__ part_new, const double* __restrict__ part_old,
-                                                       int np, const int* __restrict__ ctrl) {
+                                                       int np, int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double rn = sum_partials(part_new, np, sh);
     double ro = sum_partials(part_old, np, sh);
     double be = rn / ro;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-        p[i] = z[i] + be * p[i];
+        p[i] = (V)((double)z[i] + be * (double)p[i]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_ITER] += 1;
 }
 
-// scalar = sum of partials into sc[idx] (one block)
-__global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict__ part, int np, double* __restrict__ sc,
-                                                        int idx) {
-    __shared__ double sh[kThreads / 64];
-    double s = sum_partials(part, np, sh);
-    if (threadIdx.x == 0) sc[idx] = s;
-}
-
-__global__ void k_copy_skip(long long N, double* __restrict__ dst, const double* __restrict__ src,
-                            const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[C_STOP]) return;
+template <typename TD, typename TS>
+__global__ void k_copy_cast(long long N, TD* __restrict__ dst, const TS* __restrict__ src) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+        dst[i] = (TD)src[i];
 }
+
+__global__ void k_axpy(long long N, double* __restrict__ y, double a, const double* __restrict__ x) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+
+template <typename V, typename T>
+__global__ void k_restrict0_v(int M, const int* __restrict__ part_map, const V* __restrict__ r, long long sub3,
+                              T* __restrict__ R) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    int v = part_map[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) R[3 * (size_t)s + k] = (v >= 0) ? (T)r[sub3 + 3 * (size_t)v + k] : (T)0;
+}
+
+__global__ void k_ctrl_init(int* ctrl, int maxit) {
+    ctrl[C_STOP] = 0;
+    ctrl[C_ITER] = 1;
+    ctrl[C_MAXIT] = maxit;
+    ctrl[C_NOCONV] = 0;
+}
+
+__global__ void k_set_scalar(double* sc, int idx, double v) { sc[idx] = v; }
 
 __global__ void k_diag_inv(int K, const int* __restrict__ rows, const int* __restrict__ cols,
                            const double* __restrict__ vals, double* __restrict__ Dinv) {
@@ -177,22 +275,174 @@ __global__ void k_diag_inv(int K, const int* __restrict__ rows, const int* __res
     R[2] = C * id; R[5] = -(a * h - b * g) * id; R[8] = (a * e - b * d) * id;
 }
 
-__global__ void k_diag_apply(int nb, const double* __restrict__ Dinv, const double* __restrict__ r,
-                             double* __restrict__ z, const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[C_STOP]) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    const double* D = Dinv + 9 * (size_t)i;
-    double r0 = r[3 * (size_t)i], r1 = r[3 * (size_t)i + 1], r2 = r[3 * (size_t)i + 2];
-    z[3 * (size_t)i] = D[0] * r0 + D[3] * r1 + D[6] * r2;
-    z[3 * (size_t)i + 1] = D[1] * r0 + D[4] * r1 + D[7] * r2;
-    z[3 * (size_t)i + 2] = D[2] * r0 + D[5] * r1 + D[8] * r2;
-}
-
 int grid_for(long long N) {
     int g = div_up(N, kThreads);
     return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
 }
+
+// ---- the PCG engine --------------------------------------------------------
+// Vectors of type V (double or float), matrix values MV, MAS data T.
+template <typename V, typename MV, typename T>
+struct Engine {
+    Ctx& c;
+    long long N;
+    int gN, gS;
+    V *r, *z, *p, *Ap;
+    double* x;
+    double *part_rz[2], *part_pap, *part_rr, *part_bb;
+    double* sc;
+    int* ctrl;
+    const MV* val;
+    bool mas;
+    T *R, *Z;
+    const double* Dinv;
+    int rule;
+    int maxit = 0;
+
+    static constexpr int G = 8;
+
+    // kernels of iteration k with parity par = k & 1
+    void enqueue_iteration(int par) {
+        hipStream_t s = c.stream;
+        const Bsr& B = c.bsr;
+        double* rz_prev = part_rz[par ^ 1];
+        double* rz_cur = part_rz[par];
+        prof_begin(c, PROF_SPMV);
+        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, B.nb, B.rowptr.p, B.col.p, val,
+                           B.nnzb, p, Ap, part_pap, ctrl);
+        SGIPC_LAUNCH_CHECK();
+        prof_end(c, PROF_SPMV);
+        const long long sub3 = 3LL * c.sub_offset;
+        if (mas) {
+            hipLaunchKernelGGL((k_update_xr<V, T, true>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
+                               rz_prev, gN, part_rr, ctrl, sub3, c.mas.n, c.mas.real_map.p, R);
+            SGIPC_LAUNCH_CHECK();
+            if (sizeof(T) == 8) mas_up_solve64(c, ctrl);
+            else mas_up_solve32(c, ctrl);
+            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, N, r, z, sub3, c.mas.n, c.mas.L,
+                               c.mas.node_of.p, Z, rz_cur, rz_prev, part_rr, gN, sc, ctrl, rule);
+            SGIPC_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL((k_update_xr<V, T, false>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
+                               rz_prev, gN, part_rr, ctrl, sub3, 0, (const int*)nullptr, (T*)nullptr);
+            SGIPC_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_cur, rz_prev,
+                               part_rr, gN, sc, ctrl, rule);
+            SGIPC_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL((k_update_p<V>), dim3(gN), dim3(kThreads), 0, s, N, p, z, rz_cur, rz_prev, gN, ctrl);
+        SGIPC_LAUNCH_CHECK();
+    }
+
+    // r holds the right-hand side, x the initial guess is zero.  Returns the
+    // host copy of rz0 (0 means a zero right-hand side).
+    double init(double tol, int max_iter) {
+        hipStream_t s = c.stream;
+        maxit = max_iter;
+        hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, max_iter);
+        SGIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL, tol);
+        SGIPC_LAUNCH_CHECK();
+        fill<double>(x, 0.0, N, s);
+        hipLaunchKernelGGL((k_dot_part<V, V>), dim3(gN), dim3(kThreads), 0, s, N, r, r, part_bb, (const int*)nullptr);
+        SGIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
+        SGIPC_LAUNCH_CHECK();
+        const long long sub3 = 3LL * c.sub_offset;
+        if (mas) {
+            hipLaunchKernelGGL((k_restrict0_v<V, T>), dim3(div_up(c.mas.M, 256)), dim3(256), 0, s, c.mas.M,
+                               c.mas.part_map.p, r, sub3, R);
+            SGIPC_LAUNCH_CHECK();
+            if (sizeof(T) == 8) mas_up_solve64(c, ctrl);
+            else mas_up_solve32(c, ctrl);
+            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, N, r, z, sub3, c.mas.n, c.mas.L,
+                               c.mas.node_of.p, Z, part_rz[0], part_rz[1], part_rr, gN, sc, ctrl, -1);
+            SGIPC_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, part_rz[0],
+                               part_rz[1], part_rr, gN, sc, ctrl, -1);
+            SGIPC_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
+        SGIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_copy_cast<V, V>), dim3(gN), dim3(kThreads), 0, s, N, p, z);
+        SGIPC_LAUNCH_CHECK();
+        double rz0 = 0.0;
+        SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
+        return rz0;
+    }
+
+    size_t graph_key() const {
+        size_t h = 1469598103934665603ull;
+        auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
+        const Bsr& B = c.bsr;
+        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)B.nb); mix((size_t)B.nnzb);
+        mix((size_t)B.rowptr.p); mix((size_t)B.col.p); mix((size_t)val); mix((size_t)r); mix((size_t)z);
+        mix((size_t)p); mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc); mix((size_t)ctrl);
+        mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
+        mix((size_t)c.sub_offset); mix((size_t)c.stream);
+        if (mas) {
+            const MasState& m = c.mas;
+            mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.going_next.p);
+            mix((size_t)m.node_of.p); mix((size_t)m.level_size.p); mix((size_t)m.part_map.p); mix((size_t)m.real_map.p);
+            mix((size_t)m.inv64.p); mix((size_t)m.inv32.p);
+            for (int l = 0; l < m.L; ++l) mix((size_t)m.level_size_h[2 * l]);
+        }
+        return h;
+    }
+
+    // Run iterations until the device stop flag is set; returns the
+    // reference iteration count and whether it converged.
+    int run(PcgGraph& gcache, int check_every, bool& converged) {
+        hipStream_t s = c.stream;
+        if (maxit <= 1) {  // no iteration fits the budget
+            converged = false;
+            return maxit;
+        }
+        // profiling records events per kernel, which a captured graph cannot
+        const bool use_graph = !c.prof.on;
+        int pairs_per_check = std::max(1, check_every / 2);
+        if (use_graph) {
+            size_t key = graph_key();
+            if (!gcache.exec || gcache.key != key) {
+                if (gcache.exec) { (void)hipGraphExecDestroy(gcache.exec); gcache.exec = nullptr; }
+                hipGraph_t g = nullptr;
+                SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_iteration(1);
+                    enqueue_iteration(0);
+                } catch (...) {
+                    hipGraph_t dead = nullptr;
+                    (void)hipStreamEndCapture(s, &dead);
+                    if (dead) (void)hipGraphDestroy(dead);
+                    throw;
+                }
+                SGIPC_HIP(hipStreamEndCapture(s, &g));
+                hipError_t e = hipGraphInstantiate(&gcache.exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                SGIPC_HIP(e);
+                gcache.key = key;
+            }
+        }
+        int hc[C_NUM] = {0};
+        for (;;) {
+            for (int q = 0; q < pairs_per_check; ++q) {
+                if (use_graph) {
+                    SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
+                } else {
+                    enqueue_iteration(1);
+                    enqueue_iteration(0);
+                }
+            }
+            SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+            SGIPC_HIP(hipStreamSynchronize(s));
+            if (hc[C_STOP]) break;
+        }
+        converged = hc[C_NOCONV] == 0;
+        return hc[C_STOP];
+    }
+};
 
 }  // namespace
 
@@ -206,128 +456,140 @@ void diag_build(Ctx& c) {
     }
 }
 
-// z = M^-1 r with the configured preconditioner; `ctrl` (device) lets the
-// PCG loop skip work after convergence (nullptr outside PCG).
-static void precond_apply_ctrl(Ctx& c, const double* r, double* z, const int* ctrl) {
+// z = M^-1 r with the configured preconditioner (standalone, outside PCG)
+void precond_apply(Ctx& c, const double* r, double* z) {
     long long N = 3LL * c.nb;
     hipStream_t s = c.stream;
-    if (c.precond_kind == 1) {
-        hipLaunchKernelGGL(k_diag_apply, dim3(div_up(c.nb, 256)), dim3(256), 0, s, c.nb, c.pcg.Dinv.p, r, z, ctrl);
-        SGIPC_LAUNCH_CHECK();
-        return;
-    }
     if (c.precond_kind == 2 || c.precond_kind == 3) {
         long long off = 3LL * c.sub_offset, len = 3LL * c.mas.n;
-        if (off > 0 || off + len < N) {  // identity outside the MAS subsystem range
-            hipLaunchKernelGGL(k_copy_skip, dim3(grid_for(N)), dim3(kThreads), 0, s, N, z, r, ctrl);
+        if (off > 0 || off + len < N) {
+            hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(grid_for(N)), dim3(kThreads), 0, s, N, z, r);
             SGIPC_LAUNCH_CHECK();
         }
         if (c.precond_kind == 2) mas_apply64(c, r + off, z + off);
         else mas_apply32(c, r + off, z + off);
         return;
     }
-    hipLaunchKernelGGL(k_copy_skip, dim3(grid_for(N)), dim3(kThreads), 0, s, N, z, r, ctrl);
+    double* part = c.pcg.partials.reserve(5 * kPart);
+    int* ctrl = c.pcg.ctrl.reserve(C_NUM + 8);
+    fill<int>(ctrl, 0, C_NUM, s);
+    hipLaunchKernelGGL((k_simple_precond_dot<double>), dim3(grid_for(N)), dim3(kThreads), 0, s, N, r, z,
+                       c.precond_kind == 1 ? (const double*)c.pcg.Dinv.p : (const double*)nullptr, part, part, part, 1,
+                       (const double*)nullptr, ctrl, -1);
     SGIPC_LAUNCH_CHECK();
 }
 
-void precond_apply(Ctx& c, const double* r, double* z) { precond_apply_ctrl(c, r, z, nullptr); }
+static double device_norm2(Ctx& c, const double* v, long long N, double* part, double* sc, int idx) {
+    hipStream_t s = c.stream;
+    int g = grid_for(N);
+    hipLaunchKernelGGL((k_dot_part<double, double>), dim3(g), dim3(kThreads), 0, s, N, v, v, part, (const int*)nullptr);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part, g, sc, idx);
+    SGIPC_LAUNCH_CHECK();
+    double h = 0.0;
+    SGIPC_HIP(hipMemcpyAsync(&h, sc + idx, sizeof(double), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    return h;
+}
 
 void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStats& st) {
     if (!c.have_bcoo) throw state_error("solve: convert first");
     if (!c.have_bsr) bsr_build(c);
-    if (o.precision != 0) throw arg_error("solve: mixed precision is not available in this build");
     hipStream_t s = c.stream;
     const long long N = 3LL * c.nb;
     PcgWork& w = c.pcg;
-    double* r = w.r.reserve(N + 1);
-    double* z = w.z.reserve(N + 1);
-    double* p = w.p.reserve(N + 1);
-    double* Ap = w.Ap.reserve(N + 1);
-    double* parts = w.partials.reserve(5 * kPart);
-    double* part_rz[2] = {parts, parts + kPart};
-    double* part_pap = parts + 2 * kPart;
-    double* part_rr = parts + 3 * kPart;
-    double* part_bb = parts + 4 * kPart;
+    const bool mas = (c.precond_kind == 2 || c.precond_kind == 3);
+    if (mas && !c.mas.ready) throw state_error("solve: MAS preconditioner not built");
+    if (c.precond_kind == 1 && !w.Dinv.p) throw state_error("solve: block-Jacobi preconditioner not built");
+    double* parts = w.partials.reserve(6 * kPart);
     double* sc = w.scal.reserve(S_NUM + 8);
     int* ctrl = w.ctrl.reserve(C_NUM + 8);
-    fill<int>(ctrl, 0, C_NUM, s);
-    double host_sc[S_NUM] = {0};
-    host_sc[S_TOL] = o.tol;
-    SGIPC_HIP(hipMemcpyAsync(sc, host_sc, sizeof(host_sc), hipMemcpyHostToDevice, s));
     const int max_iter = o.max_iter > 0 ? o.max_iter : (int)(o.max_iter_ratio * (double)N);
     const int gN = grid_for(N);
     const Bsr& B = c.bsr;
-    constexpr int G = 8;
-    const int gS = std::max(1, std::min(kPart, div_up((long long)B.nb * G, kThreads)));
-    fill<double>(x, 0.0, N, s);
-    hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, r, b, (const int*)nullptr);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, b, b, part_bb, (const int*)nullptr);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
-    SGIPC_LAUNCH_CHECK();
-    precond_apply_ctrl(c, r, z, nullptr);
-    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, r, z, part_rz[0], (const int*)nullptr);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, p, z, (const int*)nullptr);
-    SGIPC_LAUNCH_CHECK();
-    double rz0 = 0.0;
-    SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    st.rz0 = rz0;
-    int stop_iter = 0;
-    if (rz0 == 0.0) {
-        // zero right-hand side: x = 0 is exact (the reference would divide 0/0)
-        st.iterations = 0;
-        st.converged = 1;
+    const int gS = std::max(1, std::min(kPart, div_up((long long)B.nb * 8, kThreads)));
+    st.refinements = 0;
+    if (o.precision == 0) {
+        auto run64 = [&](auto tag) {
+            using T = decltype(tag);
+            const bool fp32mas = sizeof(T) == 4;
+            Engine<double, double, T> E{c, N, gN, gS,
+                                        w.r.reserve(N + 1), w.z.reserve(N + 1), w.p.reserve(N + 1), w.Ap.reserve(N + 1),
+                                        x, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
+                                        parts + 4 * kPart, sc, ctrl, B.val.p, mas,
+                                        mas ? (T*)(fp32mas ? (void*)c.mas.R32.p : (void*)c.mas.R64.p) : nullptr,
+                                        mas ? (T*)(fp32mas ? (void*)c.mas.Z32.p : (void*)c.mas.Z64.p) : nullptr,
+                                        c.precond_kind == 1 ? w.Dinv.p : nullptr, o.stop_rule};
+            hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, E.r, b);
+            SGIPC_LAUNCH_CHECK();
+            double rz0 = E.init(o.tol, max_iter);
+            st.rz0 = rz0;
+            if (rz0 == 0.0) {
+                // zero right-hand side: x = 0 is exact (the reference would divide 0/0)
+                st.iterations = 0;
+                st.converged = 1;
+            } else {
+                bool conv = false;
+                st.iterations = E.run(fp32mas ? w.graph32 : w.graph64, o.check_every, conv);
+                st.converged = conv ? 1 : 0;
+            }
+        };
+        if (c.precond_kind == 3) run64(float{});
+        else run64(double{});
     } else {
-        const int check_every = o.check_every > 0 ? o.check_every : 8;
-        int k = 1;
-        for (; k < max_iter; ++k) {
-            prof_begin(c, PROF_SPMV);
-            hipLaunchKernelGGL(k_spmv_pap<G>, dim3(gS), dim3(kThreads), 0, s, B.nb, B.rowptr.p, B.col.p, B.val.p,
-                               B.nnzb, p, Ap, part_pap, ctrl);
+        // mixed precision: FP32 SpMV / MAS / Krylov vectors with FP64 dots,
+        // outer iterative refinement on the FP64 true residual
+        if (!mas && c.precond_kind != 0 && c.precond_kind != 1) throw arg_error("solve: unsupported preconditioner");
+        float* rf = w.rf.reserve(N + 1);
+        double* r64 = w.r64b.reserve(N + 1);
+        double* d64 = w.d64.reserve(N + 1);
+        Engine<float, float, float> E{c, N, gN, gS,
+                                      rf, w.zf.reserve(N + 1), w.pf.reserve(N + 1), w.Apf.reserve(N + 1),
+                                      d64, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
+                                      parts + 4 * kPart, sc, ctrl, B.valf.p, mas,
+                                      mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
+                                      c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
+        fill<double>(x, 0.0, N, s);
+        hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
+        SGIPC_LAUNCH_CHECK();
+        double* part_o = parts + 5 * kPart;
+        const double bb = device_norm2(c, b, N, part_o, sc, S_NUM);
+        const double bn = std::sqrt(bb);
+        int total = 0;
+        bool conv = false;
+        double rn = bn;
+        st.rz0 = 0.0;
+        for (int outer = 0; outer < 50 && total < max_iter; ++outer) {
+            if (rn <= o.tol * bn || bn == 0.0) { conv = true; break; }
+            double eta = std::max(0.9 * o.tol * bn / rn, o.inner_tol);
+            hipLaunchKernelGGL((k_copy_cast<float, double>), dim3(gN), dim3(kThreads), 0, s, N, rf, r64);
             SGIPC_LAUNCH_CHECK();
-            prof_end(c, PROF_SPMV);
-            hipLaunchKernelGGL(k_update_xr, dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
-                               part_rz[(k - 1) & 1], gN, part_rr, sc, ctrl, k, o.stop_rule);
+            double rz0 = E.init(eta, max_iter - total);
+            if (outer == 0) st.rz0 = rz0;
+            if (rz0 == 0.0) { conv = true; break; }
+            bool c2 = false;
+            int it = E.run(w.graph_mixed, o.check_every, c2);
+            total += it;
+            hipLaunchKernelGGL(k_axpy, dim3(gN), dim3(kThreads), 0, s, N, x, 1.0, (const double*)d64);
             SGIPC_LAUNCH_CHECK();
-            if (o.stop_rule == 1) {
-                hipLaunchKernelGGL(k_check_rr, dim3(1), dim3(kThreads), 0, s, part_rr, gN, sc, ctrl, k);
-                SGIPC_LAUNCH_CHECK();
-            }
-            precond_apply_ctrl(c, r, z, ctrl);
-            hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, r, z, part_rz[k & 1], ctrl);
+            // FP64 true residual r = b - A x
+            hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
             SGIPC_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_update_p, dim3(gN), dim3(kThreads), 0, s, N, p, z, part_rz[k & 1],
-                               part_rz[(k - 1) & 1], gN, ctrl);
-            SGIPC_LAUNCH_CHECK();
-            if ((k % check_every) == 0 || k + 1 >= max_iter) {
-                SGIPC_HIP(hipMemcpyAsync(&stop_iter, ctrl + C_STOP, sizeof(int), hipMemcpyDeviceToHost, s));
-                SGIPC_HIP(hipStreamSynchronize(s));
-                if (stop_iter) break;
-            }
+            spmv64(c, -1.0, x, 1.0, r64);
+            rn = std::sqrt(device_norm2(c, r64, N, part_o, sc, S_NUM));
+            st.refinements = outer + 1;
         }
-        SGIPC_HIP(hipMemcpyAsync(&stop_iter, ctrl + C_STOP, sizeof(int), hipMemcpyDeviceToHost, s));
-        SGIPC_HIP(hipStreamSynchronize(s));
-        st.converged = stop_iter ? 1 : 0;
-        st.iterations = stop_iter ? stop_iter : max_iter;
+        if (rn <= o.tol * bn) conv = true;
+        st.iterations = total;
+        st.converged = conv ? 1 : 0;
     }
-    // true residual ||b - A x|| / ||b||, FP64 (tmp = b - A x)
+    // true residual ||b - A x|| / ||b||, FP64
     double* tmp = w.xw.reserve(N + 1);
-    hipLaunchKernelGGL(k_copy_skip, dim3(gN), dim3(kThreads), 0, s, N, tmp, b, (const int*)nullptr);
+    hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, tmp, b);
     SGIPC_LAUNCH_CHECK();
     spmv64(c, -1.0, x, 1.0, tmp);
-    hipLaunchKernelGGL(k_dot_part, dim3(gN), dim3(kThreads), 0, s, N, tmp, tmp, part_rr, (const int*)nullptr);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rr, gN, sc, (int)S_RZ0);
-    SGIPC_LAUNCH_CHECK();
-    double rr = 0.0, bb = 0.0;
-    SGIPC_HIP(hipMemcpyAsync(&rr, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipMemcpyAsync(&bb, sc + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
+    double rr = device_norm2(c, tmp, N, parts + 5 * kPart, sc, S_NUM);
+    double bb = device_norm2(c, b, N, parts + 5 * kPart, sc, S_NUM);
     st.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 

@@ -151,3 +151,76 @@ def test_pcg_matches_oracle(torch_cuda, oracle, cube10, precond):
     assert st.converged and st.rel_residual <= 1.5e-6
     assert np.linalg.norm(A @ x - ref["b"]) <= 1.5e-6 * np.linalg.norm(ref["b"])
     assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(xs)
+
+
+def test_pcg_iteration_budget_matches_oracle(torch_cuda, oracle, cube10):
+    # an unreachable tolerance: both stop at the budget with the same iterate
+    print("case: pcg budget max_iter=6", flush=True)
+    ref = oracle_system(oracle, cube10)
+    sys_ = make_system(cube10)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    kr, kc, kv = ref["bcoo"]
+    mas = oracle.Mas(cube10.n, sys_.part_map, sys_.real_map, sys_.ns, sys_.nn, sys_.nl, cube10.bank)
+    mas.build(kr, kc, kv)
+    st = sys_.solve(tol=1e-30, stop_rule=0, max_iter=6)
+    xo, ito, _ = oracle.pcg(kr, kc, kv, cube10.n, ref["b"], precond=2, mas=mas, tol=1e-30, rule=0, max_iter=6)
+    assert st.iterations == ito == 6 and not st.converged
+    x = sys_.dx.cpu().numpy()
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
+def test_pcg_graph_replays_match_eager(torch_cuda, cube10):
+    # captured iteration graphs (profiling off) against the eager launch
+    # sequence (profiling on), replayed over several solves with other
+    # device work in between; results must be bit-identical
+    import torch
+    from sgipc import capi
+    print("case: pcg graph replay vs eager", flush=True)
+    sys_ = make_system(cube10)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    b0 = sys_.b.clone()
+    xs = []
+    for rep in range(3):
+        sys_.b.copy_(b0 * (1.0 + rep))
+        st = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000)
+        xs.append((st.iterations, sys_.dx.clone()))
+        torch.cuda.synchronize()
+        _ = (sys_.b * 2.0).sum().item()  # unrelated eager work between replays
+    sys_.ctx.call("sgipc_profile_enable", 1)
+    try:
+        for rep in range(3):
+            sys_.b.copy_(b0 * (1.0 + rep))
+            st = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000)
+            it, xg = xs[rep]
+            assert st.iterations == it
+            assert torch.equal(sys_.dx, xg)
+    finally:
+        sys_.ctx.call("sgipc_profile_enable", 0)
+    sys_.b.copy_(b0)
+
+
+@pytest.mark.parametrize("mode", ["mas_fp32", "mixed_mas", "mixed_diag", "mixed_none"])
+def test_reduced_precision_solves(torch_cuda, oracle, cube10, mode):
+    # FP32 preconditioner storage and the mixed FP32/FP64 solve reach the
+    # same FP64 true relative residual as the FP64 solve
+    from sgipc import capi
+    print("case: reduced precision %s" % mode, flush=True)
+    ref = oracle_system(oracle, cube10)
+    kind = {"mas_fp32": capi.PRECOND_MAS_FP32, "mixed_mas": capi.PRECOND_MAS, "mixed_diag": capi.PRECOND_DIAG,
+            "mixed_none": capi.PRECOND_NONE}[mode]
+    prec = capi.PRECISION_FP64 if mode == "mas_fp32" else capi.PRECISION_MIXED
+    sys_ = make_system(cube10, kind)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    kr, kc, kv = ref["bcoo"]
+    A = dense_from_bcoo(kr, kc, kv, cube10.n)
+    xs = np.linalg.solve(A, ref["b"])
+    st64 = sys_.solve(tol=1e-6, stop_rule=1, max_iter=20000)
+    st = sys_.solve(tol=1e-6, stop_rule=1, max_iter=20000, precision=prec)
+    x = sys_.dx.cpu().numpy()
+    assert st.converged and st.rel_residual <= 1e-6
+    assert np.linalg.norm(A @ x - ref["b"]) <= 1.5e-6 * np.linalg.norm(ref["b"])
+    assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(xs)
+    # reduced precision costs at most a modest number of extra iterations
+    assert st.iterations <= 1.5 * st64.iterations + 10
+    if prec == capi.PRECISION_MIXED:
+        assert st.refinements >= 1
