@@ -1,4 +1,5 @@
 // extern "C" boundary of libsgipc (declarations: include/sgipc.h).
+#include <vector>
 #include "../../include/sgipc.h"
 #include "context.hpp"
 
@@ -361,8 +362,13 @@ int sgipc_mas_bank_inverse(sgipc_ctx* c, int b, double* inv) {
         if (b < 0 || b >= m.nbanks || !inv) throw sgipc::arg_error("mas_bank_inverse: bad bank");
         set_device(c);
         SGIPC_HIP(hipStreamSynchronize(c->stream));
-        size_t D = 3 * (size_t)m.bank;
-        SGIPC_HIP(hipMemcpy(inv, m.inv64.p + (size_t)b * D * D, sizeof(double) * D * D, hipMemcpyDeviceToHost));
+        // device copy: upper triangle packed by rows; expand to dense row-major
+        const size_t D = 3 * (size_t)m.bank, NP = D * (D + 1) / 2;
+        std::vector<double> up(NP);
+        SGIPC_HIP(hipMemcpy(up.data(), m.inv64.p + (size_t)b * NP, sizeof(double) * NP, hipMemcpyDeviceToHost));
+        size_t e = 0;
+        for (size_t r = 0; r < D; ++r)
+            for (size_t q = r; q < D; ++q, ++e) inv[r * D + q] = inv[q * D + r] = up[e];
     });
 }
 

@@ -8,16 +8,18 @@
 
 namespace sgipc {
 
-// Block-3x3 full symmetric CSR (both triangles) derived from the upper BCOO;
-// values SoA by component (val[c * nnzb + k], c = r + 3 col).
-struct Bsr {
-    int nb = 0;        // block rows
-    int nnzb = 0;      // stored blocks (2K - diag)
-    DBuf<int> rowptr;  // nb + 1
-    DBuf<int> col;     // nnzb
-    DBuf<double> val;  // 9 * nnzb
-    DBuf<float> valf;  // 9 * nnzb (mixed precision copy)
-    DBuf<int> src;     // nnzb: upper BCOO block index, negative-1 if transposed
+// Symmetric SpMV structure over the upper BCOO resident in the triplet view.
+// Block row j owns its upper blocks, BCOO entries [uniq_start[j],
+// uniq_start[j+1]) sorted by column, and a list of references {k, i} to the
+// upper blocks k = (i, j), i < j, applied transposed, sorted by k.
+struct SymBcoo {
+    int nb = 0;         // block rows
+    int K = 0;          // upper blocks
+    int nlow = 0;       // strictly-lower references
+    DBuf<int> lptr;     // nb + 1
+    DBuf<int2> lref;    // nlow
+    DBuf<float> valf;   // 9 * K FP32 copy of the BCOO values (mixed precision)
+    bool valf_ok = false;
 };
 
 struct MasState {
@@ -41,8 +43,10 @@ struct MasState {
     DBuf<int> coarse_table;  // 6*n (itable layout)
     DBuf<int> node_of;       // L*n : global node id of vertex at level l (l=0: slot)
     DBuf<double> bank_upper; // nbanks * npack * 9 (Galerkin, packed upper, col-major blocks)
-    DBuf<double> inv64;      // nbanks * D*D dense symmetric inverse (FP64)
-    DBuf<float> inv32;       // nbanks * D*D dense symmetric inverse (FP32)
+    DBuf<double> inv64;      // nbanks * D(D+1)/2 symmetric inverse, upper triangle packed by rows (FP64)
+    DBuf<float> inv32;       // same, FP32
+    DBuf<int> l1lo, l1hi;    // per level-1 bank: range [lo, hi) of level-0 banks holding its children
+    int nb1 = 0;             // level-1 banks
     DBuf<double> R64, Z64;   // 3*total
     DBuf<float> R32, Z32;
     DBuf<int> sub_index;     // BCOO block indices in the subsystem range
@@ -103,7 +107,7 @@ struct Ctx {
     DBuf<int> bcoo_row, bcoo_col;
     DBuf<int> row_count, row_start, bucket, uniq_count, uniq_start;
     DBuf<int> dev_int;    // small device ints (totals)
-    Bsr bsr;
+    SymBcoo bsr;
     bool have_bsr = false;
 
     // preconditioner
@@ -137,6 +141,7 @@ void fem_rhs(Ctx& c, int n, const double* fb, const double* shape, const int* bt
 
 int bcoo_convert(Ctx& c, long long nt, int nblockrows);
 void bsr_build(Ctx& c);
+const float* bsr_values_f32(Ctx& c);
 void spmv64(Ctx& c, double a, const double* x, double b, double* y);
 
 void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, const unsigned* nnum,
@@ -145,8 +150,11 @@ void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, co
 void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs);
 void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
-void mas_up_solve64(Ctx& c, const int* ctrl);
-void mas_up_solve32(Ctx& c, const int* ctrl);
+// fused multilevel restriction of r[sub3 .. sub3 + 3n) into R_l (FP64 or FP32
+// preconditioner data); bank solves of all levels; both skip when ctrl[0] != 0
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl);
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl);
+void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl);
 void diag_build(Ctx& c);
 
 void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,

@@ -158,71 +158,6 @@ __global__ void k_copy_back(int K, const int* __restrict__ r, const int* __restr
 }
 
 // ---- full symmetric block CSR --------------------------------------------
-__global__ void k_bsr_count(int K, const int* __restrict__ r, const int* __restrict__ c, int* __restrict__ cnt) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    atomicAdd(cnt + r[k], 1);
-    if (r[k] != c[k]) atomicAdd(cnt + c[k], 1);
-}
-
-// place each stored block: direct entries keep BCOO order inside a row
-// segment; transposed entries go to the row's tail via an atomic cursor and
-// are ordered afterwards by k_bsr_sort_tail.
-__global__ void k_bsr_place(int K, const int* __restrict__ r, const int* __restrict__ c,
-                            const int* __restrict__ ustart, const int* __restrict__ rowptr,
-                            int* __restrict__ cursor, int* __restrict__ bcol, int* __restrict__ bsrc) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    int i = r[k], j = c[k];
-    int nup = ustart[i + 1] - ustart[i];
-    int pos = rowptr[i] + (k - ustart[i]);
-    bcol[pos] = j;
-    bsrc[pos] = k;
-    if (i != j) {
-        int nupj = ustart[j + 1] - ustart[j];
-        int p2 = rowptr[j] + nupj + atomicAdd(cursor + j, 1);
-        bcol[p2] = i;
-        bsrc[p2] = -k - 1;
-    }
-    (void)nup;
-}
-
-__global__ void k_bsr_sort_tail(int nb, const int* __restrict__ ustart, const int* __restrict__ rowptr,
-                                int* __restrict__ bcol, int* __restrict__ bsrc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    int b = rowptr[i] + (ustart[i + 1] - ustart[i]), e = rowptr[i + 1];
-    for (int a = b + 1; a < e; ++a) {  // insertion sort by source block (descending -k-1 == ascending k)
-        int sv = bsrc[a], cv = bcol[a];
-        int q = a - 1;
-        while (q >= b && bsrc[q] < sv) {
-            bsrc[q + 1] = bsrc[q];
-            bcol[q + 1] = bcol[q];
-            --q;
-        }
-        bsrc[q + 1] = sv;
-        bcol[q + 1] = cv;
-    }
-}
-
-__global__ void k_bsr_values(int nnzb, const int* __restrict__ bsrc, const double* __restrict__ bv,
-                             double* __restrict__ val, float* __restrict__ valf) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnzb) return;
-    int s = bsrc[e];
-    bool tr = s < 0;
-    int k = tr ? (-s - 1) : s;
-    const double* A = bv + 9 * (size_t)k;  // column-major: A(p,q) = A[p + 3q]
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            double v = tr ? A[q + 3 * p] : A[p + 3 * q];
-            val[(size_t)(p + 3 * q) * nnzb + e] = v;
-            if (valf) valf[(size_t)(p + 3 * q) * nnzb + e] = (float)v;
-        }
-}
-
 }  // namespace
 
 int bcoo_convert(Ctx& c, long long nt, int nbr) {
@@ -284,46 +219,6 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
     c.K = K;
     c.have_bcoo = true;
     return K;
-}
-
-// Build the full symmetric block-CSR from the BCOO now resident in the
-// triplet view (rows/cols/vals [0, K)).
-void bsr_build(Ctx& c) {
-    if (!c.have_bcoo) throw state_error("bsr_build: convert first");
-    hipStream_t s = c.stream;
-    int nb = c.nb, K = c.K;
-    Bsr& B = c.bsr;
-    int* cnt = c.row_count.reserve((size_t)nb + 1);
-    int* rowptr = B.rowptr.reserve((size_t)nb + 2);
-    int* dint = c.dev_int.reserve(8);
-    fill<int>(cnt, 0, nb, s);
-    if (K > 0) {
-        hipLaunchKernelGGL(k_bsr_count, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, cnt);
-        SGIPC_LAUNCH_CHECK();
-    }
-    exclusive_scan(c, cnt, rowptr, nb, dint + 4, s);
-    int nnzb = 0;
-    SGIPC_HIP(hipMemcpyAsync(&nnzb, dint + 4, sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    B.nb = nb;
-    B.nnzb = nnzb;
-    int* bcol = B.col.reserve((size_t)nnzb + 1);
-    int* bsrc = B.src.reserve((size_t)nnzb + 1);
-    double* val = B.val.reserve(9 * (size_t)nnzb + 1);
-    float* valf = B.valf.reserve(9 * (size_t)nnzb + 1);
-    fill<int>(cnt, 0, nb, s);
-    if (K > 0) {
-        hipLaunchKernelGGL(k_bsr_place, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols,
-                           c.uniq_start.p, rowptr, cnt, bcol, bsrc);
-        SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bsr_sort_tail, dim3(div_up(nb, 256)), dim3(256), 0, s, nb, c.uniq_start.p, rowptr,
-                           bcol, bsrc);
-        SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bsr_values, dim3(div_up(nnzb, 256)), dim3(256), 0, s, nnzb, bsrc, c.t_vals, val,
-                           valf);
-        SGIPC_LAUNCH_CHECK();
-    }
-    c.have_bsr = true;
 }
 
 }  // namespace sgipc

@@ -82,9 +82,8 @@ __global__ void k_collision(int np, const int4* __restrict__ pairs, int node_off
                             const int* __restrict__ real_map, const int* __restrict__ coarse_space,
                             unsigned* __restrict__ dst) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int v[4], cp[4];
-    int cnt = decode_pair(pairs[p], node_offset, v);
+    int v[4] = {-1, -1, -1, -1}, cp[4];
+    int cnt = (p < np) ? decode_pair(pairs[p], node_offset, v) : 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (i >= cnt || v[i] < 0 || v[i] >= n) cp[i] = -1;
@@ -101,10 +100,31 @@ __global__ void k_collision(int np, const int4* __restrict__ pairs, int node_off
                 msk[j] |= 1u << (cp[i] % B);
             }
         }
+    // Coarse levels funnel many pairs onto few nodes: merge equal targets
+    // across the wave (a few leader rounds) and skip ORs that add no bit.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (cp[i] < 0 || msk[i] == 0u) continue;
-        atomicOr(dst + (level < 0 ? v[i] : cp[i]), msk[i]);
+        int addr = (cp[i] >= 0 && msk[i] != 0u) ? (level < 0 ? v[i] : cp[i]) : -1;
+        unsigned m = msk[i];
+        unsigned long long active = __ballot(addr >= 0);
+        for (int round = 0; round < 4 && active; ++round) {
+            int leader = __ffsll((long long)active) - 1;
+            int a = __shfl(addr, leader);
+            bool same = addr >= 0 && addr == a;
+            unsigned mm = same ? m : 0u;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mm |= __shfl_xor(mm, off);
+            if ((int)(threadIdx.x & 63) == leader) {
+                unsigned cur = __hip_atomic_load(dst + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((cur | mm) != cur) atomicOr(dst + a, mm);
+            }
+            active &= ~__ballot(same);
+            if (same) addr = -1;
+        }
+        if (addr >= 0) {
+            unsigned cur = __hip_atomic_load(dst + addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((cur | m) != cur) atomicOr(dst + addr, m);
+        }
     }
 }
 
@@ -332,6 +352,11 @@ __global__ void __launch_bounds__(256) k_project(int K, const int* __restrict__ 
         if (acc[i] != 0.0) atomicAdd(g + i, acc[i]);
 }
 
+// element (r, c), r <= c, of a symmetric D x D matrix stored as its upper
+// triangle packed by rows
+template <int D>
+__device__ inline int sym_index(int r, int c) { return r * (2 * D - r + 1) / 2 + (c - r); }
+
 // ---- bank inverse: unpivoted in-place Gauss-Jordan in LDS -------------------
 template <int B>
 __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_inverse(int nbanks, const double* __restrict__ bank_upper,
@@ -372,83 +397,60 @@ __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_inverse(int nbanks,
         }
         __syncthreads();
     }
-    // symmetrise: upper triangle mirrored; store dense row-major
+    // symmetric result: keep the upper triangle, packed by rows
     if (i < D) {
-        double* o64 = inv64 + (size_t)b * D * D;
-        float* o32 = inv32 + (size_t)b * D * D;
-        for (int r = 0; r < D; ++r) {
-            double v = (r <= i) ? S[r][i] : S[i][r];
-            o64[(size_t)r * D + i] = v;
-            o32[(size_t)r * D + i] = (float)v;
+        constexpr int NP = D * (D + 1) / 2;
+        double* o64 = inv64 + (size_t)b * NP;
+        float* o32 = inv32 + (size_t)b * NP;
+        for (int r = 0; r <= i; ++r) {
+            int e = sym_index<D>(r, i);
+            o64[e] = S[r][i];
+            o32[e] = (float)S[r][i];
         }
     }
 }
 
 // ---- apply -----------------------------------------------------------------
-template <typename T>
-__global__ void k_restrict0(int M, const int* __restrict__ part_map, const double* __restrict__ r, T* __restrict__ R) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int v = part_map[s];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) R[3 * (size_t)s + k] = (v >= 0) ? (T)r[3 * (size_t)v + k] : (T)0;
-}
+// address-space-qualified pointers for the direct global -> LDS load builtin
+typedef __attribute__((address_space(1))) const void* gptr_t;
+typedef __attribute__((address_space(3))) void* lptr_t;
 
-// Aggregate level-(l-1) nodes into their parents (component sums inside each
-// bank, lowest lane of the component writes; no atomics).
-// nodes: global ids [beg, beg+cnt) (level 0: slots [0, M) with parent via
-// going_next[vertex]).
-template <typename T>
-__global__ void __launch_bounds__(kTB) k_restrict_up(int level_from, int B, const int* __restrict__ ls, int M,
-                                                      const int* __restrict__ part_map,
-                                                      const int* __restrict__ going_next, T* __restrict__ R,
-                                                      const int* __restrict__ ctrl) {
-    if (ctrl && ctrl[0]) return;
-    __shared__ int par[kTB];
-    __shared__ T sv[3][kTB];
-    int c = blockIdx.x * blockDim.x + threadIdx.x;  // local index within level
-    int lane = threadIdx.x % B, bbase = threadIdx.x - lane;
-    int cnt, beg;
-    if (level_from == 0) { cnt = M; beg = 0; }
-    else { cnt = ls[2 * level_from + 0]; beg = ls[2 * level_from + 1]; }
-    int p = -1;
-    T v0 = 0, v1 = 0, v2 = 0;
-    if (c < cnt) {
-        size_t g = (size_t)beg + c;
-        if (level_from == 0) {
-            int vert = part_map[c];
-            p = (vert >= 0) ? going_next[vert] : -1;
-        } else {
-            p = going_next[g];
-        }
-        v0 = R[3 * g]; v1 = R[3 * g + 1]; v2 = R[3 * g + 2];
-    }
-    par[threadIdx.x] = p;
-    sv[0][threadIdx.x] = v0; sv[1][threadIdx.x] = v1; sv[2][threadIdx.x] = v2;
-    __syncthreads();
-    if (p < 0) return;
-    for (int l = 0; l < lane; ++l)
-        if (par[bbase + l] == p) return;  // not the lowest lane of its component
-    T s0 = 0, s1 = 0, s2 = 0;
-    for (int l = lane; l < B; ++l)
-        if (par[bbase + l] == p) { s0 += sv[0][bbase + l]; s1 += sv[1][bbase + l]; s2 += sv[2][bbase + l]; }
-    R[3 * (size_t)p] = s0; R[3 * (size_t)p + 1] = s1; R[3 * (size_t)p + 2] = s2;
-}
-
+// z_bank = A^-1 r_bank for every bank of every level: the packed inverse is
+// staged through LDS (the bytes of this kernel are almost all inverse
+// bytes), then lane i forms row i, summing over columns in order.
 template <int B, typename T>
 __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_solve(const T* __restrict__ inv, const T* __restrict__ R,
                                                                     T* __restrict__ Z, const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0]) return;
     constexpr int D = 3 * B;
+    constexpr int NP = D * (D + 1) / 2;
+    constexpr int NT = (B == 16) ? 64 : 128;
+    constexpr int NV = NP * (int)sizeof(T) / 16;  // exact: NP * sizeof(T) is a multiple of 16 for B = 16, 32
+    static_assert(NP * sizeof(T) % 16 == 0, "packed bank must be a whole number of 16-byte vectors");
+    using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    constexpr int PER = (NV + NT - 1) / NT;
+    __shared__ V4 As[PER * NT];  // padded to whole 1 KB wave chunks
     __shared__ T rs[D];
-    int b = blockIdx.x, i = threadIdx.x;
-    if (i < D) rs[i] = R[(size_t)b * D + i];
+    const int b = blockIdx.x, i = threadIdx.x;
+    const V4* src = reinterpret_cast<const V4*>(inv + (size_t)b * NP);
+    // Direct global -> LDS loads (16 B per lane, one 1 KB chunk per wave and
+    // instruction), all in flight together; lanes past the bank re-read its
+    // last vector into the padding.
+    const int wv = __builtin_amdgcn_readfirstlane(i / 64), lane = i % 64;
+    T rv = R[(size_t)b * D + min(i, D - 1)];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q0 = k * NT + wv * 64;
+        __builtin_amdgcn_global_load_lds((gptr_t)(src + min(q0 + lane, NV - 1)), (lptr_t)(As + q0), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    if (i < D) rs[i] = rv;
     __syncthreads();
     if (i >= D) return;
-    const T* A = inv + (size_t)b * D * D;
+    const T* A = reinterpret_cast<const T*>(As);
     T acc = 0;
 #pragma unroll 8
-    for (int j = 0; j < D; ++j) acc += A[(size_t)j * D + i] * rs[j];  // symmetric: column i == row i
+    for (int j = 0; j < D; ++j) acc += A[j <= i ? sym_index<D>(j, i) : sym_index<D>(i, j)] * rs[j];
     Z[(size_t)b * D + i] = acc;
 }
 
@@ -465,6 +467,136 @@ __global__ void k_collect(int n, int L, const int* __restrict__ node_of, const T
     z[3 * (size_t)v] = (double)s0; z[3 * (size_t)v + 1] = (double)s1; z[3 * (size_t)v + 2] = (double)s2;
 }
 
+// Component sums of one bank-aligned group of kTB nodes held in LDS: the
+// lowest lane of each component (equal parent id inside its bank) sums the
+// component in lane order.  Returns true for that lane, with the sums.
+template <typename T>
+__device__ inline bool component_sum(const int* par, T (*sv)[kTB], int B, T& s0, T& s1, T& s2) {
+    int t = threadIdx.x, lane = t % B, bbase = t - lane;
+    int p = par[t];
+    if (p < 0) return false;
+    for (int l = 0; l < lane; ++l)
+        if (par[bbase + l] == p) return false;
+    s0 = 0; s1 = 0; s2 = 0;
+    for (int l = lane; l < B; ++l)
+        if (par[bbase + l] == p) { s0 += sv[0][bbase + l]; s1 += sv[1][bbase + l]; s2 += sv[2][bbase + l]; }
+    return true;
+}
+
+// Multilevel restriction R_l = sum of children, levels 0 to 2 in one launch.
+// Workgroup b owns level-1 bank b: it gathers the level-0 slots of every
+// level-0 bank holding one of its children (reading the residual r through
+// part_map, writing R_0 for the banks whose first child lands in b), forms
+// its level-1 sums, and from them the level-2 nodes (whose children always
+// lie in one level-1 bank).  Sum orders are those of the level-by-level
+// component sums, so results do not depend on the schedule.
+template <typename V, typename T>
+__global__ void __launch_bounds__(kTB) k_restrict_all(const V* __restrict__ r, long long sub3, int B, int L,
+                                                       const int* __restrict__ ls, const int* __restrict__ part_map,
+                                                       const int* __restrict__ going_next,
+                                                       const int* __restrict__ psum0, const int* __restrict__ l1lo,
+                                                       const int* __restrict__ l1hi, T* __restrict__ R,
+                                                       const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
+    __shared__ int par[kTB];
+    __shared__ T sv[3][kTB];
+    __shared__ T l1v[3][32];
+    const int t = threadIdx.x, b = blockIdx.x;
+    const int cnt1 = ls[2], beg1 = ls[3];
+    // level 0 -> 1
+    const int lo = l1lo[b] * B, hi = l1hi[b] * B;
+    for (int c0 = lo; c0 < hi; c0 += kTB) {
+        int sl = c0 + t;
+        int v = (sl < hi) ? part_map[sl] : -1;
+        int p = -1;
+        T v0 = 0, v1 = 0, v2 = 0;
+        if (v >= 0) {
+            const V* rv = r + sub3 + 3 * (size_t)v;
+            v0 = (T)rv[0]; v1 = (T)rv[1]; v2 = (T)rv[2];
+            int loc = going_next[v] - beg1;
+            if (loc / B == b) p = loc;
+            if (psum0[sl / B] / B == b) {  // this workgroup writes the bank's R_0
+                R[3 * (size_t)sl] = v0; R[3 * (size_t)sl + 1] = v1; R[3 * (size_t)sl + 2] = v2;
+            }
+        }
+        par[t] = p;
+        sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
+        __syncthreads();
+        T s0, s1, s2;
+        if (component_sum<T>(par, sv, B, s0, s1, s2)) {
+            int p2 = par[t];
+            size_t g = (size_t)beg1 + p2;
+            R[3 * g] = s0; R[3 * g + 1] = s1; R[3 * g + 2] = s2;
+            l1v[0][p2 - b * B] = s0; l1v[1][p2 - b * B] = s1; l1v[2][p2 - b * B] = s2;
+        }
+        __syncthreads();
+    }
+    if (L < 3) return;
+    // level 1 -> 2 inside this bank
+    if (t < kTB) {
+        int loc = b * B + t;
+        bool valid = t < B && loc < cnt1;
+        par[t] = valid ? going_next[(size_t)beg1 + loc] : -1;
+        if (t < B) {
+            sv[0][t] = valid ? l1v[0][t] : (T)0;
+            sv[1][t] = valid ? l1v[1][t] : (T)0;
+            sv[2][t] = valid ? l1v[2][t] : (T)0;
+        }
+    }
+    __syncthreads();
+    if (t < B) {
+        T s0, s1, s2;
+        if (component_sum<T>(par, sv, B, s0, s1, s2)) {
+            size_t g = (size_t)par[t];
+            R[3 * g] = s0; R[3 * g + 1] = s1; R[3 * g + 2] = s2;
+        }
+    }
+}
+
+// Levels 2 -> 3 -> ... -> L-1 (few nodes): one workgroup, level by level.
+// Its own global writes are read back by other threads of the workgroup
+// only after a barrier.
+template <typename T>
+__global__ void __launch_bounds__(kTB) k_restrict_top(int B, int L, const int* __restrict__ ls,
+                                                       const int* __restrict__ going_next, T* __restrict__ R,
+                                                       const int* __restrict__ ctrl) {
+    if (ctrl && ctrl[0]) return;
+    __shared__ int par[kTB];
+    __shared__ T sv[3][kTB];
+    const int t = threadIdx.x;
+    for (int l = 2; l + 1 < L; ++l) {
+        const int cnt = ls[2 * l], beg = ls[2 * l + 1];
+        for (int c0 = 0; c0 < cnt; c0 += kTB) {
+            int cc = c0 + t;
+            bool valid = cc < cnt;
+            size_t g = (size_t)beg + cc;
+            par[t] = valid ? going_next[g] : -1;
+            sv[0][t] = valid ? R[3 * g] : (T)0;
+            sv[1][t] = valid ? R[3 * g + 1] : (T)0;
+            sv[2][t] = valid ? R[3 * g + 2] : (T)0;
+            __syncthreads();
+            T s0, s1, s2;
+            if (component_sum<T>(par, sv, B, s0, s1, s2)) {
+                size_t q = (size_t)par[t];
+                R[3 * q] = s0; R[3 * q + 1] = s1; R[3 * q + 2] = s2;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// level-0 bank range of each level-1 bank (children of its nodes)
+__global__ void k_l1_ranges(int nb0, int B, const int* __restrict__ prefix0, const int* __restrict__ psum0,
+                            int* __restrict__ lo, int* __restrict__ hi) {
+    int cb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cb >= nb0 || prefix0[cb] <= 0) return;
+    int f = psum0[cb] / B, e = (psum0[cb] + prefix0[cb] - 1) / B;
+    for (int b = f; b <= e; ++b) {
+        atomicMin(lo + b, cb);
+        atomicMax(hi + b, cb + 1);
+    }
+}
+
 int num_levels(int nn, int B) {
     int nLevel = 1;
     int levelSz = (nn + B - 1) / B * B;
@@ -477,25 +609,29 @@ int num_levels(int nn, int B) {
     return nLevel > 6 ? 6 : nLevel;
 }
 
-// restriction to the coarse levels + bank solves on all levels; R level 0
-// must already hold the slot-ordered residual.  `ctrl` (device, may be
-// null) skips the work once the PCG loop has stopped.
-template <typename T>
-void mas_up_solve_t(Ctx& c, T* R, T* Z, const T* inv, const int* ctrl) {
+template <typename V, typename T>
+void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl) {
     MasState& m = c.mas;
-    hipStream_t s = c.stream;
-    for (int l = 0; l + 1 < m.L; ++l) {
-        int cnt = (l == 0) ? m.M : m.level_size_h[2 * l];
-        if (cnt <= 0) break;
-        hipLaunchKernelGGL(k_restrict_up<T>, dim3(div_up(cnt, kTB)), dim3(kTB), 0, s, l, m.bank, m.level_size.p, m.M,
-                           m.part_map.p, m.going_next.p, R, ctrl);
+    if (m.nb1 <= 0) return;
+    hipLaunchKernelGGL((k_restrict_all<V, T>), dim3(m.nb1), dim3(kTB), 0, c.stream, r, sub3, m.bank, m.L,
+                       (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
+                       (const int*)m.psum0.p, (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
+    SGIPC_LAUNCH_CHECK();
+    if (m.L >= 4) {
+        hipLaunchKernelGGL((k_restrict_top<T>), dim3(1), dim3(kTB), 0, c.stream, m.bank, m.L,
+                           (const int*)m.level_size.p, (const int*)m.going_next.p, R, ctrl);
         SGIPC_LAUNCH_CHECK();
     }
+}
+
+template <typename T>
+void mas_solve_t(Ctx& c, const T* inv, const T* R, T* Z, const int* ctrl) {
+    MasState& m = c.mas;
     prof_begin(c, PROF_MAS_SOLVE);
     if (m.bank == 16)
-        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, s, inv, R, Z, ctrl);
+        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, c.stream, inv, R, Z, ctrl);
     else
-        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, s, inv, R, Z, ctrl);
+        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, c.stream, inv, R, Z, ctrl);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_MAS_SOLVE);
 }
@@ -505,9 +641,8 @@ void mas_apply_t(Ctx& c, const double* r, double* z, T* R, T* Z, const T* inv) {
     MasState& m = c.mas;
     hipStream_t s = c.stream;
     if (m.n <= 0) return;
-    hipLaunchKernelGGL(k_restrict0<T>, dim3(div_up(m.M, 256)), dim3(256), 0, s, m.M, m.part_map.p, r, R);
-    SGIPC_LAUNCH_CHECK();
-    mas_up_solve_t<T>(c, R, Z, inv, nullptr);
+    mas_restrict_t<double, T>(c, r, 0, R, nullptr);
+    mas_solve_t<T>(c, inv, R, Z, nullptr);
     hipLaunchKernelGGL(k_collect<T>, dim3(div_up(m.n, 256)), dim3(256), 0, s, m.n, m.L, m.node_of.p, Z, z);
     SGIPC_LAUNCH_CHECK();
 }
@@ -633,8 +768,22 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
     const int npack = B * (B + 1) / 2;
     const int D = 3 * B;
     m.bank_upper.reserve((size_t)m.nbanks * npack * 9);
-    m.inv64.reserve((size_t)m.nbanks * D * D);
-    m.inv32.reserve((size_t)m.nbanks * D * D);
+    m.inv64.reserve((size_t)m.nbanks * (D * (D + 1) / 2));
+    m.inv32.reserve((size_t)m.nbanks * (D * (D + 1) / 2));
+    // level-0 bank ranges of the level-1 banks (fused restriction)
+    {
+        const int nb0 = M / B;
+        m.nb1 = div_up(m.level_size_h[2], B);
+        int* lo = m.l1lo.reserve((size_t)m.nb1 + 1);
+        int* hi = m.l1hi.reserve((size_t)m.nb1 + 1);
+        fill<int>(lo, nb0, m.nb1 + 1, s);
+        fill<int>(hi, 0, m.nb1 + 1, s);
+        if (nb0 > 0) {
+            hipLaunchKernelGGL(k_l1_ranges, dim3(div_up(nb0, 256)), dim3(256), 0, s, nb0, B, m.prefix0.p, m.psum0.p,
+                               lo, hi);
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
     fill<double>(m.bank_upper.p, 0.0, (long long)m.nbanks * npack * 9, s);
     if (c.K > 0) {
         // coarse levels whose banks fit the LDS accumulator (top levels first)
@@ -678,7 +827,17 @@ void mas_apply32(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
     mas_apply_t<float>(c, r, z, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p);
 }
-void mas_up_solve64(Ctx& c, const int* ctrl) { mas_up_solve_t<double>(c, c.mas.R64.p, c.mas.Z64.p, c.mas.inv64.p, ctrl); }
-void mas_up_solve32(Ctx& c, const int* ctrl) { mas_up_solve_t<float>(c, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p, ctrl); }
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl) {
+    if (fp32) mas_restrict_t<double, float>(c, r, sub3, c.mas.R32.p, ctrl);
+    else mas_restrict_t<double, double>(c, r, sub3, c.mas.R64.p, ctrl);
+}
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl) {
+    if (!fp32) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
+    mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl);
+}
+void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl) {
+    if (fp32) mas_solve_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
+    else mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+}
 
 }  // namespace sgipc

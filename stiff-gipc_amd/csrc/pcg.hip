@@ -11,7 +11,7 @@
 // is set.  The host only polls that flag every few iterations, so the
 // iteration count and the iterate are exactly those of the reference loop
 // (stop test on the stale rz after the x/r update).
-#include "context.hpp"
+#include "spmv_sym.hpp"
 #include <cmath>
 
 namespace sgipc {
@@ -20,8 +20,8 @@ namespace {
 enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
 enum { C_STOP = 0, C_ITER, C_MAXIT, C_NOCONV, C_NUM };  // C_STOP: iteration at which the loop stopped (0 = running)
 
-constexpr int kThreads = 256;
-constexpr int kRedBlocks = 512;   // producers of dot partials use at most this many blocks
+constexpr int kThreads = 1024;  // workgroup size of every PCG kernel (16 waves)
+constexpr int kRedBlocks = 1024;  // producers of dot partials use at most this many blocks
 constexpr int kPart = 1024;       // partial slots per dot
 
 __device__ inline double block_sum_all(double v, double* sh) {
@@ -68,58 +68,49 @@ __global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict
     if (threadIdx.x == 0) sc[idx] = s;
 }
 
-// Ap = A p (full block-CSR) with per-block partials of p.Ap.  First kernel of
-// an iteration: enforces the iteration budget.
+// Ap = A p (symmetric, upper BCOO + transposed references) with
+// per-workgroup partials of p.Ap.  First kernel of an iteration: enforces
+// the iteration budget (iteration k runs while k < max_iter).
 template <int G, typename MV, typename V>
-__global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                       const MV* __restrict__ val, int nnzb,
-                                                       const V* __restrict__ p, V* __restrict__ Ap,
-                                                       double* __restrict__ part, int* __restrict__ ctrl) {
+__global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ lptr, const int2* __restrict__ lref,
+                                                       const int* __restrict__ ustart, const int* __restrict__ ucol,
+                                                       const MV* __restrict__ val, const V* __restrict__ p,
+                                                       V* __restrict__ Ap, double* __restrict__ part,
+                                                       int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
     if (ctrl[C_ITER] >= ctrl[C_MAXIT]) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
         return;
     }
     __shared__ double sh[kThreads / 64];
+    constexpr int RPB = kThreads / G;
+    const int nchunks = (nb + RPB - 1) / RPB;
+    ChunkWalk w = xcd_chunks(nchunks);
     double contrib = 0.0;
-    for (int base = blockIdx.x * (kThreads / G); base < nb; base += gridDim.x * (kThreads / G)) {
-        int row = base + threadIdx.x / G, sub = threadIdx.x % G;
+    for (int ch = w.first; ch < w.end; ch += w.step) {
+        int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         MV s0 = 0, s1 = 0, s2 = 0;
-        if (row < nb) {
-            int e1 = rowptr[row + 1];
-            for (int e = rowptr[row] + sub; e < e1; e += G) {
-                int j = col[e];
-                MV x0 = (MV)p[3 * (size_t)j], x1 = (MV)p[3 * (size_t)j + 1], x2 = (MV)p[3 * (size_t)j + 2];
-                s0 += val[(size_t)0 * nnzb + e] * x0 + val[(size_t)3 * nnzb + e] * x1 + val[(size_t)6 * nnzb + e] * x2;
-                s1 += val[(size_t)1 * nnzb + e] * x0 + val[(size_t)4 * nnzb + e] * x1 + val[(size_t)7 * nnzb + e] * x2;
-                s2 += val[(size_t)2 * nnzb + e] * x0 + val[(size_t)5 * nnzb + e] * x1 + val[(size_t)8 * nnzb + e] * x2;
-            }
-        }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off, G);
-            s1 += __shfl_xor(s1, off, G);
-            s2 += __shfl_xor(s2, off, G);
-        }
+        if (row < nb) sym_row_lane<G, MV, V>(row, sub, lptr, lref, ustart, ucol, val, p, s0, s1, s2);
+        group_sum3<G, MV>(s0, s1, s2);
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;
-            Ap[o] = (V)s0; Ap[o + 1] = (V)s1; Ap[o + 2] = (V)s2;
-            contrib += (double)p[o] * (double)(V)s0 + (double)p[o + 1] * (double)(V)s1 + (double)p[o + 2] * (double)(V)s2;
+            V a0 = (V)s0, a1 = (V)s1, a2 = (V)s2;
+            Ap[o] = a0; Ap[o + 1] = a1; Ap[o + 2] = a2;
+            contrib += (double)p[o] * (double)a0 + (double)p[o + 1] * (double)a1 + (double)p[o + 2] * (double)a2;
         }
     }
     write_partial(contrib, part, sh);
 }
 
-// alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; partials of r.r ;
-// optionally scatter r into the MAS level-0 slots (fused restriction).
-template <typename V, typename TR, bool FUSE>
+// alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; partials of r.r ; then
+// the iteration counter advances (read again only by the next iteration's
+// kernels and the stop tests, which report ctrl[C_ITER] - 1).
+template <typename V>
 __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __restrict__ x, V* __restrict__ r,
                                                         const V* __restrict__ p, const V* __restrict__ Ap,
                                                         const double* __restrict__ part_pap, int np_pap,
                                                         const double* __restrict__ part_rz, int np_rz,
-                                                        double* __restrict__ part_rr, const int* __restrict__ ctrl,
-                                                        long long sub3, int n_mas, const int* __restrict__ real_map,
-                                                        TR* __restrict__ R0) {
+                                                        double* __restrict__ part_rr, int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double rz = sum_partials(part_rz, np_rz, sh);
@@ -131,12 +122,9 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
         V ri = (V)((double)r[i] - al * (double)Ap[i]);
         r[i] = ri;
         v += (double)ri * (double)ri;
-        if (FUSE) {
-            long long j = i - sub3;
-            if (j >= 0 && j < 3LL * n_mas) R0[3 * (size_t)real_map[j / 3] + (j % 3)] = (TR)ri;
-        }
     }
     write_partial(v, part_rr, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_ITER] += 1;
 }
 
 // Stop tests, evaluated identically by every block of the kernel that
@@ -153,15 +141,15 @@ __device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_pr
         double rr = sum_partials(part_rr, np, sh);
         stop = sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB]);
     }
-    if (stop && blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_STOP] = ctrl[C_ITER];
+    if (stop && blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_STOP] = ctrl[C_ITER] - 1;
     return stop;
 }
 
 // Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range (identity
-// elsewhere) and partials of r.z.
+// elsewhere) and partials of r.z; one thread per block row.
 template <typename V, typename T>
-__global__ void __launch_bounds__(kThreads) k_collect_dot(long long N, const V* __restrict__ r, V* __restrict__ z,
-                                                          long long sub3, int n_mas, int L, const int* __restrict__ node_of,
+__global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __restrict__ r, V* __restrict__ z,
+                                                          int sub_off, int n_mas, int L, const int* __restrict__ node_of,
                                                           const T* __restrict__ Z, double* __restrict__ part_rz,
                                                           const double* __restrict__ part_rz_prev,
                                                           const double* __restrict__ part_rr, int np,
@@ -171,19 +159,23 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(long long N, const V* 
     __shared__ double sh[kThreads / 64];
     if (stop_test(rule, part_rz_prev, part_rr, np, sc, ctrl, sh)) return;
     double v = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-        long long j = i - sub3;
-        V zi;
-        if (j >= 0 && j < 3LL * n_mas) {
-            int vt = (int)(j / 3), k = (int)(j % 3);
-            T s = 0;
-            for (int l = 0; l < L; ++l) s += Z[3 * (size_t)node_of[(size_t)l * n_mas + vt] + k];
-            zi = (V)s;
+    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nb; row += gridDim.x * blockDim.x) {
+        size_t o = 3 * (size_t)row;
+        V r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
+        V z0, z1, z2;
+        int vt = row - sub_off;
+        if (vt >= 0 && vt < n_mas) {
+            T s0 = 0, s1 = 0, s2 = 0;
+            for (int l = 0; l < L; ++l) {
+                size_t g = 3 * (size_t)node_of[(size_t)l * n_mas + vt];
+                s0 += Z[g]; s1 += Z[g + 1]; s2 += Z[g + 2];
+            }
+            z0 = (V)s0; z1 = (V)s1; z2 = (V)s2;
         } else {
-            zi = r[i];
+            z0 = r0; z1 = r1; z2 = r2;
         }
-        z[i] = zi;
-        v += (double)r[i] * (double)zi;
+        z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+        v += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
     }
     write_partial(v, part_rz, sh);
 }
@@ -228,7 +220,6 @@ __global__ void __launch_bounds__(kThreads) k_update_p(long long N, V* __restric
     double be = rn / ro;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
         p[i] = (V)((double)z[i] + be * (double)p[i]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_ITER] += 1;
 }
 
 template <typename TD, typename TS>
@@ -240,16 +231,6 @@ __global__ void k_copy_cast(long long N, TD* __restrict__ dst, const TS* __restr
 __global__ void k_axpy(long long N, double* __restrict__ y, double a, const double* __restrict__ x) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
         y[i] += a * x[i];
-}
-
-template <typename V, typename T>
-__global__ void k_restrict0_v(int M, const int* __restrict__ part_map, const V* __restrict__ r, long long sub3,
-                              T* __restrict__ R) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int v = part_map[s];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) R[3 * (size_t)s + k] = (v >= 0) ? (T)r[sub3 + 3 * (size_t)v + k] : (T)0;
 }
 
 __global__ void k_ctrl_init(int* ctrl, int maxit) {
@@ -300,36 +281,43 @@ struct Engine {
     int maxit = 0;
 
     static constexpr int G = 8;
+    static constexpr bool kFp32Mas = sizeof(T) == 4;
+    // every producer of a dot's partials (update_xr, collect, simple
+    // precond) runs gN workgroups, so consumers always sum gN partials
+    int gR = 1;
+
+    // preconditioner apply fused with the r.z partials (and the stop test)
+    void precond_dot(double* rz_out, const double* rz_prev, int rule_now) {
+        hipStream_t s = c.stream;
+        if (mas) {
+            mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl);
+            mas_bank_solve(c, kFp32Mas, ctrl);
+            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gR), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
+                               c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, rz_out, rz_prev, part_rr, gN, sc, ctrl,
+                               rule_now);
+            SGIPC_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_out,
+                               rz_prev, part_rr, gN, sc, ctrl, rule_now);
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
 
     // kernels of iteration k with parity par = k & 1
     void enqueue_iteration(int par) {
         hipStream_t s = c.stream;
-        const Bsr& B = c.bsr;
+        const SymBcoo& S = c.bsr;
         double* rz_prev = part_rz[par ^ 1];
         double* rz_cur = part_rz[par];
         prof_begin(c, PROF_SPMV);
-        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, B.nb, B.rowptr.p, B.col.p, val,
-                           B.nnzb, p, Ap, part_pap, ctrl);
+        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.lptr.p, S.lref.p,
+                           c.uniq_start.p, c.t_cols, val, p, Ap, part_pap, ctrl);
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
-        const long long sub3 = 3LL * c.sub_offset;
-        if (mas) {
-            hipLaunchKernelGGL((k_update_xr<V, T, true>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
-                               rz_prev, gN, part_rr, ctrl, sub3, c.mas.n, c.mas.real_map.p, R);
-            SGIPC_LAUNCH_CHECK();
-            if (sizeof(T) == 8) mas_up_solve64(c, ctrl);
-            else mas_up_solve32(c, ctrl);
-            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, N, r, z, sub3, c.mas.n, c.mas.L,
-                               c.mas.node_of.p, Z, rz_cur, rz_prev, part_rr, gN, sc, ctrl, rule);
-            SGIPC_LAUNCH_CHECK();
-        } else {
-            hipLaunchKernelGGL((k_update_xr<V, T, false>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS,
-                               rz_prev, gN, part_rr, ctrl, sub3, 0, (const int*)nullptr, (T*)nullptr);
-            SGIPC_LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_cur, rz_prev,
-                               part_rr, gN, sc, ctrl, rule);
-            SGIPC_LAUNCH_CHECK();
-        }
+        hipLaunchKernelGGL((k_update_xr<V>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS, rz_prev, gN,
+                           part_rr, ctrl);
+        SGIPC_LAUNCH_CHECK();
+        precond_dot(rz_cur, rz_prev, rule);
         hipLaunchKernelGGL((k_update_p<V>), dim3(gN), dim3(kThreads), 0, s, N, p, z, rz_cur, rz_prev, gN, ctrl);
         SGIPC_LAUNCH_CHECK();
     }
@@ -339,6 +327,7 @@ struct Engine {
     double init(double tol, int max_iter) {
         hipStream_t s = c.stream;
         maxit = max_iter;
+        gR = gN;
         hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, max_iter);
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL, tol);
@@ -348,21 +337,7 @@ struct Engine {
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
         SGIPC_LAUNCH_CHECK();
-        const long long sub3 = 3LL * c.sub_offset;
-        if (mas) {
-            hipLaunchKernelGGL((k_restrict0_v<V, T>), dim3(div_up(c.mas.M, 256)), dim3(256), 0, s, c.mas.M,
-                               c.mas.part_map.p, r, sub3, R);
-            SGIPC_LAUNCH_CHECK();
-            if (sizeof(T) == 8) mas_up_solve64(c, ctrl);
-            else mas_up_solve32(c, ctrl);
-            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, N, r, z, sub3, c.mas.n, c.mas.L,
-                               c.mas.node_of.p, Z, part_rz[0], part_rz[1], part_rr, gN, sc, ctrl, -1);
-            SGIPC_LAUNCH_CHECK();
-        } else {
-            hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, part_rz[0],
-                               part_rz[1], part_rr, gN, sc, ctrl, -1);
-            SGIPC_LAUNCH_CHECK();
-        }
+        precond_dot(part_rz[0], part_rz[1], -1);
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_copy_cast<V, V>), dim3(gN), dim3(kThreads), 0, s, N, p, z);
@@ -376,18 +351,19 @@ struct Engine {
     size_t graph_key() const {
         size_t h = 1469598103934665603ull;
         auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
-        const Bsr& B = c.bsr;
-        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)B.nb); mix((size_t)B.nnzb);
-        mix((size_t)B.rowptr.p); mix((size_t)B.col.p); mix((size_t)val); mix((size_t)r); mix((size_t)z);
+        const SymBcoo& S = c.bsr;
+        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)gR); mix((size_t)S.nb); mix((size_t)S.K);
+        mix((size_t)S.nlow); mix((size_t)S.lptr.p); mix((size_t)S.lref.p); mix((size_t)c.uniq_start.p);
+        mix((size_t)c.t_cols); mix((size_t)val); mix((size_t)r); mix((size_t)z);
         mix((size_t)p); mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc); mix((size_t)ctrl);
         mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
-        mix((size_t)c.sub_offset); mix((size_t)c.stream);
+        mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
         if (mas) {
             const MasState& m = c.mas;
-            mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.going_next.p);
-            mix((size_t)m.node_of.p); mix((size_t)m.level_size.p); mix((size_t)m.part_map.p); mix((size_t)m.real_map.p);
+            mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
+            mix((size_t)m.nb1); mix((size_t)m.going_next.p); mix((size_t)m.node_of.p); mix((size_t)m.level_size.p);
+            mix((size_t)m.part_map.p); mix((size_t)m.psum0.p); mix((size_t)m.l1lo.p); mix((size_t)m.l1hi.p);
             mix((size_t)m.inv64.p); mix((size_t)m.inv32.p);
-            for (int l = 0; l < m.L; ++l) mix((size_t)m.level_size_h[2 * l]);
         }
         return h;
     }
@@ -506,8 +482,9 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
     int* ctrl = w.ctrl.reserve(C_NUM + 8);
     const int max_iter = o.max_iter > 0 ? o.max_iter : (int)(o.max_iter_ratio * (double)N);
     const int gN = grid_for(N);
-    const Bsr& B = c.bsr;
-    const int gS = std::max(1, std::min(kPart, div_up((long long)B.nb * 8, kThreads)));
+    // SpMV: chunks of kThreads / 8 block rows, a multiple of 8 workgroups (one range per XCD)
+    const int nchunks = (int)div_up((long long)c.nb, kThreads / 8);
+    const int gS = std::min(kPart, std::max(8, (int)div_up(nchunks, 8) * 8));
     st.refinements = 0;
     if (o.precision == 0) {
         auto run64 = [&](auto tag) {
@@ -516,7 +493,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
             Engine<double, double, T> E{c, N, gN, gS,
                                         w.r.reserve(N + 1), w.z.reserve(N + 1), w.p.reserve(N + 1), w.Ap.reserve(N + 1),
                                         x, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
-                                        parts + 4 * kPart, sc, ctrl, B.val.p, mas,
+                                        parts + 4 * kPart, sc, ctrl, (const double*)c.t_vals, mas,
                                         mas ? (T*)(fp32mas ? (void*)c.mas.R32.p : (void*)c.mas.R64.p) : nullptr,
                                         mas ? (T*)(fp32mas ? (void*)c.mas.Z32.p : (void*)c.mas.Z64.p) : nullptr,
                                         c.precond_kind == 1 ? w.Dinv.p : nullptr, o.stop_rule};
@@ -546,7 +523,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
         Engine<float, float, float> E{c, N, gN, gS,
                                       rf, w.zf.reserve(N + 1), w.pf.reserve(N + 1), w.Apf.reserve(N + 1),
                                       d64, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
-                                      parts + 4 * kPart, sc, ctrl, B.valf.p, mas,
+                                      parts + 4 * kPart, sc, ctrl, bsr_values_f32(c), mas,
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
                                       c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
         fill<double>(x, 0.0, N, s);

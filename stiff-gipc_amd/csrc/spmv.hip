@@ -1,62 +1,139 @@
-// Symmetric block-3x3 SpMV  y = a A x + b y  on the full block-CSR built from
-// the upper BCOO (both triangles stored, values SoA by component).
+// Symmetric block-3x3 SpMV  y = a A x + b y  on the upper BCOO resident in
+// the triplet view, with the transposed-reference structure built here.
 //
-// Reference: S/linear_system/utils/spmv.cu:8-133 (upper BCOO, warp
-// head-segmented row sums, FP64 atomics for the transposed half).  Here each
-// row is owned by a group of G lanes: a gather over the row's blocks, a
-// shuffle reduction inside the group, one store per row -- no atomics,
-// deterministic, coalesced SoA value streams.
-#include "context.hpp"
+// Reference: S/linear_system/utils/spmv.cu:8-133.  The transposed half is
+// gathered per row from a list of references to upper blocks instead of
+// scattered with atomics, so each row is written once and the sum order is
+// fixed.
+#include "spmv_sym.hpp"
 
 namespace sgipc {
+namespace {
 
 template <int G, typename TV, typename TX, typename TY>
-__global__ void __launch_bounds__(256) k_spmv_bsr(int nb, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ col, const TV* __restrict__ val,
-                                                  int nnzb, double a, const TX* __restrict__ x, double b,
-                                                  TY* __restrict__ y) {
-    int gid = (blockIdx.x * blockDim.x + threadIdx.x);
-    int row = gid / G, sub = gid % G;
-    if (row >= nb) return;  // G divides 64, so whole groups leave together
-    int e0 = rowptr[row], e1 = rowptr[row + 1];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int e = e0 + sub; e < e1; e += G) {
-        int j = col[e];
-        double x0 = (double)x[3 * (size_t)j], x1 = (double)x[3 * (size_t)j + 1], x2 = (double)x[3 * (size_t)j + 2];
-        // val[(p + 3q) * nnzb + e] = A(p,q)
-        s0 += (double)val[(size_t)0 * nnzb + e] * x0 + (double)val[(size_t)3 * nnzb + e] * x1 +
-              (double)val[(size_t)6 * nnzb + e] * x2;
-        s1 += (double)val[(size_t)1 * nnzb + e] * x0 + (double)val[(size_t)4 * nnzb + e] * x1 +
-              (double)val[(size_t)7 * nnzb + e] * x2;
-        s2 += (double)val[(size_t)2 * nnzb + e] * x0 + (double)val[(size_t)5 * nnzb + e] * x1 +
-              (double)val[(size_t)8 * nnzb + e] * x2;
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, G);
-        s1 += __shfl_xor(s1, off, G);
-        s2 += __shfl_xor(s2, off, G);
-    }
-    if (sub == 0) {
-        size_t o = 3 * (size_t)row;
-        if (b == 0.0) {
-            y[o] = (TY)(a * s0); y[o + 1] = (TY)(a * s1); y[o + 2] = (TY)(a * s2);
-        } else {
-            y[o] = (TY)(a * s0 + b * (double)y[o]);
-            y[o + 1] = (TY)(a * s1 + b * (double)y[o + 1]);
-            y[o + 2] = (TY)(a * s2 + b * (double)y[o + 2]);
+__global__ void __launch_bounds__(256) k_spmv_sym(int nb, const int* __restrict__ lptr, const int2* __restrict__ lref,
+                                                  const int* __restrict__ ustart, const int* __restrict__ ucol,
+                                                  const TV* __restrict__ vals, double a, const TX* __restrict__ x,
+                                                  double b, TY* __restrict__ y) {
+    constexpr int RPB = 256 / G;
+    const int nchunks = (nb + RPB - 1) / RPB;
+    ChunkWalk w = xcd_chunks(nchunks);
+    for (int ch = w.first; ch < w.end; ch += w.step) {
+        int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
+        TV s0 = 0, s1 = 0, s2 = 0;
+        if (row < nb) sym_row_lane<G, TV, TX>(row, sub, lptr, lref, ustart, ucol, vals, x, s0, s1, s2);
+        group_sum3<G, TV>(s0, s1, s2);
+        if (sub == 0 && row < nb) {
+            size_t o = 3 * (size_t)row;
+            if (b == 0.0) {
+                y[o] = (TY)(a * s0); y[o + 1] = (TY)(a * s1); y[o + 2] = (TY)(a * s2);
+            } else {
+                y[o] = (TY)(a * s0 + b * (double)y[o]);
+                y[o + 1] = (TY)(a * s1 + b * (double)y[o + 1]);
+                y[o + 2] = (TY)(a * s2 + b * (double)y[o + 2]);
+            }
         }
     }
 }
 
+// lower-reference counts per row: block (i, j), i < j, is referenced by row j
+__global__ void k_low_count(int K, const int* __restrict__ r, const int* __restrict__ c, int* __restrict__ cnt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (r[k] != c[k]) atomicAdd(cnt + c[k], 1);
+}
+
+__global__ void k_low_place(int K, const int* __restrict__ r, const int* __restrict__ c, const int* __restrict__ lptr,
+                            int* __restrict__ cursor, int2* __restrict__ lref) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    int i = r[k], j = c[k];
+    if (i == j) return;
+    int pos = lptr[j] + atomicAdd(cursor + j, 1);
+    lref[pos] = make_int2(k, i);
+}
+
+// order each row's references by block index (= by row i): fixed sum order
+__global__ void k_low_sort(int nb, const int* __restrict__ lptr, int2* __restrict__ lref) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    int b = lptr[j], e = lptr[j + 1];
+    for (int a = b + 1; a < e; ++a) {
+        int2 v = lref[a];
+        int q = a - 1;
+        while (q >= b && lref[q].x > v.x) {
+            lref[q + 1] = lref[q];
+            --q;
+        }
+        lref[q + 1] = v;
+    }
+}
+
+__global__ void k_to_f32(long long n, const double* __restrict__ s, float* __restrict__ d) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
+}
+
+}  // namespace
+
+// Build the transposed-reference structure for the BCOO now resident in the
+// triplet view (rows/cols/vals [0, K)); uniq_start holds the row starts.
+void bsr_build(Ctx& c) {
+    if (!c.have_bcoo) throw state_error("bsr_build: convert first");
+    hipStream_t s = c.stream;
+    int nb = c.nb, K = c.K;
+    SymBcoo& S = c.bsr;
+    int* cnt = c.row_count.reserve((size_t)nb + 1);
+    int* lptr = S.lptr.reserve((size_t)nb + 2);
+    int* dint = c.dev_int.reserve(8);
+    fill<int>(cnt, 0, nb, s);
+    if (K > 0) {
+        hipLaunchKernelGGL(k_low_count, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, cnt);
+        SGIPC_LAUNCH_CHECK();
+    }
+    exclusive_scan(c, cnt, lptr, nb, dint + 4, s);
+    int nlow = 0;
+    SGIPC_HIP(hipMemcpyAsync(&nlow, dint + 4, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    S.nb = nb;
+    S.K = K;
+    S.nlow = nlow;
+    int2* lref = S.lref.reserve((size_t)nlow + 1);
+    fill<int>(cnt, 0, nb, s);
+    if (K > 0) {
+        hipLaunchKernelGGL(k_low_place, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, lptr, cnt,
+                           lref);
+        SGIPC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_low_sort, dim3(div_up(nb, 256)), dim3(256), 0, s, nb, lptr, lref);
+        SGIPC_LAUNCH_CHECK();
+    }
+    S.valf_ok = false;
+    c.have_bsr = true;
+}
+
+// FP32 copy of the BCOO values for the mixed-precision solve
+const float* bsr_values_f32(Ctx& c) {
+    if (!c.have_bsr) bsr_build(c);
+    SymBcoo& S = c.bsr;
+    float* vf = S.valf.reserve(9 * (size_t)S.K + 1);
+    if (!S.valf_ok && S.K > 0) {
+        long long n = 9LL * S.K;
+        int g = (int)std::min<long long>(div_up(n, 256), 8192);
+        hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, c.stream, n, (const double*)c.t_vals, vf);
+        SGIPC_LAUNCH_CHECK();
+    }
+    S.valf_ok = true;
+    return vf;
+}
+
 void spmv64(Ctx& c, double a, const double* x, double b, double* y) {
     if (!c.have_bsr) bsr_build(c);
-    const Bsr& B = c.bsr;
+    const SymBcoo& S = c.bsr;
     constexpr int G = 8;
-    int threads = 256;
-    int blocks = div_up((long long)B.nb * G, threads);
-    hipLaunchKernelGGL((k_spmv_bsr<G, double, double, double>), dim3(blocks), dim3(threads), 0, c.stream, B.nb,
-                       B.rowptr.p, B.col.p, B.val.p, B.nnzb, a, x, b, y);
+    int nchunks = div_up((long long)S.nb, 256 / G);
+    int blocks = std::max(8, (int)div_up(nchunks, 8) * 8);
+    hipLaunchKernelGGL((k_spmv_sym<G, double, double, double>), dim3(blocks), dim3(256), 0, c.stream, S.nb, S.lptr.p,
+                       S.lref.p, c.uniq_start.p, c.t_cols, (const double*)c.t_vals, a, x, b, y);
     SGIPC_LAUNCH_CHECK();
 }
 
