@@ -39,6 +39,7 @@ struct MasState {
     DBuf<int> level_size;  // 2*(L+2)
     DBuf<int> coarse_space;  // L*n
     DBuf<int> going_next;    // cap
+    DBuf<unsigned> comp_mask; // cap: bank lanes of each coarse node's component (levels >= 1)
     size_t going_cap = 0;
     DBuf<int> coarse_table;  // 6*n (itable layout)
     DBuf<int> node_of;       // L*n : global node id of vertex at level l (l=0: slot)
@@ -155,6 +156,13 @@ void mas_apply32(Ctx& c, const double* r, double* z);
 void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl);
 void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl);
 void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl);
+// PCG x/r update fused with the restriction of the new residual (levels
+// 0..L-1); g1 + gx workgroups, each writing one r.r partial
+void mas_update_layout(const Ctx& c, int& g1, int& gx);
+void mas_update_restrict(Ctx& c, double* x, double* r, const double* p, const double* Ap, const double* part_pap,
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32);
+void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const float* Ap, const double* part_pap,
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32);
 void diag_build(Ctx& c);
 
 void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,

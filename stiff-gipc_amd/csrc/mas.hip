@@ -15,6 +15,7 @@
 // a deterministic per-bank component sum (no FP atomics); the apply is a
 // bank-per-workgroup dense GEMV on the symmetric inverse.
 #include "context.hpp"
+#include "pcg_dev.hpp"
 #include <algorithm>
 
 namespace sgipc {
@@ -216,7 +217,8 @@ __global__ void k_lx_masks(int n, int B, int level, const unsigned* __restrict__
 }
 
 __global__ void __launch_bounds__(kTB) k_lx_closure(int nmax, int B, int level, const int* __restrict__ ls,
-                                                     unsigned* __restrict__ next_mask, int* __restrict__ next_prefix) {
+                                                     unsigned* __restrict__ next_mask, int* __restrict__ next_prefix,
+                                                     unsigned* __restrict__ comp_mask) {
     __shared__ unsigned sm[kTB];
     __shared__ unsigned sc[kTB];
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,7 +231,10 @@ __global__ void __launch_bounds__(kTB) k_lx_closure(int nmax, int B, int level, 
     if (valid) cl = closure(sm[threadIdx.x], lane, sm + bbase);
     sc[threadIdx.x] = (valid && (cl & ltmask(lane)) == 0u) ? 1u : 0u;
     __syncthreads();
-    if (valid) next_mask[c] = cl;
+    if (valid) {
+        next_mask[c] = cl;
+        comp_mask[(size_t)ls[2 * level + 1] + c] = cl;  // component lanes of global node beg + c
+    }
     if (lane == 0 && c < nmax) {
         int cnt = 0;
         for (int l = 0; l < B; ++l) cnt += (int)sc[bbase + l];
@@ -467,20 +472,79 @@ __global__ void k_collect(int n, int L, const int* __restrict__ node_of, const T
     z[3 * (size_t)v] = (double)s0; z[3 * (size_t)v + 1] = (double)s1; z[3 * (size_t)v + 2] = (double)s2;
 }
 
-// Component sums of one bank-aligned group of kTB nodes held in LDS: the
-// lowest lane of each component (equal parent id inside its bank) sums the
-// component in lane order.  Returns true for that lane, with the sums.
+// Component sum of one bank-aligned group of nodes held in LDS (one node
+// per thread).  `mask` holds the bank lanes of this node's component (its
+// closure, self included); only the lowest lane of the component sums, in
+// ascending lane order.  Returns true for that lane, with the sums.
 template <typename T>
-__device__ inline bool component_sum(const int* par, T (*sv)[kTB], int B, T& s0, T& s1, T& s2) {
+__device__ inline bool component_sum(unsigned mask, const T* a0, const T* a1, const T* a2, int B, T& s0, T& s1,
+                                     T& s2) {
     int t = threadIdx.x, lane = t % B, bbase = t - lane;
-    int p = par[t];
-    if (p < 0) return false;
-    for (int l = 0; l < lane; ++l)
-        if (par[bbase + l] == p) return false;
+    if (mask & ltmask(lane)) return false;
     s0 = 0; s1 = 0; s2 = 0;
-    for (int l = lane; l < B; ++l)
-        if (par[bbase + l] == p) { s0 += sv[0][bbase + l]; s1 += sv[1][bbase + l]; s2 += sv[2][bbase + l]; }
+    for (unsigned m = mask; m; m &= m - 1u) {
+        int l = lowbit(m);
+        s0 += a0[bbase + l]; s1 += a1[bbase + l]; s2 += a2[bbase + l];
+    }
     return true;
+}
+
+// level-0 slots -> level-1 sums of the nodes of level-1 bank b, then
+// level 1 -> 2 inside that bank.  `value(v, sl, own, ...)` gives the
+// level-0 value of vertex v at slot sl (own: this workgroup writes the
+// slot's bank).  fine_mask / comp_mask: component lanes of each level-0
+// vertex / coarse node.
+template <typename T, typename F>
+__device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, const int* __restrict__ part_map,
+                                     const int* __restrict__ going_next, const unsigned* __restrict__ fine_mask,
+                                     const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
+                                     const int* __restrict__ l1lo, const int* __restrict__ l1hi, T* __restrict__ R,
+                                     T (*sv)[kTB], T (*l1v)[32], F value) {
+    const int t = threadIdx.x;
+    const int lo = l1lo[b] * B, hi = l1hi[b] * B;
+    for (int c0 = lo; c0 < hi; c0 += kTB) {
+        int sl = c0 + t;
+        int v = (sl < hi) ? part_map[sl] : -1;
+        int p = -1;
+        unsigned mask = 0u;
+        T v0 = 0, v1 = 0, v2 = 0;
+        if (v >= 0) {
+            bool own = psum0[sl / B] / B == b;
+            value(v, sl, own, v0, v1, v2);
+            if (own) { R[3 * (size_t)sl] = v0; R[3 * (size_t)sl + 1] = v1; R[3 * (size_t)sl + 2] = v2; }
+            int loc = going_next[v] - beg1;
+            if (loc / B == b) { p = loc; mask = fine_mask[v]; }
+        }
+        sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
+        __syncthreads();
+        T s0, s1, s2;
+        if (p >= 0 && component_sum<T>(mask, sv[0], sv[1], sv[2], B, s0, s1, s2)) {
+            size_t g = (size_t)beg1 + p;
+            R[3 * g] = s0; R[3 * g + 1] = s1; R[3 * g + 2] = s2;
+            l1v[0][p - b * B] = s0; l1v[1][p - b * B] = s1; l1v[2][p - b * B] = s2;
+        }
+        __syncthreads();
+    }
+    if (L < 3) return;
+    int loc = b * B + t;
+    bool valid = t < B && loc < cnt1;
+    int q = -1;
+    unsigned mask = 0u;
+    if (valid) {
+        q = going_next[(size_t)beg1 + loc];
+        mask = comp_mask[(size_t)beg1 + loc];
+    }
+    if (t < B) {
+        sv[0][t] = valid ? l1v[0][t] : (T)0;
+        sv[1][t] = valid ? l1v[1][t] : (T)0;
+        sv[2][t] = valid ? l1v[2][t] : (T)0;
+    }
+    __syncthreads();
+    T s0, s1, s2;
+    if (valid && component_sum<T>(mask, sv[0], sv[1], sv[2], B, s0, s1, s2)) {
+        R[3 * (size_t)q] = s0; R[3 * (size_t)q + 1] = s1; R[3 * (size_t)q + 2] = s2;
+    }
+    __syncthreads();
 }
 
 // Multilevel restriction R_l = sum of children, levels 0 to 2 in one launch.
@@ -494,94 +558,118 @@ template <typename V, typename T>
 __global__ void __launch_bounds__(kTB) k_restrict_all(const V* __restrict__ r, long long sub3, int B, int L,
                                                        const int* __restrict__ ls, const int* __restrict__ part_map,
                                                        const int* __restrict__ going_next,
+                                                       const unsigned* __restrict__ fine_mask,
+                                                       const unsigned* __restrict__ comp_mask,
                                                        const int* __restrict__ psum0, const int* __restrict__ l1lo,
                                                        const int* __restrict__ l1hi, T* __restrict__ R,
                                                        const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0]) return;
-    __shared__ int par[kTB];
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
-    const int t = threadIdx.x, b = blockIdx.x;
-    const int cnt1 = ls[2], beg1 = ls[3];
-    // level 0 -> 1
-    const int lo = l1lo[b] * B, hi = l1hi[b] * B;
-    for (int c0 = lo; c0 < hi; c0 += kTB) {
-        int sl = c0 + t;
-        int v = (sl < hi) ? part_map[sl] : -1;
-        int p = -1;
-        T v0 = 0, v1 = 0, v2 = 0;
-        if (v >= 0) {
-            const V* rv = r + sub3 + 3 * (size_t)v;
-            v0 = (T)rv[0]; v1 = (T)rv[1]; v2 = (T)rv[2];
-            int loc = going_next[v] - beg1;
-            if (loc / B == b) p = loc;
-            if (psum0[sl / B] / B == b) {  // this workgroup writes the bank's R_0
-                R[3 * (size_t)sl] = v0; R[3 * (size_t)sl + 1] = v1; R[3 * (size_t)sl + 2] = v2;
+    auto value = [&](int v, int, bool, T& v0, T& v1, T& v2) {
+        const V* rv = r + sub3 + 3 * (size_t)v;
+        v0 = (T)rv[0]; v1 = (T)rv[1]; v2 = (T)rv[2];
+    };
+    restrict_bank<T>(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, fine_mask, comp_mask, psum0, l1lo, l1hi, R,
+                     sv, l1v, value);
+}
+
+// One PCG step fused with the restriction of its new residual:
+// alpha = rz / p.Ap, x += alpha p, r -= alpha Ap, partials of r.r, and
+// R_0..R_2 of the new r.  Workgroups [0, g1) walk the level-1 banks: each
+// recomputes r for every vertex of its banks (identical arithmetic wherever
+// a level-0 bank straddles two level-1 banks) and stores x, r and R_0 only
+// for the banks it owns; workgroups [g1, grid) update the dofs outside the
+// preconditioned range.  The iteration counter advances at the end.
+template <typename V, typename T>
+__global__ void __launch_bounds__(kTB) k_update_restrict(
+    long long N, double* __restrict__ x, V* __restrict__ r, const V* __restrict__ p, const V* __restrict__ Ap,
+    const double* __restrict__ part_pap, int np_pap, const double* __restrict__ part_rz, int np_rz,
+    double* __restrict__ part_rr, int* __restrict__ ctrl, int sub_off, int n_mas, int B, int L,
+    const int* __restrict__ ls, const int* __restrict__ part_map, const int* __restrict__ going_next,
+    const unsigned* __restrict__ fine_mask, const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
+    const int* __restrict__ l1lo, const int* __restrict__ l1hi, int nb1, int g1, T* __restrict__ R) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double sh[kTB / 64];
+    __shared__ T sv[3][kTB];
+    __shared__ T l1v[3][32];
+    const double rz = sum_partials(part_rz, np_rz, sh);
+    const double pap = sum_partials(part_pap, np_pap, sh);
+    const double al = rz / pap;
+    double acc = 0.0;
+    if ((int)blockIdx.x < g1) {
+        const int cnt1 = ls[2], beg1 = ls[3];
+        auto value = [&](int v, int, bool own, T& v0, T& v1, T& v2) {
+            const size_t o = 3 * ((size_t)sub_off + v);
+            V r0 = (V)((double)r[o] - al * (double)Ap[o]);
+            V r1 = (V)((double)r[o + 1] - al * (double)Ap[o + 1]);
+            V r2 = (V)((double)r[o + 2] - al * (double)Ap[o + 2]);
+            if (own) {
+                x[o] += al * (double)p[o]; x[o + 1] += al * (double)p[o + 1]; x[o + 2] += al * (double)p[o + 2];
+                r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
+                acc += (double)r0 * (double)r0 + (double)r1 * (double)r1 + (double)r2 * (double)r2;
             }
-        }
-        par[t] = p;
-        sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
-        __syncthreads();
-        T s0, s1, s2;
-        if (component_sum<T>(par, sv, B, s0, s1, s2)) {
-            int p2 = par[t];
-            size_t g = (size_t)beg1 + p2;
-            R[3 * g] = s0; R[3 * g + 1] = s1; R[3 * g + 2] = s2;
-            l1v[0][p2 - b * B] = s0; l1v[1][p2 - b * B] = s1; l1v[2][p2 - b * B] = s2;
-        }
-        __syncthreads();
-    }
-    if (L < 3) return;
-    // level 1 -> 2 inside this bank
-    if (t < kTB) {
-        int loc = b * B + t;
-        bool valid = t < B && loc < cnt1;
-        par[t] = valid ? going_next[(size_t)beg1 + loc] : -1;
-        if (t < B) {
-            sv[0][t] = valid ? l1v[0][t] : (T)0;
-            sv[1][t] = valid ? l1v[1][t] : (T)0;
-            sv[2][t] = valid ? l1v[2][t] : (T)0;
+            v0 = (T)r0; v1 = (T)r1; v2 = (T)r2;
+        };
+        for (int b = blockIdx.x; b < nb1; b += g1)
+            restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask, psum0, l1lo, l1hi, R,
+                             sv, l1v, value);
+    } else {
+        const long long m0 = 3LL * sub_off, m1 = 3LL * ((long long)sub_off + n_mas), nout = N - (m1 - m0);
+        for (long long q = (long long)(blockIdx.x - g1) * kTB + threadIdx.x; q < nout;
+             q += (long long)(gridDim.x - g1) * kTB) {
+            long long i = q < m0 ? q : q + (m1 - m0);
+            x[i] += al * (double)p[i];
+            V ri = (V)((double)r[i] - al * (double)Ap[i]);
+            r[i] = ri;
+            acc += (double)ri * (double)ri;
         }
     }
-    __syncthreads();
-    if (t < B) {
-        T s0, s1, s2;
-        if (component_sum<T>(par, sv, B, s0, s1, s2)) {
-            size_t g = (size_t)par[t];
-            R[3 * g] = s0; R[3 * g + 1] = s1; R[3 * g + 2] = s2;
-        }
-    }
+    write_partial(acc, part_rr, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_ITER] += 1;
 }
 
 // Levels 2 -> 3 -> ... -> L-1 (few nodes): one workgroup, level by level.
-// Its own global writes are read back by other threads of the workgroup
-// only after a barrier.
+// A level's sums stay in LDS as the next level's input when that level has
+// at most kTop nodes; otherwise they are read back from R (written by this
+// workgroup before a barrier).
+constexpr int kTop = 1024;
 template <typename T>
-__global__ void __launch_bounds__(kTB) k_restrict_top(int B, int L, const int* __restrict__ ls,
-                                                       const int* __restrict__ going_next, T* __restrict__ R,
-                                                       const int* __restrict__ ctrl) {
+__global__ void __launch_bounds__(kTop) k_restrict_top(int B, int L, const int* __restrict__ ls,
+                                                        const int* __restrict__ going_next,
+                                                        const unsigned* __restrict__ comp_mask, T* __restrict__ R,
+                                                        const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0]) return;
-    __shared__ int par[kTB];
-    __shared__ T sv[3][kTB];
+    __shared__ T sv[3][kTop];
+    __shared__ T nx[3][kTop];
     const int t = threadIdx.x;
+    bool in_lds = false;  // this level's values are in nx
     for (int l = 2; l + 1 < L; ++l) {
-        const int cnt = ls[2 * l], beg = ls[2 * l + 1];
-        for (int c0 = 0; c0 < cnt; c0 += kTB) {
+        const int cnt = ls[2 * l], beg = ls[2 * l + 1], cntn = ls[2 * l + 2], begn = ls[2 * l + 3];
+        const bool next_fits = cntn <= kTop;
+        for (int c0 = 0; c0 < cnt; c0 += kTop) {
             int cc = c0 + t;
             bool valid = cc < cnt;
             size_t g = (size_t)beg + cc;
-            par[t] = valid ? going_next[g] : -1;
-            sv[0][t] = valid ? R[3 * g] : (T)0;
-            sv[1][t] = valid ? R[3 * g + 1] : (T)0;
-            sv[2][t] = valid ? R[3 * g + 2] : (T)0;
+            int pp = valid ? going_next[g] : -1;
+            unsigned mask = valid ? comp_mask[g] : 0u;
+            T v0 = 0, v1 = 0, v2 = 0;
+            if (valid) {
+                if (in_lds) { v0 = nx[0][cc]; v1 = nx[1][cc]; v2 = nx[2][cc]; }
+                else { v0 = R[3 * g]; v1 = R[3 * g + 1]; v2 = R[3 * g + 2]; }
+            }
+            __syncthreads();  // nx fully read before this pass overwrites it
+            sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
             __syncthreads();
             T s0, s1, s2;
-            if (component_sum<T>(par, sv, B, s0, s1, s2)) {
-                size_t q = (size_t)par[t];
+            if (valid && component_sum<T>(mask, sv[0], sv[1], sv[2], B, s0, s1, s2)) {
+                size_t q = (size_t)pp;
                 R[3 * q] = s0; R[3 * q + 1] = s1; R[3 * q + 2] = s2;
+                if (next_fits) { int j = (int)(q - begn); nx[0][j] = s0; nx[1][j] = s1; nx[2][j] = s2; }
             }
             __syncthreads();
         }
+        in_lds = next_fits;
     }
 }
 
@@ -615,11 +703,12 @@ void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl) {
     if (m.nb1 <= 0) return;
     hipLaunchKernelGGL((k_restrict_all<V, T>), dim3(m.nb1), dim3(kTB), 0, c.stream, r, sub3, m.bank, m.L,
                        (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
-                       (const int*)m.psum0.p, (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
+                       (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                       (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
     SGIPC_LAUNCH_CHECK();
     if (m.L >= 4) {
-        hipLaunchKernelGGL((k_restrict_top<T>), dim3(1), dim3(kTB), 0, c.stream, m.bank, m.L,
-                           (const int*)m.level_size.p, (const int*)m.going_next.p, R, ctrl);
+        hipLaunchKernelGGL((k_restrict_top<T>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
+                           (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p, R, ctrl);
         SGIPC_LAUNCH_CHECK();
     }
 }
@@ -696,6 +785,7 @@ void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, co
     m.level_size.reserve(2 * ((size_t)m.L + 2));
     m.coarse_space.reserve((size_t)m.L * n + 1);
     m.going_next.reserve(m.going_cap);
+    m.comp_mask.reserve(m.going_cap);
     m.coarse_table.reserve(6 * (size_t)n + 1);
     m.node_of.reserve((size_t)m.L * n + 1);
     m.level_size_h.assign(2 * ((size_t)m.L + 2), 0);
@@ -745,7 +835,7 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         }
         int nbk = div_up(nmax, B);
         hipLaunchKernelGGL(k_lx_closure, dim3(div_up((long long)nbk * B, kTB)), dim3(kTB), 0, s, nbk * B, B, level, ls,
-                           m.next_mask.p, m.next_prefix.p);
+                           m.next_mask.p, m.next_prefix.p, m.comp_mask.p);
         SGIPC_LAUNCH_CHECK();
         exclusive_scan(c, m.next_prefix.p, m.next_psum.p, nbk, tot + 6, s);
         hipLaunchKernelGGL(k_set_level_size, dim3(1), dim3(1), 0, s, ls, level, tot + 6, B, 0);
@@ -835,6 +925,59 @@ void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* 
     if (!fp32) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
     mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl);
 }
+// workgroups of the fused update: level-1 banks (grid-stride) plus a few for
+// the dofs outside the preconditioned range; one r.r partial each
+void mas_update_layout(const Ctx& c, int& g1, int& gx) {
+    const MasState& m = c.mas;
+    long long nout = 3LL * c.nb - 3LL * m.n;
+    gx = nout > 0 ? (int)std::min<long long>(64, div_up(nout, kTB)) : 0;
+    g1 = std::max(1, std::min(m.nb1, kPart - gx));
+}
+
+template <typename V>
+static void update_restrict_t(Ctx& c, double* x, V* r, const V* p, const V* Ap, const double* part_pap, int np_pap,
+                              const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
+    MasState& m = c.mas;
+    int g1, gx;
+    mas_update_layout(c, g1, gx);
+    const long long N = 3LL * c.nb;
+    if (fp32) {
+        hipLaunchKernelGGL((k_update_restrict<V, float>), dim3(g1 + gx), dim3(kTB), 0, c.stream, N, x, r, p, Ap,
+                           part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, c.sub_offset, m.n, m.bank, m.L,
+                           (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
+                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                           (const int*)m.l1lo.p, (const int*)m.l1hi.p, m.nb1, g1, m.R32.p);
+    } else {
+        if (sizeof(V) != 8) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
+        hipLaunchKernelGGL((k_update_restrict<V, double>), dim3(g1 + gx), dim3(kTB), 0, c.stream, N, x, r, p, Ap,
+                           part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, c.sub_offset, m.n, m.bank, m.L,
+                           (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
+                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                           (const int*)m.l1lo.p, (const int*)m.l1hi.p, m.nb1, g1, m.R64.p);
+    }
+    SGIPC_LAUNCH_CHECK();
+    if (m.L >= 4) {
+        if (fp32)
+            hipLaunchKernelGGL((k_restrict_top<float>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
+                               (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
+                               m.R32.p, (const int*)ctrl);
+        else
+            hipLaunchKernelGGL((k_restrict_top<double>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
+                               (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
+                               m.R64.p, (const int*)ctrl);
+        SGIPC_LAUNCH_CHECK();
+    }
+}
+
+void mas_update_restrict(Ctx& c, double* x, double* r, const double* p, const double* Ap, const double* part_pap,
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
+    update_restrict_t<double>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32);
+}
+void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const float* Ap, const double* part_pap,
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
+    update_restrict_t<float>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32);
+}
+
 void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl) {
     if (fp32) mas_solve_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
     else mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);

@@ -5,48 +5,23 @@
 // dot), vector updates :105-122; config pcg_solver.h:6-16.
 //
 // Here every scalar (rz, alpha, beta, the stop decision) lives in device
-// memory: the dot products finish in the last workgroup of the producing
-// kernel (fixed-order, deterministic), alpha/beta are computed on the GPU,
-// and every per-iteration kernel exits at once when the device `done` flag
-// is set.  The host only polls that flag every few iterations, so the
-// iteration count and the iterate are exactly those of the reference loop
-// (stop test on the stale rz after the x/r update).
+// memory: each producer of a dot product writes one partial per workgroup
+// and every consumer workgroup sums the partials itself in a fixed order
+// (deterministic, no atomics, no host round trip).  The iteration counter
+// and the stop flag are device words too, so an iteration pair is a fixed
+// launch sequence that is captured once into a HIP graph and replayed; the
+// host only polls the stop flag every few iterations.  The iteration count
+// and the iterate are those of the reference loop (stop test on the stale
+// rz after the x/r update).
 #include "spmv_sym.hpp"
+#include "pcg_dev.hpp"
 #include <cmath>
 
 namespace sgipc {
 namespace {
 
-enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
-enum { C_STOP = 0, C_ITER, C_MAXIT, C_NOCONV, C_NUM };  // C_STOP: iteration at which the loop stopped (0 = running)
-
-constexpr int kThreads = 1024;  // workgroup size of every PCG kernel (16 waves)
-constexpr int kRedBlocks = 1024;  // producers of dot partials use at most this many blocks
-constexpr int kPart = 1024;       // partial slots per dot
-
-__device__ inline double block_sum_all(double v, double* sh) {
-    // result broadcast to every thread; fixed reduction tree
-    v = wave_sum(v);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    double s = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
-    return s;
-}
-
-// sum of np partials, computed redundantly (same order) by every block
-__device__ inline double sum_partials(const double* __restrict__ part, int np, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
-    return block_sum_all(v, sh);
-}
-
-__device__ inline void write_partial(double v, double* __restrict__ part, double* sh) {
-    double s = block_sum_all(v, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
+constexpr int kThreads = 1024;    // workgroup size of the PCG kernels (16 waves)
+constexpr int kRedBlocks = 1024;  // producers of dot partials use at most this many workgroups
 
 // partial dot products of a.b, one per block (FP64 accumulation)
 template <typename VA, typename VB>
@@ -83,6 +58,7 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
         return;
     }
     __shared__ double sh[kThreads / 64];
+    auto pv = [p](size_t i) { return p[i]; };
     constexpr int RPB = kThreads / G;
     const int nchunks = (nb + RPB - 1) / RPB;
     ChunkWalk w = xcd_chunks(nchunks);
@@ -90,7 +66,7 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
     for (int ch = w.first; ch < w.end; ch += w.step) {
         int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         MV s0 = 0, s1 = 0, s2 = 0;
-        if (row < nb) sym_row_lane<G, MV, V>(row, sub, lptr, lref, ustart, ucol, val, p, s0, s1, s2);
+        if (row < nb) sym_row_lane<G, MV>(row, sub, lptr, lref, ustart, ucol, val, pv, s0, s1, s2);
         group_sum3<G, MV>(s0, s1, s2);
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;
@@ -131,14 +107,15 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
 // follows the x/r update (so no block depends on another's flag write):
 // rule 0 (reference): |rz_prev| <= tol * rz0 on the pre-update rz;
 // rule 1: ||r|| <= tol * ||b|| on the updated residual.  rule < 0: none.
-__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, const double* __restrict__ part_rr,
-                                 int np, const double* __restrict__ sc, int* __restrict__ ctrl, double* sh) {
+__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, int np_rz,
+                                 const double* __restrict__ part_rr, int np_rr, const double* __restrict__ sc,
+                                 int* __restrict__ ctrl, double* sh) {
     bool stop = false;
     if (rule == 0) {
-        double rz = sum_partials(part_rz_prev, np, sh);
+        double rz = sum_partials(part_rz_prev, np_rz, sh);
         stop = fabs(rz) <= sc[S_TOL] * sc[S_RZ0];
     } else if (rule == 1) {
-        double rr = sum_partials(part_rr, np, sh);
+        double rr = sum_partials(part_rr, np_rr, sh);
         stop = sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB]);
     }
     if (stop && blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_STOP] = ctrl[C_ITER] - 1;
@@ -151,13 +128,13 @@ template <typename V, typename T>
 __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __restrict__ r, V* __restrict__ z,
                                                           int sub_off, int n_mas, int L, const int* __restrict__ node_of,
                                                           const T* __restrict__ Z, double* __restrict__ part_rz,
-                                                          const double* __restrict__ part_rz_prev,
-                                                          const double* __restrict__ part_rr, int np,
+                                                          const double* __restrict__ part_rz_prev, int np_rz,
+                                                          const double* __restrict__ part_rr, int np_rr,
                                                           const double* __restrict__ sc, int* __restrict__ ctrl,
                                                           int rule) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
-    if (stop_test(rule, part_rz_prev, part_rr, np, sc, ctrl, sh)) return;
+    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     double v = 0.0;
     for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nb; row += gridDim.x * blockDim.x) {
         size_t o = 3 * (size_t)row;
@@ -184,13 +161,13 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, const V* __restrict__ r, V* __restrict__ z,
                                                                  const double* __restrict__ Dinv, double* __restrict__ part_rz,
-                                                                 const double* __restrict__ part_rz_prev,
-                                                                 const double* __restrict__ part_rr, int np,
+                                                                 const double* __restrict__ part_rz_prev, int np_rz,
+                                                                 const double* __restrict__ part_rr, int np_rr,
                                                                  const double* __restrict__ sc, int* __restrict__ ctrl,
                                                                  int rule) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
-    if (stop_test(rule, part_rz_prev, part_rr, np, sc, ctrl, sh)) return;
+    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         V zi;
@@ -208,11 +185,11 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
     write_partial(v, part_rz, sh);
 }
 
-// beta = rz_new / rz_old ; p = z + beta p ; next iteration
+// beta = rz_new / rz_old ; p = z + beta p
 template <typename V>
 __global__ void __launch_bounds__(kThreads) k_update_p(long long N, V* __restrict__ p, const V* __restrict__ z,
                                                        const double* __restrict__ part_new, const double* __restrict__ part_old,
-                                                       int np, int* __restrict__ ctrl) {
+                                                       int np, const int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
     double rn = sum_partials(part_new, np, sh);
@@ -279,26 +256,23 @@ struct Engine {
     const double* Dinv;
     int rule;
     int maxit = 0;
+    int gU = 1;  // workgroups (and r.r partials) of the x/r update
 
     static constexpr int G = 8;
     static constexpr bool kFp32Mas = sizeof(T) == 4;
-    // every producer of a dot's partials (update_xr, collect, simple
-    // precond) runs gN workgroups, so consumers always sum gN partials
-    int gR = 1;
 
-    // preconditioner apply fused with the r.z partials (and the stop test)
+    // z = M^-1 r fused with the r.z partials (gN workgroups) and the stop test
     void precond_dot(double* rz_out, const double* rz_prev, int rule_now) {
         hipStream_t s = c.stream;
         if (mas) {
-            mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl);
             mas_bank_solve(c, kFp32Mas, ctrl);
-            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gR), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
-                               c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, rz_out, rz_prev, part_rr, gN, sc, ctrl,
-                               rule_now);
+            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
+                               c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, rz_out, rz_prev, gN, part_rr, gU, sc,
+                               ctrl, rule_now);
             SGIPC_LAUNCH_CHECK();
         } else {
             hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_out,
-                               rz_prev, part_rr, gN, sc, ctrl, rule_now);
+                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now);
             SGIPC_LAUNCH_CHECK();
         }
     }
@@ -307,18 +281,23 @@ struct Engine {
     void enqueue_iteration(int par) {
         hipStream_t s = c.stream;
         const SymBcoo& S = c.bsr;
-        double* rz_prev = part_rz[par ^ 1];
-        double* rz_cur = part_rz[par];
+        double* rz_last = part_rz[par ^ 1];  // rz_(k-1)
+        double* rz_this = part_rz[par];      // receives rz_k
         prof_begin(c, PROF_SPMV);
         hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.lptr.p, S.lref.p,
-                           c.uniq_start.p, c.t_cols, val, p, Ap, part_pap, ctrl);
+                           c.uniq_start.p, c.t_cols, val, (const V*)p, Ap, part_pap, ctrl);
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
-        hipLaunchKernelGGL((k_update_xr<V>), dim3(gN), dim3(kThreads), 0, s, N, x, r, p, Ap, part_pap, gS, rz_prev, gN,
-                           part_rr, ctrl);
-        SGIPC_LAUNCH_CHECK();
-        precond_dot(rz_cur, rz_prev, rule);
-        hipLaunchKernelGGL((k_update_p<V>), dim3(gN), dim3(kThreads), 0, s, N, p, z, rz_cur, rz_prev, gN, ctrl);
+        if (mas) {
+            mas_update_restrict(c, x, r, p, Ap, part_pap, gS, rz_last, gN, part_rr, ctrl, kFp32Mas);
+        } else {
+            hipLaunchKernelGGL((k_update_xr<V>), dim3(gN), dim3(kThreads), 0, s, N, x, r, (const V*)p, (const V*)Ap,
+                               (const double*)part_pap, gS, (const double*)rz_last, gN, part_rr, ctrl);
+            SGIPC_LAUNCH_CHECK();
+        }
+        precond_dot(rz_this, rz_last, rule);
+        hipLaunchKernelGGL((k_update_p<V>), dim3(gN), dim3(kThreads), 0, s, N, p, (const V*)z, (const double*)rz_this,
+                           (const double*)rz_last, gN, (const int*)ctrl);
         SGIPC_LAUNCH_CHECK();
     }
 
@@ -327,7 +306,13 @@ struct Engine {
     double init(double tol, int max_iter) {
         hipStream_t s = c.stream;
         maxit = max_iter;
-        gR = gN;
+        if (mas) {
+            int g1, gx;
+            mas_update_layout(c, g1, gx);
+            gU = g1 + gx;
+        } else {
+            gU = gN;
+        }
         hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, max_iter);
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL, tol);
@@ -337,10 +322,11 @@ struct Engine {
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
         SGIPC_LAUNCH_CHECK();
+        if (mas) mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl);
         precond_dot(part_rz[0], part_rz[1], -1);
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
         SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_copy_cast<V, V>), dim3(gN), dim3(kThreads), 0, s, N, p, z);
+        hipLaunchKernelGGL((k_copy_cast<V, V>), dim3(gN), dim3(kThreads), 0, s, N, p, (const V*)z);
         SGIPC_LAUNCH_CHECK();
         double rz0 = 0.0;
         SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -352,18 +338,19 @@ struct Engine {
         size_t h = 1469598103934665603ull;
         auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
         const SymBcoo& S = c.bsr;
-        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)gR); mix((size_t)S.nb); mix((size_t)S.K);
+        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)gU); mix((size_t)S.nb); mix((size_t)S.K);
         mix((size_t)S.nlow); mix((size_t)S.lptr.p); mix((size_t)S.lref.p); mix((size_t)c.uniq_start.p);
-        mix((size_t)c.t_cols); mix((size_t)val); mix((size_t)r); mix((size_t)z);
-        mix((size_t)p); mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc); mix((size_t)ctrl);
-        mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
+        mix((size_t)c.t_cols); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
+        mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
+        mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
         if (mas) {
             const MasState& m = c.mas;
             mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
             mix((size_t)m.nb1); mix((size_t)m.going_next.p); mix((size_t)m.node_of.p); mix((size_t)m.level_size.p);
             mix((size_t)m.part_map.p); mix((size_t)m.psum0.p); mix((size_t)m.l1lo.p); mix((size_t)m.l1hi.p);
-            mix((size_t)m.inv64.p); mix((size_t)m.inv32.p);
+            mix((size_t)m.inv64.p); mix((size_t)m.inv32.p); mix((size_t)m.R64.p); mix((size_t)m.R32.p);
+            mix((size_t)m.Z64.p); mix((size_t)m.Z32.p);
         }
         return h;
     }
@@ -450,8 +437,8 @@ void precond_apply(Ctx& c, const double* r, double* z) {
     int* ctrl = c.pcg.ctrl.reserve(C_NUM + 8);
     fill<int>(ctrl, 0, C_NUM, s);
     hipLaunchKernelGGL((k_simple_precond_dot<double>), dim3(grid_for(N)), dim3(kThreads), 0, s, N, r, z,
-                       c.precond_kind == 1 ? (const double*)c.pcg.Dinv.p : (const double*)nullptr, part, part, part, 1,
-                       (const double*)nullptr, ctrl, -1);
+                       c.precond_kind == 1 ? (const double*)c.pcg.Dinv.p : (const double*)nullptr, part, part, 1, part,
+                       1, (const double*)nullptr, ctrl, -1);
     SGIPC_LAUNCH_CHECK();
 }
 
@@ -491,7 +478,8 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
             using T = decltype(tag);
             const bool fp32mas = sizeof(T) == 4;
             Engine<double, double, T> E{c, N, gN, gS,
-                                        w.r.reserve(N + 1), w.z.reserve(N + 1), w.p.reserve(N + 1), w.Ap.reserve(N + 1),
+                                        w.r.reserve(N + 1), w.z.reserve(N + 1), w.p.reserve(N + 1),
+                                        w.Ap.reserve(N + 1),
                                         x, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
                                         parts + 4 * kPart, sc, ctrl, (const double*)c.t_vals, mas,
                                         mas ? (T*)(fp32mas ? (void*)c.mas.R32.p : (void*)c.mas.R64.p) : nullptr,
@@ -521,7 +509,8 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
         double* r64 = w.r64b.reserve(N + 1);
         double* d64 = w.d64.reserve(N + 1);
         Engine<float, float, float> E{c, N, gN, gS,
-                                      rf, w.zf.reserve(N + 1), w.pf.reserve(N + 1), w.Apf.reserve(N + 1),
+                                      rf, w.zf.reserve(N + 1), w.pf.reserve(N + 1),
+                                      w.Apf.reserve(N + 1),
                                       d64, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
                                       parts + 4 * kPart, sc, ctrl, bsr_values_f32(c), mas,
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,

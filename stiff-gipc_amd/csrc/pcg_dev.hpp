@@ -1,0 +1,43 @@
+// Device-side pieces shared by the PCG kernels and the MAS kernels that run
+// inside a PCG iteration: the control words, the device scalars and the
+// fixed-order workgroup reductions used for every dot product.
+#pragma once
+#include "common.hpp"
+
+namespace sgipc {
+
+// device scalars: rz0, b.b, tolerance
+enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
+// control words: C_STOP = iteration at which the loop stopped (0 = running),
+// C_ITER = current iteration (from 1), C_MAXIT = iteration budget,
+// C_NOCONV = stopped by the budget
+enum { C_STOP = 0, C_ITER, C_MAXIT, C_NOCONV, C_NUM };
+
+constexpr int kPart = 1024;  // partial slots per dot product
+
+// workgroup sum broadcast to every thread; fixed reduction tree
+__device__ inline double block_sum_all(double v, double* sh) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    return s;
+}
+
+// sum of np partials, computed redundantly (same order) by every workgroup
+__device__ inline double sum_partials(const double* __restrict__ part, int np, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[i];
+    return block_sum_all(v, sh);
+}
+
+// one partial per workgroup, at part[blockIdx.x]
+__device__ inline void write_partial(double v, double* __restrict__ part, double* sh) {
+    double s = block_sum_all(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+}  // namespace sgipc

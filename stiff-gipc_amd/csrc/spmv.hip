@@ -21,7 +21,8 @@ __global__ void __launch_bounds__(256) k_spmv_sym(int nb, const int* __restrict_
     for (int ch = w.first; ch < w.end; ch += w.step) {
         int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         TV s0 = 0, s1 = 0, s2 = 0;
-        if (row < nb) sym_row_lane<G, TV, TX>(row, sub, lptr, lref, ustart, ucol, vals, x, s0, s1, s2);
+        auto xv = [x](size_t i) { return x[i]; };
+        if (row < nb) sym_row_lane<G, TV>(row, sub, lptr, lref, ustart, ucol, vals, xv, s0, s1, s2);
         group_sum3<G, TV>(s0, s1, s2);
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;

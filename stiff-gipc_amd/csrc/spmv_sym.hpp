@@ -47,11 +47,12 @@ __device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, T
     }
 }
 
-// partial row sums of this lane (before the group reduction)
-template <int G, typename TV, typename TX>
+// partial row sums of this lane (before the group reduction); xv(i) returns
+// entry i of the vector being multiplied
+template <int G, typename TV, typename XF>
 __device__ inline void sym_row_lane(int row, int sub, const int* __restrict__ lptr, const int2* __restrict__ lref,
                                     const int* __restrict__ ustart, const int* __restrict__ ucol,
-                                    const TV* __restrict__ vals, const TX* __restrict__ x, TV& s0, TV& s1, TV& s2) {
+                                    const TV* __restrict__ vals, const XF& xv, TV& s0, TV& s1, TV& s2) {
     int l0 = lptr[row], nl = lptr[row + 1] - l0;
     int u0 = ustart[row], nu = ustart[row + 1] - u0;
     for (int e = sub; e < nl + nu; e += G) {
@@ -66,7 +67,7 @@ __device__ inline void sym_row_lane(int row, int sub, const int* __restrict__ lp
         }
         TV a[9];
         load_block<TV>(vals, k, tr, a);
-        TV x0 = (TV)x[3 * (size_t)j], x1 = (TV)x[3 * (size_t)j + 1], x2 = (TV)x[3 * (size_t)j + 2];
+        TV x0 = (TV)xv(3 * (size_t)j), x1 = (TV)xv(3 * (size_t)j + 1), x2 = (TV)xv(3 * (size_t)j + 2);
         s0 += a[0] * x0 + a[3] * x1 + a[6] * x2;
         s1 += a[1] * x0 + a[4] * x1 + a[7] * x2;
         s2 += a[2] * x0 + a[5] * x1 + a[8] * x2;
