@@ -32,7 +32,22 @@ def parse():
     ap.add_argument("--cells", type=int, default=40)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    # a plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of its
+    # dispatches calibrate the counters against exact byte counts
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
 
 
 def spmv_alg_bytes(K, nb):
@@ -71,6 +86,9 @@ def cpu_baseline(sc, pairs, mat_index, counts):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (run with --gpus 1)" % a.gpus,
               file=sys.stderr)

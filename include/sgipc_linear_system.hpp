@@ -1,0 +1,181 @@
+// sgipc_linear_system.hpp -- header-only C++ facade over the sgipc C ABI with
+// the shape of Stiff-GIPC's linear-system classes (namespace gipc,
+// StiffGIPC/linear_system/): one GlobalLinearSystem that owns the solver
+// configuration and the preconditioner, a triplet matrix the caller
+// assembles into, and solve_linear_system() returning the PCG iteration
+// count.  Errors throw sgipc::Error (the C ABI returns status codes; the
+// reference aborts).
+//
+// Mapping to the reference:
+//   GlobalLinearSystem            global_linear_system.h:24-66
+//   solve_linear_system()         global_linear_system.cu:124-133
+//   PCGSolverConfig               solver/pcg_solver.h:6-16
+//   TripletMatrix                 GIPCTripletMatrix, linear_system/global_matrix.h:18-203
+//   set_mas_partition()           setMAS_partition + initPreconditioner_Neighbor,
+//                                 gl_main.cu:1066-1093, MASPreconditioner.cu:2264-2302
+//   kinetic/fem/barrier/mass/rhs  GIPC::computeGradientAndHessian, GIPC.cu:10110-10344
+#ifndef SGIPC_LINEAR_SYSTEM_HPP
+#define SGIPC_LINEAR_SYSTEM_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sgipc.h"
+
+namespace sgipc {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc, const sgipc_ctx* ctx, const char* where) {
+    if (rc == SGIPC_OK) return;
+    std::string msg = where;
+    msg += ": ";
+    msg += ctx ? sgipc_last_error(ctx) : "context creation failed";
+    throw Error(rc, msg);
+}
+
+// PCGSolverConfig (solver/pcg_solver.h:6-16) plus this build's options
+struct PCGSolverConfig {
+    double global_tol_rate = 1e-4;           // pcg_threshold (parameterSetting.txt:14)
+    double max_iter_ratio = 0.3;             // max iterations = ratio * dof
+    int max_iter = 0;                        // > 0 overrides the ratio
+    int stop_rule = SGIPC_STOP_REF_RZ;       // reference |rz| <= tol rz0 test
+    int precision = SGIPC_PRECISION_FP64;
+    int check_every = 8;
+};
+
+// View of the caller's device triplet arrays (GIPCTripletMatrix).  Assembly
+// calls append at global_triplet_offset and advance it, as GIPC does.
+struct TripletMatrix {
+    double* values = nullptr;  // 9 doubles per block, column-major
+    int* rows = nullptr;
+    int* cols = nullptr;
+    long long capacity = 0;
+    long long global_triplet_offset = 0;
+};
+
+class GlobalLinearSystem {
+public:
+    explicit GlobalLinearSystem(int device = 0, void* hip_stream = nullptr) {
+        check(sgipc_ctx_create(device, &ctx_), nullptr, "sgipc_ctx_create");
+        if (hip_stream) check(sgipc_ctx_set_stream(ctx_, hip_stream), ctx_, "sgipc_ctx_set_stream");
+    }
+    ~GlobalLinearSystem() {
+        if (ctx_) sgipc_ctx_destroy(ctx_);
+    }
+    GlobalLinearSystem(const GlobalLinearSystem&) = delete;
+    GlobalLinearSystem& operator=(const GlobalLinearSystem&) = delete;
+
+    sgipc_ctx* handle() const { return ctx_; }
+    PCGSolverConfig& config() { return cfg_; }
+    const sgipc_solve_stats& last_stats() const { return stats_; }
+    TripletMatrix& triplets() { return trip_; }
+
+    // ---- one-time setup --------------------------------------------------
+    void set_fem_mesh(int n, int T, const uint32_t* h_tets) {
+        check(sgipc_set_fem_mesh(ctx_, n, T, h_tets), ctx_, "sgipc_set_fem_mesh");
+        n_fem_ = n;
+    }
+    // Vertex neighbours from the tets and bank slot maps from ascending part
+    // ids (host), then the MAS topology upload.
+    void set_mas_partition(int n, int T, const uint32_t* h_tets, const int* h_part_id, int bank = 16) {
+        std::vector<uint32_t> start(n > 0 ? n : 1), num(n > 0 ? n : 1), list(12 * (size_t)T + 1);
+        int len = 0;
+        check(sgipc_build_vertex_neighbors(n, T, h_tets, start.data(), num.data(), list.data(), (int)list.size(),
+                                           &len),
+              ctx_, "sgipc_build_vertex_neighbors");
+        const int cap = n > 0 ? bank * (h_part_id[n - 1] + 1) : 0;
+        std::vector<int> part_map(cap > 0 ? cap : 1), real_map(n > 0 ? n : 1);
+        int M = 0;
+        check(sgipc_build_bank_map(n, h_part_id, bank, part_map.data(), cap, &M, real_map.data()), ctx_,
+              "sgipc_build_bank_map");
+        check(sgipc_set_mas_topology(ctx_, n, 0, start.data(), num.data(), list.data(), len, part_map.data(), M,
+                                     real_map.data(), bank),
+              ctx_, "sgipc_set_mas_topology");
+    }
+    void set_triplet_matrix(double* d_vals, int* d_rows, int* d_cols, long long capacity) {
+        check(sgipc_triplets_view(ctx_, d_vals, d_rows, d_cols, capacity), ctx_, "sgipc_triplets_view");
+        trip_ = TripletMatrix{d_vals, d_rows, d_cols, capacity, 0};
+    }
+    void set_preconditioner(int kind) { precond_ = kind; }
+    // contact pairs of the current iterate (MAS contact connectivity)
+    void set_contact_pairs(const int* d_pairs, int n_pairs) {
+        d_pairs_ = d_pairs;
+        n_pairs_ = n_pairs;
+    }
+
+    // ---- assembly: GIPC::computeGradientAndHessian ------------------------
+    void begin_assembly() { trip_.global_triplet_offset = 0; }
+    void kinetic_gradient(int n, const double* d_x, const double* d_xtilde, const double* d_mass, double* d_shape) {
+        check(sgipc_kinetic_gradient(ctx_, n, d_x, d_xtilde, d_mass, d_shape), ctx_, "sgipc_kinetic_gradient");
+    }
+    void assemble_barrier(int n_pairs, const int* d_pairs, const int* d_mat_index, const int counts[3],
+                          const double* d_x, const double* d_xrest, double dhat, double kappa, const int* d_btype,
+                          double* d_fb) {
+        check(sgipc_assemble_barrier(ctx_, n_pairs, d_pairs, d_mat_index, counts, d_x, d_xrest, dhat, kappa,
+                                     d_btype, 0, trip_.global_triplet_offset, d_fb),
+              ctx_, "sgipc_assemble_barrier");
+        trip_.global_triplet_offset += 16LL * counts[2] + 9LL * counts[1] + 4LL * counts[0];
+    }
+    void assemble_fem(int T, const double* d_x, const uint32_t* d_tets, const double* d_DmInv, const double* d_vol,
+                      const double* d_mu, const double* d_lambda, double dt, const int* d_btype, double* d_shape) {
+        check(sgipc_assemble_fem(ctx_, T, d_x, d_tets, d_DmInv, d_vol, d_mu, d_lambda, dt, d_btype, 0,
+                                 trip_.global_triplet_offset, d_shape),
+              ctx_, "sgipc_assemble_fem");
+        trip_.global_triplet_offset += 10LL * T;
+    }
+    void mass_diagonal(int n, const double* d_mass) {
+        check(sgipc_mass_triplets(ctx_, n, d_mass, 0, trip_.global_triplet_offset), ctx_, "sgipc_mass_triplets");
+        trip_.global_triplet_offset += n;
+    }
+    // FEMLinearSubsystem::assemble: b = fb + shape, zero on pinned vertices
+    void assemble_rhs(int n, const double* d_fb, const double* d_shape, const int* d_btype, double* d_b) {
+        check(sgipc_fem_rhs(ctx_, n, d_fb, d_shape, d_btype, d_b), ctx_, "sgipc_fem_rhs");
+    }
+
+    // ---- solve: GlobalLinearSystem::solve_linear_system -----------------
+    // convert the assembled triplets in place, build the preconditioner, run
+    // PCG; returns the iteration count (0 for an empty system).
+    std::size_t solve_linear_system(const double* d_b, double* d_x) {
+        if (n_fem_ <= 0 || trip_.global_triplet_offset == 0) return 0;
+        int K = 0;
+        check(sgipc_convert(ctx_, trip_.global_triplet_offset, n_fem_, &K), ctx_, "sgipc_convert");
+        check(sgipc_precond_build(ctx_, precond_, 0, n_fem_, d_pairs_, n_pairs_), ctx_, "sgipc_precond_build");
+        sgipc_solve_opts o{cfg_.precision, cfg_.stop_rule, cfg_.global_tol_rate, cfg_.max_iter_ratio, cfg_.max_iter,
+                           cfg_.check_every};
+        check(sgipc_solve(ctx_, d_b, d_x, &o, &stats_), ctx_, "sgipc_solve");
+        return (std::size_t)stats_.iterations;
+    }
+    // helpers of the solver contract (i_linear_system_solver.h:43-53)
+    void spmv(double a, const double* d_x, double b, double* d_y) {
+        check(sgipc_spmv(ctx_, a, d_x, b, d_y), ctx_, "sgipc_spmv");
+    }
+    void apply_preconditioner(const double* d_r, double* d_z) {
+        check(sgipc_precond_apply(ctx_, d_r, d_z), ctx_, "sgipc_precond_apply");
+    }
+    void synchronize() { check(sgipc_ctx_synchronize(ctx_), ctx_, "sgipc_ctx_synchronize"); }
+
+private:
+    sgipc_ctx* ctx_ = nullptr;
+    PCGSolverConfig cfg_;
+    sgipc_solve_stats stats_{};
+    TripletMatrix trip_;
+    int n_fem_ = 0;
+    int precond_ = SGIPC_PRECOND_MAS;
+    const int* d_pairs_ = nullptr;
+    int n_pairs_ = 0;
+};
+
+}  // namespace sgipc
+
+#endif

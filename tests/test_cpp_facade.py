@@ -1,0 +1,76 @@
+"""The C++ facade (include/sgipc_linear_system.hpp) as a GIPC-style caller
+would use it: stiff-gipc_amd/newton_step runs one Newton step of a scene
+file through GlobalLinearSystem::solve_linear_system.  On the GPU its result
+must be identical to the Python driver's (same kernels, same order); on the
+CPU its error paths must report cleanly."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "stiff-gipc_amd", "newton_step")
+
+
+def write_scene(path, sc, precond=2, stop_rule=0, tol=1e-4, max_iter=0):
+    with open(path, "wb") as f:
+        f.write(b"SGIPCSC1")
+        f.write(struct.pack("<6i2d", sc.n, sc.T, sc.bank, precond, stop_rule, max_iter, sc.dt, tol))
+        for a, dt in ((sc.x, "<f8"), (sc.x_tilde, "<f8"), (sc.mass, "<f8"), (sc.btype, "<i4"), (sc.tets, "<u4"),
+                      (sc.DmInv, "<f8"), (sc.vol, "<f8"), (sc.mu, "<f8"), (sc.lam, "<f8"), (sc.part, "<i4")):
+            f.write(np.ascontiguousarray(a, dtype=dt).tobytes())
+
+
+def read_out(path, n):
+    with open(path, "rb") as f:
+        it, conv = struct.unpack("<2i", f.read(8))
+        rr, = struct.unpack("<d", f.read(8))
+        dx = np.frombuffer(f.read(), dtype="<f8")
+    assert dx.size == 3 * n
+    return it, conv, rr, dx
+
+
+def have_gpu():
+    import torch
+    return torch.cuda.device_count() > 0
+
+
+def test_facade_rejects_bad_scene_file(tmp_path):
+    assert os.path.exists(EXE), "build first (make -C stiff-gipc_amd newton_step)"
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"NOTASCENE" + b"\0" * 64)
+    p = subprocess.run([EXE, str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "bad scene" in p.stderr
+
+
+def test_facade_reports_missing_device(tmp_path, cube4):
+    if have_gpu():
+        pytest.skip("a HIP device is present; the GPU test covers the facade")
+    path = tmp_path / "cube4.bin"
+    write_scene(path, cube4)
+    p = subprocess.run([EXE, str(path), str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 3 and "sgipc_ctx_create" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stop_rule,tol", [(0, 1e-4), (1, 1e-6)])
+def test_facade_newton_step_matches_python_driver(tmp_path, cube10, stop_rule, tol):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from sgipc import capi
+    from sgipc.system import NewtonSystem
+    print("case: C++ facade newton step cube10 rule=%d" % stop_rule, flush=True)
+    path = tmp_path / "cube10.bin"
+    out = tmp_path / "out.bin"
+    write_scene(path, cube10, precond=capi.PRECOND_MAS, stop_rule=stop_rule, tol=tol, max_iter=5000)
+    p = subprocess.run([EXE, str(path), str(out)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    it, conv, rr, dx = read_out(out, cube10.n)
+    sysm = NewtonSystem(cube10, "cuda:0", precond=capi.PRECOND_MAS)
+    st = sysm.newton_step(tol=tol, stop_rule=stop_rule, max_iter=5000)
+    ref = sysm.dx.cpu().numpy()
+    assert it == st.iterations and conv == st.converged
+    assert np.array_equal(dx, ref)
