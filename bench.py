@@ -8,6 +8,8 @@ iteration's hot path on a fixed iterate: kinetic + FEM (+ barrier) gradient
 and Hessian triplets, mass diagonal, triplet -> BCOO, MAS build, MAS-PCG to a
 1e-6 true relative residual.  Inputs are resident on the GPU before timing.
 
+By default both forms run on the same system: the mixed FP32/FP64 solve
+(reported as `value`) and the plain FP64 solve (`fp64`, with the ratio).
 Prints one JSON line (metric: Newton solves/s) with the roofline of the
 dominant kernel (the PCG SpMV) measured live with HIP events, and the CPU
 baseline (the oracle's FP64 restatement of the same step, one thread).
@@ -30,7 +32,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=40)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration) and exit")
@@ -50,38 +52,89 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
 
 
-def spmv_alg_bytes(K, nb):
-    # upper BCOO: 72 B values + 8 B (row, col) per block; x gathered once
-    # (24 B per block row) and y written once (24 B per block row)
-    return 80.0 * K + 48.0 * nb
+def spmv_alg_bytes(K, nb, fp32):
+    # upper BCOO: 9 values (72 B FP64 / 36 B FP32) + 8 B (row, col) per
+    # block; x gathered once and y written once per block row
+    vb = 36.0 if fp32 else 72.0
+    xb = 12.0 if fp32 else 24.0
+    return (vb + 8.0) * K + 2.0 * xb * nb
 
 
-def cpu_baseline(sc, pairs, mat_index, counts):
+def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5):
+    """The oracle's FP64 C restatement of the same Newton step, one thread,
+    repeated until at least min_secs of CPU work (bounded by max_steps)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    t0 = time.perf_counter()
-    if pairs is not None:
-        fb, bv, br, bc = O.barrier_assemble(pairs, mat_index, counts, sc.x, sc.x_rest, sc.dhat, sc.kappa, sc.btype)
-    else:
-        fb = np.zeros(3 * sc.n); bv = np.zeros((0, 9)); br = np.zeros(0, np.int32); bc = np.zeros(0, np.int32)
-    shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)
-    grad, vals, rows, cols = O.fem_assemble(sc.x, sc.tets, sc.DmInv, sc.vol, sc.mu, sc.lam, sc.dt, sc.btype, 0,
-                                            shape.copy())
-    mv, mr, mc = O.mass_triplets(sc.mass)
-    kr, kc, kv = O.convert(np.concatenate([br, rows, mr]), np.concatenate([bc, cols, mc]),
-                           np.concatenate([bv, vals, mv]))
-    b = O.fem_rhs(fb.reshape(-1, 3), grad, sc.btype).reshape(-1)
-    t1 = time.perf_counter()
-    ns, nn, nl = O.vertex_neighbors(sc.n, sc.tets)
+    ns, nn, nl = O.vertex_neighbors(sc.n, sc.tets)  # topology setup: one-time, untimed (as on the GPU)
     pm, rm = O.bank_map(sc.part, sc.bank)
-    mas = O.Mas(sc.n, pm, rm, ns, nn, nl, sc.bank)
-    t2 = time.perf_counter()
-    mas.build(kr, kc, kv, pairs=pairs, fp32_store=False)
-    x, it, rr = O.pcg(kr, kc, kv, sc.n, b, precond=2, mas=mas, tol=1e-6, rule=1, max_iter=100000)
-    t3 = time.perf_counter()
-    secs = (t1 - t0) + (t3 - t2)  # topology setup is one-time, excluded as on the GPU
-    return secs, it
+    total, steps, it = 0.0, 0, 0
+    while steps < max_steps and (steps == 0 or total < min_secs):
+        t0 = time.perf_counter()
+        if pairs is not None:
+            fb, bv, br, bc = O.barrier_assemble(pairs, mat_index, counts, sc.x, sc.x_rest, sc.dhat, sc.kappa,
+                                                sc.btype)
+        else:
+            fb = np.zeros(3 * sc.n); bv = np.zeros((0, 9)); br = np.zeros(0, np.int32); bc = np.zeros(0, np.int32)
+        shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)
+        grad, vals, rows, cols = O.fem_assemble(sc.x, sc.tets, sc.DmInv, sc.vol, sc.mu, sc.lam, sc.dt, sc.btype, 0,
+                                                shape.copy())
+        mv, mr, mc = O.mass_triplets(sc.mass)
+        kr, kc, kv = O.convert(np.concatenate([br, rows, mr]), np.concatenate([bc, cols, mc]),
+                               np.concatenate([bv, vals, mv]))
+        b = O.fem_rhs(fb.reshape(-1, 3), grad, sc.btype).reshape(-1)
+        mas = O.Mas(sc.n, pm, rm, ns, nn, nl, sc.bank)
+        mas.build(kr, kc, kv, pairs=pairs, fp32_store=False)
+        x, it, rr = O.pcg(kr, kc, kv, sc.n, b, precond=2, mas=mas, tol=1e-6, rule=1, max_iter=100000)
+        total += time.perf_counter() - t0
+        steps += 1
+        del mas
+    return total, steps, it
+
+
+def run_mode(sysm, precision, steps, warmup):
+    """Warm-up, then exactly `steps` timed Newton steps; then two steps with
+    the library's HIP-event timing for the kernel figures."""
+    import ctypes as C
+    import torch
+    from sgipc import capi
+    solve_kw = dict(tol=1e-6, stop_rule=capi.STOP_TRUE_REL, precision=precision, max_iter=100000)
+    for _ in range(warmup):
+        sysm.newton_step(**solve_kw)
+    torch.cuda.synchronize()
+    iters, pcg_s, rel = 0, 0.0, []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sysm.assemble()
+        sysm.convert()
+        sysm.build_precond()
+        ts = time.perf_counter()
+        st = sysm.solve(**solve_kw)
+        pcg_s += time.perf_counter() - ts
+        iters += st.iterations
+        rel.append(st.rel_residual)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+
+    ctx = sysm.ctx
+    ctx.call("sgipc_profile_enable", 1)
+    for slot in range(7):
+        ctx.call("sgipc_profile_read", slot, None, None, 1)
+    for _ in range(2):
+        sysm.newton_step(**solve_kw)
+    calls = C.c_longlong(0); ms = C.c_double(0.0)
+    ctx.call("sgipc_profile_read", capi.PROF_SPMV, C.byref(calls), C.byref(ms), 0)
+    phase = {}
+    for name, slot in (("fem_kernel", capi.PROF_FEM), ("barrier_kernel", capi.PROF_BARRIER),
+                       ("convert", capi.PROF_CONVERT), ("precond_build", capi.PROF_PRECOND_BUILD),
+                       ("pcg", capi.PROF_SOLVE), ("mas_bank_solve_kernel", capi.PROF_MAS_SOLVE)):
+        c2 = C.c_longlong(0); m2 = C.c_double(0.0)
+        ctx.call("sgipc_profile_read", slot, C.byref(c2), C.byref(m2), 0)
+        phase[name] = round(m2.value / max(c2.value, 1), 4)
+    ctx.call("sgipc_profile_enable", 0)
+    return dict(value=steps / secs, ms_per_step=1e3 * secs / steps, iters=iters / steps,
+                pcg_iters_per_s=iters / pcg_s, rel_max=max(rel), phase_ms=phase,
+                spmv_ms=ms.value / max(calls.value, 1), spmv_calls=calls.value)
 
 
 def main():
@@ -101,103 +154,79 @@ def main():
     pt, ee = scene.facing_contact_candidates(sc)
     sysm = NewtonSystem(sc, "cuda:0", precond=capi.PRECOND_MAS)
     sysm.setup_contacts(pt, ee)  # narrow phase once per Newton iterate, outside the timed hot path
-    prec = capi.PRECISION_FP64 if a.precision == "fp64" else capi.PRECISION_MIXED
-    solve_kw = dict(tol=1e-6, stop_rule=capi.STOP_TRUE_REL, precision=prec, max_iter=100000)
 
-    for _ in range(a.warmup):
-        sysm.newton_step(**solve_kw)
-    torch.cuda.synchronize()
+    modes = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    res = {}
+    for m in modes:
+        res[m] = run_mode(sysm, capi.PRECISION_FP64 if m == "fp64" else capi.PRECISION_MIXED, a.steps, a.warmup)
+    main_mode = "mixed" if "mixed" in res else "fp64"
+    r = res[main_mode]
 
-    iters = 0
-    pcg_s = 0.0
-    rel = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        sysm.assemble()
-        sysm.convert()
-        sysm.build_precond()
-        ts = time.perf_counter()
-        st = sysm.solve(**solve_kw)
-        pcg_s += time.perf_counter() - ts
-        iters += st.iterations
-        rel.append(st.rel_residual)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = a.steps / secs
-
-    # live roofline of the dominant kernel (PCG SpMV), HIP events on the
-    # library stream over two profiled steps
-    import ctypes as C
-    ctx = sysm.ctx
-    ctx.call("sgipc_profile_enable", 1)
-    for slot in range(7):
-        ctx.call("sgipc_profile_read", slot, None, None, 1)
-    for _ in range(2):
-        sysm.newton_step(**solve_kw)
-    calls = C.c_longlong(0); ms = C.c_double(0.0)
-    ctx.call("sgipc_profile_read", capi.PROF_SPMV, C.byref(calls), C.byref(ms), 0)
-    phase = {}
-    for name, slot in (("fem_kernel", capi.PROF_FEM), ("convert", capi.PROF_CONVERT),
-                       ("precond_build", capi.PROF_PRECOND_BUILD), ("pcg", capi.PROF_SOLVE),
-                       ("mas_bank_solve_kernel", capi.PROF_MAS_SOLVE)):
-        c2 = C.c_longlong(0); m2 = C.c_double(0.0)
-        ctx.call("sgipc_profile_read", slot, C.byref(c2), C.byref(m2), 0)
-        phase[name] = round(m2.value / max(c2.value, 1), 4)
-    ctx.call("sgipc_profile_enable", 0)
-    spmv_ms = ms.value / max(calls.value, 1)
-    alg = spmv_alg_bytes(sysm.K, sysm.n)
-    achieved = alg / (spmv_ms * 1e-3) / 1e9
-
+    # roofline of the dominant kernel (PCG SpMV) of the reported mode
+    fp32 = main_mode == "mixed"
+    alg = spmv_alg_bytes(sysm.K, sysm.n, fp32)
+    achieved = alg / (r["spmv_ms"] * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     if os.path.exists(tf):
         try:
             with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
-        except Exception:
+                traffic = json.load(f)[main_mode]["hbm_bytes_per_launch"]
+        except (OSError, ValueError, KeyError, TypeError):
             traffic = None
 
     cpu = None
     if not a.no_cpu_baseline:
-        csecs, cit = cpu_baseline(sc, sysm.pairs.cpu().numpy().reshape(-1, 4) if sysm.n_pairs else None,
-                                  sysm.mat_index.cpu().numpy() if sysm.n_pairs else None, sysm.counts)
-        cpu = {"value": round(1.0 / csecs, 5), "unit": "Newton solves/s", "cores": 1, "kind": "port",
-               "sample": "one full Newton step of the same workload (barrier + FEM assembly, convert, MAS build, "
-                         "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, "
-                         "1 thread, %.1f s" % (cit, csecs)}
+        csecs, csteps, cit = cpu_baseline(sc, sysm.pairs.cpu().numpy().reshape(-1, 4) if sysm.n_pairs else None,
+                                          sysm.mat_index.cpu().numpy() if sysm.n_pairs else None, sysm.counts)
+        cpu = {"value": round(csteps / csecs, 5), "unit": "Newton solves/s", "cores": 1, "kind": "port",
+               "sample": "%d full Newton step(s) of the same workload (barrier + FEM assembly, convert, MAS build, "
+                         "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, 1 thread, "
+                         "%.1f s" % (csteps, cit, csecs)}
 
+    kname = "k_spmv_pap<8, float, float> (PCG SpMV, FP32 values)" if fp32 else \
+        "k_spmv_pap<8, double, double> (PCG SpMV, FP64)"
     out = {
-        "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 rel. residual)",
-        "value": round(value, 4),
+        "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 true rel. residual)",
+        "value": round(r["value"], 4),
         "unit": "Newton solves/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * secs / a.steps, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32/f64",
+        "dtype": "f32/f64" if fp32 else "f64",
         "data": "synthetic",
         "config": {
             "workload": "two stacked soft boxes, 2 x %d^3 cells, 5 tets/cell, %d vertices, %d tets, gap 0.5 dhat"
                         % (a.cells, sc.n, sc.T),
             "contact_pairs": int(sysm.n_pairs),
-            "precision": a.precision,
+            "upper_blocks": int(sysm.K),
+            "precision": main_mode,
             "preconditioner": "MAS, 16-vertex banks, %d levels" % sysm.mas_levels(),
             "stop": "true relative residual 1e-6",
             "parallelism": "single GPU",
         },
-        "pcg_iters_per_step": iters / a.steps,
-        "pcg_iters_per_s": round(iters / pcg_s, 2),
-        "rel_residual_max": max(rel),
-        "phase_ms": phase,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_pap (PCG SpMV)", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "avg_ms": round(spmv_ms, 5), "alg_bytes": alg},
+        "pcg_iters_per_step": r["iters"],
+        "pcg_iters_per_s": round(r["pcg_iters_per_s"], 2),
+        "rel_residual_max": r["rel_max"],
+        "phase_ms": r["phase_ms"],
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_ms": round(r["spmv_ms"], 5), "launches": r["spmv_calls"], "alg_bytes": alg},
         "cpu_baseline": cpu,
     }
+    if main_mode == "mixed" and "fp64" in res:
+        f = res["fp64"]
+        out["fp64"] = {"value": round(f["value"], 4), "ms_per_step": round(f["ms_per_step"], 4),
+                       "pcg_iters_per_step": f["iters"], "pcg_iters_per_s": round(f["pcg_iters_per_s"], 2),
+                       "rel_residual_max": f["rel_max"], "phase_ms": f["phase_ms"],
+                       "spmv_avg_ms": round(f["spmv_ms"], 5),
+                       "spmv_achieved_GBs": round(spmv_alg_bytes(sysm.K, sysm.n, False) / (f["spmv_ms"] * 1e-3) / 1e9,
+                                                  2)}
+        out["mixed_over_fp64"] = round(r["value"] / f["value"], 4)
     print(json.dumps(out))
 
 

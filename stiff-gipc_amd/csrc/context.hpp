@@ -126,6 +126,7 @@ void prof_begin(Ctx& c, int slot);
 void prof_end(Ctx& c, int slot);
 void prof_flush(Ctx& c);
 void prof_release(Ctx& c);
+void prof_drop_tail(Ctx& c, int slot, size_t keep);
 
 void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s);
 

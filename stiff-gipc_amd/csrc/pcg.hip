@@ -389,18 +389,30 @@ struct Engine {
             }
         }
         int hc[C_NUM] = {0};
-        for (;;) {
-            for (int q = 0; q < pairs_per_check; ++q) {
-                if (use_graph) {
-                    SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
-                } else {
+        if (use_graph) {
+            for (;;) {
+                for (int q = 0; q < pairs_per_check; ++q) SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
+                SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipStreamSynchronize(s));
+                if (hc[C_STOP]) break;
+            }
+        } else {
+            // timed (eager) mode: same batching; the event records of launches
+            // after the stop (no work done) are dropped afterwards
+            const size_t base_spmv = c.prof.pend[PROF_SPMV].size();
+            const size_t base_mas = c.prof.pend[PROF_MAS_SOLVE].size();
+            for (;;) {
+                for (int q = 0; q < pairs_per_check; ++q) {
                     enqueue_iteration(1);
                     enqueue_iteration(0);
                 }
+                SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipStreamSynchronize(s));
+                if (hc[C_STOP]) break;
             }
-            SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
-            SGIPC_HIP(hipStreamSynchronize(s));
-            if (hc[C_STOP]) break;
+            const size_t useful = (size_t)(hc[C_NOCONV] ? hc[C_STOP] - 1 : hc[C_STOP]);
+            prof_drop_tail(c, PROF_SPMV, base_spmv + useful);
+            if (mas) prof_drop_tail(c, PROF_MAS_SOLVE, base_mas + useful);
         }
         converged = hc[C_NOCONV] == 0;
         return hc[C_STOP];

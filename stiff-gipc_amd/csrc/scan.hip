@@ -151,6 +151,19 @@ void prof_flush(Ctx& c) {
     }
 }
 
+// keep the first `keep` pending records of a slot, recycle the rest (used to
+// drop launches that ran after a device-side stop and did no work)
+void prof_drop_tail(Ctx& c, int slot, size_t keep) {
+    auto& v = c.prof.pend[slot];
+    if (v.size() <= keep) return;
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    for (size_t i = keep; i < v.size(); ++i) {
+        c.prof.pool.push_back(v[i].first);
+        c.prof.pool.push_back(v[i].second);
+    }
+    v.resize(keep);
+}
+
 void prof_release(Ctx& c) {
     for (int s = 0; s < PROF_NSLOTS; ++s) {
         for (auto& pr : c.prof.pend[s]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
