@@ -68,7 +68,8 @@ int sgipc_version(void);
 /* replaces GlobalLinearSystem construction (global_linear_system.h:24) */
 int sgipc_ctx_create(int device, sgipc_ctx** out);
 int sgipc_ctx_destroy(sgipc_ctx* ctx);
-/* run on a caller stream (hipStream_t); NULL restores the context's own */
+/* run on a caller stream (hipStream_t); NULL restores the context's own, a
+ * blocking stream (ordered with the legacy null stream) */
 int sgipc_ctx_set_stream(sgipc_ctx* ctx, void* hip_stream);
 int sgipc_ctx_synchronize(sgipc_ctx* ctx);
 const char* sgipc_last_error(const sgipc_ctx* ctx);
@@ -163,6 +164,24 @@ int sgipc_precond_apply(sgipc_ctx* ctx, const double* d_r, double* d_z);
 /* PCG solve of A x = b; PCGSolver::solve (linear_system/solver/pcg_solver.cu:158-267) */
 int sgipc_solve(sgipc_ctx* ctx, const double* d_b, double* d_x, const sgipc_solve_opts* opts,
                 sgipc_solve_stats* stats);
+
+/* ---- FP64 energies (line search) ------------------------------------------ */
+/* Fixed-order FP64 reductions, result on the host (synchronises the stream);
+ * the terms of GIPC::computeEnergy (StiffGIPC/GIPC.cu:10520-10579). */
+/* sum_i m_i |x_i - xtilde_i|^2 / 2; _getKineticEnergy_Reduction_3D
+ * (StiffGIPC/GIPC.cu:7330-7360) */
+int sgipc_kinetic_energy(sgipc_ctx* ctx, int n, const double* d_x, const double* d_xtilde, const double* d_mass,
+                         double* h_energy);
+/* dt^2 sum_t SNK1 energy (vol included); _getFEMEnergy_Reduction_3D
+ * (StiffGIPC/GIPC.cu:7515-7583) with __cal_StabbleNHK_energy1_3D
+ * (StiffGIPC/femEnergy.cu:1032-1050) */
+int sgipc_fem_energy(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d_tets, const double* d_DmInv,
+                     const double* d_vol, const double* d_mu, const double* d_lambda, double dt, double* h_energy);
+/* sum over int4 pairs of the RANK-2 barrier (mollified for parallel edges);
+ * _getBarrierEnergy_Reduction_3D (StiffGIPC/GIPC.cu:7768-7800) with
+ * __cal_Barrier_energy (:701-957).  dhat squared, as GIPC::dHat. */
+int sgipc_barrier_energy(sgipc_ctx* ctx, int n_pairs, const int* d_pairs, const double* d_x, const double* d_xrest,
+                         double dhat, double kappa, double* h_energy);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* HIP-event timing of selected launches on the context stream (no reference

@@ -11,6 +11,7 @@
 // s = sqrt(d / dhat)); the non-mollified Hessian is lambda0 g g^T, the
 // mollified one a PSD-projected 2x2 Gauss-Newton form in (c, s).
 #include "context.hpp"
+#include "pcg_dev.hpp"
 
 namespace sgipc {
 namespace {
@@ -231,6 +232,41 @@ __global__ void __launch_bounds__(128) k_barrier(int np, const int4* __restrict_
 }
 
 // ---- narrow phase: classification + encoding ---------------------------
+// Energy of one pair, RANK 2: kappa dhat^2 (I-1)^2 ln^2 I, times the
+// mollifier -I1 (I1 - 2 eps) / eps^2 for nearly parallel edges
+// (__cal_Barrier_energy, S/GIPC.cu:701-957); same distances as k_barrier.
+__device__ double pair_energy(int4 q, const double* __restrict__ x, const double* __restrict__ xr, double dhat,
+                              double kappa) {
+    Pair P = decode(q);
+    V3 X[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) X[a] = (a < P.nv) ? ld(x, P.v[a]) : V3{0, 0, 0};
+    double d;
+    if (P.dkind == 0) d = d_pp(X[0], X[1], nullptr);
+    else if (P.dkind == 1) d = d_pe(X[0], X[1], X[2], nullptr);
+    else if (P.dkind == 2) d = d_pt(X[0], X[1], X[2], X[3], nullptr);
+    else d = d_ee(X[0], X[1], X[2], X[3], nullptr);
+    const double I = d / dhat, b = bfun(I, kappa * dhat * dhat);
+    if (!P.moll) return b;
+    V3 v = cross(X[P.sl[1]] - X[P.sl[0]], X[P.sl[3]] - X[P.sl[2]]);
+    double cc = sqrt(dot(v, v)), I1 = cc * cc;
+    if (I1 == 0.0) return 0.0;  // no contribution, as in the reference
+    V3 r0 = ld(xr, P.v[P.sl[0]]) - ld(xr, P.v[P.sl[1]]);
+    V3 r1 = ld(xr, P.v[P.sl[2]]) - ld(xr, P.v[P.sl[3]]);
+    double eps = 1e-3 * dot(r0, r0) * dot(r1, r1);
+    return -I1 * (I1 - 2 * eps) / (eps * eps) * b;
+}
+
+__global__ void __launch_bounds__(256) k_barrier_energy(int np, const int4* __restrict__ pairs,
+                                                        const double* __restrict__ x, const double* __restrict__ xr,
+                                                        double dhat, double kappa, double* __restrict__ part) {
+    __shared__ double sh[4];
+    double e = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x)
+        e += pair_energy(pairs[p], x, xr, dhat, kappa);
+    write_partial(e, part, sh);
+}
+
 __device__ inline double det3c(V3 c0, V3 c1, V3 c2) { return dot(c0, cross(c1, c2)); }
 __device__ inline void pe_param(V3 v0, V3 a, V3 b, V3 nvec, double& px, double& py) {
     V3 b0 = b - a, b1 = cross(b0, nvec), b2 = v0 - a;
@@ -397,6 +433,16 @@ int classify_contacts(Ctx& c, int npt, const int* pt, int nee, const int* ee, co
     counts[1] = hcnt[2];
     counts[2] = hcnt[3];
     return hcnt[0];
+}
+
+double barrier_energy(Ctx& c, int np, const int* pairs, const double* x, const double* xr, double dhat,
+                      double kappa) {
+    double* part = energy_partials(c);
+    int g = energy_grid(np);
+    hipLaunchKernelGGL(k_barrier_energy, dim3(g), dim3(256), 0, c.stream, np, (const int4*)pairs, x, xr, dhat, kappa,
+                       part);
+    SGIPC_LAUNCH_CHECK();
+    return energy_finish(c, part, g);
 }
 
 }  // namespace sgipc

@@ -73,7 +73,9 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         if (device < 0 || device >= ndev) throw sgipc::arg_error("invalid device index");
         c->device = device;
         SGIPC_HIP(hipSetDevice(device));
-        SGIPC_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        // blocking stream: ordered with work on the legacy null stream, which
+        // callers (and torch's default stream) may use on the same buffers
+        SGIPC_HIP(hipStreamCreate(&c->own_stream));
         c->stream = c->own_stream;
     });
     if (rc != SGIPC_OK) { delete c; return rc; }
@@ -177,6 +179,7 @@ int sgipc_assemble_fem(sgipc_ctx* c, int T, const double* x, const uint32_t* tet
             throw sgipc::arg_error("assemble_fem: invalid argument");
         set_device(c);
         sgipc::fem_assemble(*c, T, x, tets, DmInv, vol, mu, lam, dt, btype, index_offset, triplet_offset, shape);
+        if (T > 0) c->static_begin = triplet_offset;  // FEM + mass triplets keep their indices across steps
     });
 }
 
@@ -304,6 +307,37 @@ int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts
             st->rel_residual = s2.rel_residual;
             st->rz0 = s2.rz0;
         }
+    });
+}
+
+int sgipc_kinetic_energy(sgipc_ctx* c, int n, const double* x, const double* xt, const double* m, double* e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || !e || (n > 0 && (!x || !xt || !m))) throw sgipc::arg_error("kinetic_energy: invalid argument");
+        set_device(c);
+        *e = sgipc::kinetic_energy(*c, n, x, xt, m);
+    });
+}
+
+int sgipc_fem_energy(sgipc_ctx* c, int T, const double* x, const uint32_t* tets, const double* DmInv,
+                     const double* vol, const double* mu, const double* lam, double dt, double* e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (T < 0 || !e || (T > 0 && (!x || !tets || !DmInv || !vol || !mu || !lam)))
+            throw sgipc::arg_error("fem_energy: invalid argument");
+        set_device(c);
+        *e = sgipc::fem_energy(*c, T, x, tets, DmInv, vol, mu, lam, dt);
+    });
+}
+
+int sgipc_barrier_energy(sgipc_ctx* c, int np, const int* pairs, const double* x, const double* xr, double dhat,
+                         double kappa, double* e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (np < 0 || !e || (np > 0 && (!pairs || !x || !xr)) || !(dhat > 0.0))
+            throw sgipc::arg_error("barrier_energy: invalid argument");
+        set_device(c);
+        *e = sgipc::barrier_energy(*c, np, pairs, x, xr, dhat, kappa);
     });
 }
 

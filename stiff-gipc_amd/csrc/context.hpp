@@ -9,17 +9,34 @@
 namespace sgipc {
 
 // Symmetric SpMV structure over the upper BCOO resident in the triplet view.
-// Block row j owns its upper blocks, BCOO entries [uniq_start[j],
-// uniq_start[j+1]) sorted by column, and a list of references {k, i} to the
-// upper blocks k = (i, j), i < j, applied transposed, sorted by k.
+// Block row j owns entries ent[eptr[j] .. eptr[j+1]) = {upper block index k,
+// column}: first the references to the upper blocks (i, j), i < j, applied
+// transposed (column i < j, sorted by k), then its own upper blocks (j, c),
+// c >= j, in BCOO order.
 struct SymBcoo {
     int nb = 0;         // block rows
     int K = 0;          // upper blocks
     int nlow = 0;       // strictly-lower references
-    DBuf<int> lptr;     // nb + 1
-    DBuf<int2> lref;    // nlow
+    DBuf<int> lptr;     // nb + 1 (build scratch: lower counts)
+    DBuf<int2> lref;    // nlow   (build scratch: lower references)
+    DBuf<int> eptr;     // nb + 1
+    DBuf<int2> ent;     // nlow + K
     DBuf<float> valf;   // 9 * K FP32 copy of the BCOO values (mixed precision)
     bool valf_ok = false;
+};
+
+// Sort of the static (FEM + mass) triplet segment, built where it started at
+// s0 and reused (ids shifted) while its ns row/col indices are unchanged
+// (crow/ccol hold the copy checked against on every convert).
+struct ConvertPlan {
+    bool valid = false;
+    long long s0 = 0, ns = 0;
+    int nbr = 0, Ks = 0;
+    DBuf<int> crow, ccol;       // ns: cached indices
+    DBuf<int> start, bucket;    // nbr + 1 row starts; ns bucketed ids
+    DBuf<int> sorted;           // ns: triplet ids sorted by (row, col, id)
+    DBuf<int> ucnt, ustart;     // unique blocks per row; row starts (nbr + 1)
+    DBuf<int> ucol, urun;       // per unique block: column, run start in sorted
 };
 
 struct MasState {
@@ -47,6 +64,9 @@ struct MasState {
     DBuf<double> inv64;      // nbanks * D(D+1)/2 symmetric inverse, upper triangle packed by rows (FP64)
     DBuf<float> inv32;       // same, FP32
     DBuf<int> l1lo, l1hi;    // per level-1 bank: range [lo, hi) of level-0 banks holding its children
+    DBuf<int> own_lo, own_hi;  // per level-1 bank: owned vertex range (slot-ordered vertices)
+    DBuf<double> proj_scratch; // per-workgroup partials of the top-level bank matrices
+    bool slot_ordered = false; // vertex ids increase with their slot
     int nb1 = 0;             // level-1 banks
     DBuf<double> R64, Z64;   // 3*total
     DBuf<float> R32, Z32;
@@ -107,6 +127,12 @@ struct Ctx {
     DBuf<double> bcoo_val; // 9K scratch before copy-back
     DBuf<int> bcoo_row, bcoo_col;
     DBuf<int> row_count, row_start, bucket, uniq_count, uniq_start;
+    DBuf<int> dyn_ustart, dyn_ucol, dyn_urun, merge_row;  // contact segment runs, merged rows
+    DBuf<int4> merge_desc;                                 // per merged block: contact / static runs
+    DBuf<int> long_rows, dev_int2;                         // rows sorted by one workgroup; their count
+    DBuf<double> energy_part;                              // energy partials (+ total)
+    long long static_begin = -1;  // first triplet of the FEM + mass segment (set by FEM assembly)
+    ConvertPlan cplan;
     DBuf<int> dev_int;    // small device ints (totals)
     SymBcoo bsr;
     bool have_bsr = false;
@@ -146,6 +172,16 @@ void bsr_build(Ctx& c);
 const float* bsr_values_f32(Ctx& c);
 void spmv64(Ctx& c, double a, const double* x, double b, double* y);
 
+double kinetic_energy(Ctx& c, int n, const double* x, const double* xt, const double* m);
+double fem_energy(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                  const double* mu, const double* lam, double dt);
+double barrier_energy(Ctx& c, int np, const int* pairs, const double* x, const double* xr, double dhat,
+                      double kappa);
+// energy reductions: workgroups for n items (256 threads), partial buffer,
+// fixed-order sum of g partials to the host
+int energy_grid(long long n);
+double* energy_partials(Ctx& c);
+double energy_finish(Ctx& c, double* part, int g);
 void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, const unsigned* nnum,
                       const unsigned* nlist, int nlist_len, const int* part_map, int M,
                       const int* real_map, int bank);

@@ -1,0 +1,117 @@
+// FP64 energies used by the Newton line search: kinetic and SNK1 elastic
+// here, IPC barrier in barrier.hip (sharing its distance code); each a
+// fixed-order reduction (per-workgroup partials summed
+// by one workgroup in index order), so repeated evaluations are bitwise
+// identical.
+//
+// Reference: GIPC::computeEnergy (S/GIPC.cu:10520-10579) over
+// Energy_Add_Reduction_Algorithm (:10347-10470): type 0
+// _getKineticEnergy_Reduction_3D (:7330-7360), type 1
+// _getFEMEnergy_Reduction_3D (:7515-7583) with __cal_StabbleNHK_energy1_3D
+// (S/femEnergy.cu:1032-1050), scaled by dt^2 after the sum, type 2
+// _getBarrierEnergy_Reduction_3D (:7768-7800) with __cal_Barrier_energy
+// (:701-957, RANK 2).
+#include "context.hpp"
+#include "pcg_dev.hpp"
+
+namespace sgipc {
+namespace {
+
+constexpr int kEThreads = 256;
+constexpr int kEParts = 1024;  // partials per energy
+
+struct E3 {
+    double x, y, z;
+};
+__device__ inline E3 ld3(const double* p, int i) { return {p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]}; }
+__device__ inline E3 sub(E3 a, E3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline double dot(E3 a, E3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// SNK1: 0.5 vol [mu (I2 - 3) + lam (J - 1 - mu/lam)^2], F = Ds DmInv
+__global__ void __launch_bounds__(kEThreads) k_fem_energy(int T, const double* __restrict__ x,
+                                                          const uint4* __restrict__ tets,
+                                                          const double* __restrict__ DmInv,
+                                                          const double* __restrict__ vol, const double* __restrict__ mu,
+                                                          const double* __restrict__ lam, double* __restrict__ part) {
+    __shared__ double sh[kEThreads / 64];
+    double e = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        uint4 q = tets[t];
+        E3 x0 = ld3(x, q.x), x1 = ld3(x, q.y), x2 = ld3(x, q.z), x3 = ld3(x, q.w);
+        E3 c0 = sub(x1, x0), c1 = sub(x2, x0), c2 = sub(x3, x0);  // Ds columns
+        const double* D = DmInv + 9 * (size_t)t;                 // row-major
+        double F[9];  // row-major F(i, j) = sum_k Ds(i, k) D(k, j)
+        const double ds[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) F[3 * i + j] = ds[i][0] * D[j] + ds[i][1] * D[3 + j] + ds[i][2] * D[6 + j];
+        double I2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) I2 += F[i] * F[i];
+        double J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                   F[2] * (F[3] * F[7] - F[4] * F[6]);
+        double m = mu[t], l = lam[t], Jm = J - 1.0 - m / l;
+        e += 0.5 * (m * (I2 - 3.0) + l * Jm * Jm) * vol[t];
+    }
+    write_partial(e, part, sh);
+}
+
+__global__ void __launch_bounds__(kEThreads) k_kinetic_energy(int n, const double* __restrict__ x,
+                                                              const double* __restrict__ xt,
+                                                              const double* __restrict__ m, double* __restrict__ part) {
+    __shared__ double sh[kEThreads / 64];
+    double e = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        E3 d = sub(ld3(x, i), ld3(xt, i));
+        e += dot(d, d) * m[i] * 0.5;
+    }
+    write_partial(e, part, sh);
+}
+
+__global__ void __launch_bounds__(kEThreads) k_energy_total(const double* __restrict__ part, int np,
+                                                            double* __restrict__ out) {
+    __shared__ double sh[kEThreads / 64];
+    double s = sum_partials(part, np, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace
+
+int energy_grid(long long n) {
+    long long g = div_up(n, kEThreads);
+    return (int)(g < 1 ? 1 : (g > kEParts ? kEParts : g));
+}
+
+double* energy_partials(Ctx& c) { return c.energy_part.reserve(kEParts + 1); }
+
+// sum of g per-workgroup partials (fixed order) to the host
+double energy_finish(Ctx& c, double* part, int g) {
+    double* out = part + kEParts;
+    hipLaunchKernelGGL(k_energy_total, dim3(1), dim3(kEThreads), 0, c.stream, (const double*)part, g, out);
+    SGIPC_LAUNCH_CHECK();
+    double h = 0.0;
+    SGIPC_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+double kinetic_energy(Ctx& c, int n, const double* x, const double* xt, const double* m) {
+    double* part = energy_partials(c);
+    int g = energy_grid(n);
+    hipLaunchKernelGGL(k_kinetic_energy, dim3(g), dim3(kEThreads), 0, c.stream, n, x, xt, m, part);
+    SGIPC_LAUNCH_CHECK();
+    return energy_finish(c, part, g);
+}
+
+double fem_energy(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                  const double* mu, const double* lam, double dt) {
+    double* part = energy_partials(c);
+    int g = energy_grid(T);
+    hipLaunchKernelGGL(k_fem_energy, dim3(g), dim3(kEThreads), 0, c.stream, T, x, (const uint4*)tets, DmInv, vol, mu,
+                       lam, part);
+    SGIPC_LAUNCH_CHECK();
+    return dt * dt * energy_finish(c, part, g);
+}
+
+}  // namespace sgipc

@@ -15,6 +15,7 @@
 // a deterministic per-bank component sum (no FP atomics); the apply is a
 // bank-per-workgroup dense GEMV on the symmetric inverse.
 #include "context.hpp"
+#include <utility>
 #include "pcg_dev.hpp"
 #include <algorithm>
 
@@ -357,6 +358,133 @@ __global__ void __launch_bounds__(256) k_project(int K, const int* __restrict__ 
         if (acc[i] != 0.0) atomicAdd(g + i, acc[i]);
 }
 
+// Projection of one BCOO block H = A(r, c) (r <= c, column-major) onto the
+// level-l bank entry of its node pair: the packed index and the 3x3 block
+// (H + H^T on a coarse diagonal, H^T below the bank diagonal).
+__device__ inline int project_block(int B, int li, int lj, bool same_vertex, const double* H, double* blk) {
+    if (li == lj) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) blk[p + 3 * q] = same_vertex ? H[p + 3 * q] : H[p + 3 * q] + H[q + 3 * p];
+        return pack_index(B, li, li);
+    }
+    if (li < lj) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) blk[e] = H[e];
+        return pack_index(B, li, lj);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) blk[p + 3 * q] = H[q + 3 * p];
+    return pack_index(B, lj, li);
+}
+
+// Galerkin projection for 16-vertex banks when vertex ids follow slot order:
+// workgroup b owns the vertices of the level-0 banks whose first child lands
+// in level-1 bank b, i.e. the contiguous BCOO rows [vlo_b, vhi_b).  Level 0
+// targets are unique (plain stores).  Levels 1..win_levels accumulate in an
+// LDS window of two banks around this workgroup's nodes (targets outside
+// fall back to global atomics) and are flushed once; the top levels
+// (>= top_lvl) accumulate in LDS and are written as per-workgroup partials,
+// summed afterwards in a fixed order (k_project_top_reduce).  Other levels
+// use global FP64 atomics.
+constexpr int kProjWin = 2;
+constexpr int kProjWinLevels = 2;
+constexpr int kProjBank16 = 136 * 9;  // doubles per packed 16-vertex bank
+constexpr int kProjTopBanks = 6;
+__global__ void __launch_bounds__(512) k_project_banked(
+    const int* __restrict__ ustart, const int* __restrict__ rows, const int* __restrict__ cols,
+    const double* __restrict__ vals, int off, int n, int L, const int* __restrict__ node_of,
+    const int* __restrict__ vlo, const int* __restrict__ vhi, int nbanks, double* __restrict__ bank_upper,
+    int win_levels, int top_lvl, int top_lo, int top_nb, double* __restrict__ top_scratch) {
+    constexpr int B = 16, NB = kProjBank16;
+    __shared__ double win[kProjWinLevels * kProjWin * NB];
+    __shared__ double top[kProjTopBanks * NB];
+    __shared__ int wbase[kProjWinLevels];
+    const int b = blockIdx.x, t = threadIdx.x;
+    for (int i = t; i < kProjWinLevels * kProjWin * NB; i += blockDim.x) win[i] = 0.0;
+    for (int i = t; i < top_nb * NB; i += blockDim.x) top[i] = 0.0;
+    const int v0 = vlo[b], v1 = vhi[b];
+    if (t < kProjWinLevels) wbase[t] = (v0 < v1 && t + 1 < L) ? node_of[(size_t)(t + 1) * n + v0] / B : -1;
+    __syncthreads();
+    if (v0 < v1) {
+        const int k0 = ustart[v0 + off], k1 = ustart[v1 + off];
+        for (int k = k0 + t; k < k1; k += blockDim.x) {
+            const int r = rows[k] - off, c = cols[k] - off;
+            if (c < 0 || c >= n) continue;
+            double H[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
+            for (int l = 0; l < L; ++l) {
+                const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
+                if (I / B != J / B) continue;
+                const int bank = I / B;
+                double blk[9];
+                const int pidx = project_block(B, I % B, J % B, r == c, H, blk);
+                if (l == 0) {
+                    double* dst = bank_upper + (size_t)bank * NB + 9 * pidx;  // unique per BCOO block
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) dst[e] += blk[e];
+                    continue;
+                }
+                double* dst;
+                if (l >= top_lvl) {
+                    dst = top + (size_t)(bank - top_lo) * NB + 9 * pidx;
+                } else if (l <= win_levels && bank - wbase[l - 1] >= 0 && bank - wbase[l - 1] < kProjWin) {
+                    dst = win + ((size_t)(l - 1) * kProjWin + (bank - wbase[l - 1])) * NB + 9 * pidx;
+                } else {
+                    dst = bank_upper + (size_t)bank * NB + 9 * pidx;
+                }
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = t; q < win_levels * kProjWin * NB; q += blockDim.x) {
+        const double v = win[q];
+        const int lw = q / (kProjWin * NB), w = (q / NB) % kProjWin, e = q % NB;
+        const int bank = wbase[lw] + w;
+        if (v != 0.0 && bank >= 0 && bank < nbanks) atomicAdd(bank_upper + (size_t)bank * NB + e, v);
+    }
+    double* out = top_scratch + (size_t)b * top_nb * NB;
+    for (int q = t; q < top_nb * NB; q += blockDim.x) out[q] = top[q];
+}
+
+// top-level bank entries = sum of the per-workgroup partials, fixed order
+// (four interleaved partial sums per entry, combined in order)
+__global__ void __launch_bounds__(256) k_project_top_reduce(const double* __restrict__ scratch, int nwg, int len,
+                                                            double* __restrict__ dst) {
+    __shared__ double part[4][64];
+    const int e = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+    double s = 0.0;
+    if (e < len)
+        for (int w = q; w < nwg; w += 4) s += scratch[(size_t)w * len + e];
+    part[q][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (q == 0 && e < len) dst[e] = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                                    part[3][threadIdx.x];
+}
+
+// vertex range owned by each level-1 bank (slot-ordered vertex ids)
+__global__ void k_owned_vertices(int nb0, int B, const int* __restrict__ prefix0, const int* __restrict__ psum0,
+                                 const int* __restrict__ part_map, int* __restrict__ vlo, int* __restrict__ vhi) {
+    int cb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cb >= nb0 || prefix0[cb] <= 0) return;
+    int mn = 0x7fffffff, mx = -1;
+    for (int i = 0; i < B; ++i) {
+        int v = part_map[cb * B + i];
+        if (v >= 0) { mn = min(mn, v); mx = max(mx, v); }
+    }
+    if (mx < 0) return;
+    int f = psum0[cb] / B;
+    atomicMin(vlo + f, mn);
+    atomicMax(vhi + f, mx + 1);
+}
+
 // element (r, c), r <= c, of a symmetric D x D matrix stored as its upper
 // triangle packed by rows
 template <int D>
@@ -412,6 +540,70 @@ __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_inverse(int nbanks,
             o64[e] = S[r][i];
             o32[e] = (float)S[r][i];
         }
+    }
+}
+
+// Same elimination for 16-vertex banks (D = 48) with the matrix held in
+// registers: thread i owns column i; at step j the owner of column j
+// publishes it (double-buffered LDS, one barrier per step) and every thread
+// updates its own column.  Steps are instantiated per compile-time j so every
+// register index is static.  Arithmetic and order equal k_bank_inverse's.
+template <int J, int D>
+__device__ __forceinline__ void gj_step(double (&S)[D], double (*colm)[D], int i) {
+    double* cm = colm[J & 1];
+    if (i == J) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) cm[k] = S[k];
+    }
+    __syncthreads();
+    const double rt = cm[J];
+    if (i == J) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) S[k] = (k == J) ? 1.0 : 0.0;
+    }
+    const double pv = S[J] / rt;
+    S[J] = pv;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        if (k != J) S[k] += -cm[k] * pv;
+}
+
+template <int D, int... J>
+__device__ __forceinline__ void gj_all(double (&S)[D], double (*colm)[D], int i, std::integer_sequence<int, J...>) {
+    (gj_step<J, D>(S, colm, i), ...);
+}
+
+__global__ void __launch_bounds__(64) k_bank_inverse_reg48(int nbanks, const double* __restrict__ bank_upper,
+                                                           double* __restrict__ inv64, float* __restrict__ inv32) {
+    constexpr int B = 16, D = 48, npack = B * (B + 1) / 2, NP = D * (D + 1) / 2;
+    __shared__ double colm[2][D];
+    const int b = blockIdx.x, i = threadIdx.x;
+    const bool act = i < D;
+    const double* src = bank_upper + (size_t)b * npack * 9;
+    double S[D];
+    const int ci = i / 3, qi = i % 3;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const int rj = j / 3, pj = j % 3;
+        double v = 0.0;
+        if (act) {
+            v = (ci >= rj) ? src[(size_t)pack_index(B, rj, ci) * 9 + pj + 3 * qi]
+                           : src[(size_t)pack_index(B, ci, rj) * 9 + qi + 3 * pj];
+            if (i == j && v == 0.0) v = 1.0;
+        }
+        S[j] = v;
+    }
+    gj_all<D>(S, colm, i, std::make_integer_sequence<int, D>{});
+    if (act) {
+        double* o64 = inv64 + (size_t)b * NP;
+        float* o32 = inv32 + (size_t)b * NP;
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+            if (r <= i) {
+                const int e = sym_index<D>(r, i);
+                o64[e] = S[r];
+                o32[e] = (float)S[r];
+            }
     }
 }
 
@@ -758,6 +950,9 @@ void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, co
     m.ready = false;
     m.bank = bank;
     m.n = n;
+    m.slot_ordered = true;  // vertex ids increase with their bank slot
+    for (int v = 1; v < n; ++v)
+        if (real_map[v] <= real_map[v - 1]) { m.slot_ordered = false; break; }
     m.node_offset = node_offset;
     m.M = M;
     m.L = num_levels(std::max(M, n), bank);
@@ -875,7 +1070,40 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         }
     }
     fill<double>(m.bank_upper.p, 0.0, (long long)m.nbanks * npack * 9, s);
-    if (c.K > 0) {
+    const bool banked = (B == 16) && m.slot_ordered && m.nb1 > 0 && L >= 2;
+    if (c.K > 0 && banked) {
+        // top levels: the highest levels whose banks fit kProjTopBanks
+        int top_lvl = L, top_lo = m.nbanks, top_nb = 0;
+        for (int l = L - 1; l >= 1; --l) {
+            int beg = m.level_size_h[2 * l + 1] / B;
+            int end = m.level_size_h[2 * (l + 1) + 1] / B;
+            if (top_nb + (end - beg) > kProjTopBanks) break;
+            top_nb += end - beg;
+            top_lo = beg;
+            top_lvl = l;
+        }
+        const int win_levels = std::min(kProjWinLevels, top_lvl - 1);
+        const int nb0 = M / B;
+        int* vlo = m.own_lo.reserve((size_t)m.nb1 + 1);
+        int* vhi = m.own_hi.reserve((size_t)m.nb1 + 1);
+        fill<int>(vlo, 0x7fffffff, m.nb1, s);
+        fill<int>(vhi, 0, m.nb1, s);
+        hipLaunchKernelGGL(k_owned_vertices, dim3(div_up(nb0, 256)), dim3(256), 0, s, nb0, B, m.prefix0.p, m.psum0.p,
+                           m.part_map.p, vlo, vhi);
+        SGIPC_LAUNCH_CHECK();
+        const int top_len = top_nb * kProjBank16;
+        double* scratch = m.proj_scratch.reserve((size_t)m.nb1 * (top_len > 0 ? top_len : 1));
+        hipLaunchKernelGGL(k_project_banked, dim3(m.nb1), dim3(512), 0, s, (const int*)c.uniq_start.p,
+                           (const int*)c.t_rows, (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, L,
+                           (const int*)m.node_of.p, (const int*)vlo, (const int*)vhi, m.nbanks, m.bank_upper.p,
+                           win_levels, top_lvl, top_lo, top_nb, scratch);
+        SGIPC_LAUNCH_CHECK();
+        if (top_len > 0) {
+            hipLaunchKernelGGL(k_project_top_reduce, dim3(div_up(top_len, 64)), dim3(256), 0, s,
+                               (const double*)scratch, m.nb1, top_len, m.bank_upper.p + (size_t)top_lo * kProjBank16);
+            SGIPC_LAUNCH_CHECK();
+        }
+    } else if (c.K > 0) {
         // coarse levels whose banks fit the LDS accumulator (top levels first)
         int lds_lo = m.nbanks, lds_nb = 0;
         for (int l = L - 1; l >= 1; --l) {
@@ -892,7 +1120,7 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         SGIPC_LAUNCH_CHECK();
     }
     if (B == 16)
-        hipLaunchKernelGGL(k_bank_inverse<16>, dim3(m.nbanks), dim3(64), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,
+        hipLaunchKernelGGL(k_bank_inverse_reg48, dim3(m.nbanks), dim3(64), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,
                            m.inv32.p);
     else
         hipLaunchKernelGGL(k_bank_inverse<32>, dim3(m.nbanks), dim3(128), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,

@@ -47,8 +47,7 @@ __global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict
 // per-workgroup partials of p.Ap.  First kernel of an iteration: enforces
 // the iteration budget (iteration k runs while k < max_iter).
 template <int G, typename MV, typename V>
-__global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ lptr, const int2* __restrict__ lref,
-                                                       const int* __restrict__ ustart, const int* __restrict__ ucol,
+__global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
                                                        const MV* __restrict__ val, const V* __restrict__ p,
                                                        V* __restrict__ Ap, double* __restrict__ part,
                                                        int* __restrict__ ctrl) {
@@ -66,7 +65,7 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
     for (int ch = w.first; ch < w.end; ch += w.step) {
         int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         MV s0 = 0, s1 = 0, s2 = 0;
-        if (row < nb) sym_row_lane<G, MV>(row, sub, lptr, lref, ustart, ucol, val, pv, s0, s1, s2);
+        if (row < nb) sym_row_lane<G, MV>(row, sub, eptr, ent, val, pv, s0, s1, s2);
         group_sum3<G, MV>(s0, s1, s2);
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;
@@ -284,8 +283,8 @@ struct Engine {
         double* rz_last = part_rz[par ^ 1];  // rz_(k-1)
         double* rz_this = part_rz[par];      // receives rz_k
         prof_begin(c, PROF_SPMV);
-        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.lptr.p, S.lref.p,
-                           c.uniq_start.p, c.t_cols, val, (const V*)p, Ap, part_pap, ctrl);
+        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val,
+                           (const V*)p, Ap, part_pap, ctrl);
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
         if (mas) {
@@ -339,8 +338,7 @@ struct Engine {
         auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
         const SymBcoo& S = c.bsr;
         mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)gU); mix((size_t)S.nb); mix((size_t)S.K);
-        mix((size_t)S.nlow); mix((size_t)S.lptr.p); mix((size_t)S.lref.p); mix((size_t)c.uniq_start.p);
-        mix((size_t)c.t_cols); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
+        mix((size_t)S.nlow); mix((size_t)S.eptr.p); mix((size_t)S.ent.p); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);

@@ -11,8 +11,7 @@ namespace sgipc {
 namespace {
 
 template <int G, typename TV, typename TX, typename TY>
-__global__ void __launch_bounds__(256) k_spmv_sym(int nb, const int* __restrict__ lptr, const int2* __restrict__ lref,
-                                                  const int* __restrict__ ustart, const int* __restrict__ ucol,
+__global__ void __launch_bounds__(256) k_spmv_sym(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
                                                   const TV* __restrict__ vals, double a, const TX* __restrict__ x,
                                                   double b, TY* __restrict__ y) {
     constexpr int RPB = 256 / G;
@@ -22,7 +21,7 @@ __global__ void __launch_bounds__(256) k_spmv_sym(int nb, const int* __restrict_
         int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         TV s0 = 0, s1 = 0, s2 = 0;
         auto xv = [x](size_t i) { return x[i]; };
-        if (row < nb) sym_row_lane<G, TV>(row, sub, lptr, lref, ustart, ucol, vals, xv, s0, s1, s2);
+        if (row < nb) sym_row_lane<G, TV>(row, sub, eptr, ent, vals, xv, s0, s1, s2);
         group_sum3<G, TV>(s0, s1, s2);
         if (sub == 0 && row < nb) {
             size_t o = 3 * (size_t)row;
@@ -70,6 +69,20 @@ __global__ void k_low_sort(int nb, const int* __restrict__ lptr, int2* __restric
     }
 }
 
+// per-row entry list: transposed references, then the row's upper blocks
+__global__ void k_entries(int nb, const int* __restrict__ lptr, const int2* __restrict__ lref,
+                          const int* __restrict__ ustart, const int* __restrict__ ucol, int* __restrict__ eptr,
+                          int2* __restrict__ ent) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nb) return;
+    int base = lptr[j] + ustart[j];
+    eptr[j] = base;
+    if (j == nb) return;
+    int nl = lptr[j + 1] - lptr[j];
+    for (int e = 0; e < nl; ++e) ent[base + e] = lref[lptr[j] + e];
+    for (int k = ustart[j]; k < ustart[j + 1]; ++k) ent[base + nl + (k - ustart[j])] = make_int2(k, ucol[k]);
+}
+
 __global__ void k_to_f32(long long n, const double* __restrict__ s, float* __restrict__ d) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         d[i] = (float)s[i];
@@ -108,6 +121,11 @@ void bsr_build(Ctx& c) {
         hipLaunchKernelGGL(k_low_sort, dim3(div_up(nb, 256)), dim3(256), 0, s, nb, lptr, lref);
         SGIPC_LAUNCH_CHECK();
     }
+    int* eptr = S.eptr.reserve((size_t)nb + 2);
+    int2* ent = S.ent.reserve((size_t)nlow + (size_t)K + 1);
+    hipLaunchKernelGGL(k_entries, dim3(div_up(nb + 1, 256)), dim3(256), 0, s, nb, (const int*)lptr,
+                       (const int2*)lref, (const int*)c.uniq_start.p, (const int*)c.t_cols, eptr, ent);
+    SGIPC_LAUNCH_CHECK();
     S.valf_ok = false;
     c.have_bsr = true;
 }
@@ -133,8 +151,8 @@ void spmv64(Ctx& c, double a, const double* x, double b, double* y) {
     constexpr int G = 8;
     int nchunks = div_up((long long)S.nb, 256 / G);
     int blocks = std::max(8, (int)div_up(nchunks, 8) * 8);
-    hipLaunchKernelGGL((k_spmv_sym<G, double, double, double>), dim3(blocks), dim3(256), 0, c.stream, S.nb, S.lptr.p,
-                       S.lref.p, c.uniq_start.p, c.t_cols, (const double*)c.t_vals, a, x, b, y);
+    hipLaunchKernelGGL((k_spmv_sym<G, double, double, double>), dim3(blocks), dim3(256), 0, c.stream, S.nb, S.eptr.p,
+                       S.ent.p, (const double*)c.t_vals, a, x, b, y);
     SGIPC_LAUNCH_CHECK();
 }
 

@@ -4,7 +4,7 @@
 // warp row sums plus FP64 atomics for the transposed half).  Here a group of
 // G lanes owns one block row j and sums, in a fixed order, the transposed
 // references A(i,j)^T x_i (i < j) and then the row's own upper blocks
-// A(j,k) x_k.  The values are read in place from the converted BCOO
+// A(j,k) x_k, both listed in one per-row entry array.  The values are read in place from the converted BCOO
 // (9 doubles per block, column-major): no atomics, no second copy of the
 // matrix, deterministic.
 #pragma once
@@ -47,30 +47,48 @@ __device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, T
     }
 }
 
-// partial row sums of this lane (before the group reduction); xv(i) returns
-// entry i of the vector being multiplied
+// one block-times-vector term: block k (transposed when tr) times x_j
+template <typename TV, typename XF>
+__device__ inline void block_term(const TV* __restrict__ vals, int k, int j, bool tr, const XF& xv, TV& s0, TV& s1,
+                                  TV& s2) {
+    TV a[9];
+    load_block<TV>(vals, k, tr, a);
+    TV x0 = (TV)xv(3 * (size_t)j), x1 = (TV)xv(3 * (size_t)j + 1), x2 = (TV)xv(3 * (size_t)j + 2);
+    s0 += a[0] * x0 + a[3] * x1 + a[6] * x2;
+    s1 += a[1] * x0 + a[4] * x1 + a[7] * x2;
+    s2 += a[2] * x0 + a[5] * x1 + a[8] * x2;
+}
+
+// Partial row sums of this lane (before the group reduction).  Row `row`
+// owns entries ent[eptr[row] .. eptr[row+1]) = {upper block index, column}:
+// first its transposed references (column < row), then its own upper
+// blocks.  Lane `sub` takes entries sub, sub+G, ...  xv(i) returns entry i
+// of the vector being multiplied.
 template <int G, typename TV, typename XF>
-__device__ inline void sym_row_lane(int row, int sub, const int* __restrict__ lptr, const int2* __restrict__ lref,
-                                    const int* __restrict__ ustart, const int* __restrict__ ucol,
+__device__ inline void sym_row_lane(int row, int sub, const int* __restrict__ eptr, const int2* __restrict__ ent,
                                     const TV* __restrict__ vals, const XF& xv, TV& s0, TV& s1, TV& s2) {
-    int l0 = lptr[row], nl = lptr[row + 1] - l0;
-    int u0 = ustart[row], nu = ustart[row + 1] - u0;
-    for (int e = sub; e < nl + nu; e += G) {
-        int k, j;
-        bool tr = e < nl;
-        if (tr) {
-            int2 q = lref[l0 + e];
-            k = q.x; j = q.y;
-        } else {
-            k = u0 + (e - nl);
-            j = ucol[k];
-        }
-        TV a[9];
-        load_block<TV>(vals, k, tr, a);
-        TV x0 = (TV)xv(3 * (size_t)j), x1 = (TV)xv(3 * (size_t)j + 1), x2 = (TV)xv(3 * (size_t)j + 2);
+    const int e0 = eptr[row] + sub, end = eptr[row + 1];
+    int e = e0;
+    // two entries in flight per lane for 4-byte values; 8-byte values would
+    // cost occupancy (registers) for no gain
+    if (sizeof(TV) == 4 && e + G < end) {
+        int2 q0 = ent[e], q1 = ent[e + G];
+        TV a[9], b[9];
+        load_block<TV>(vals, q0.x, q0.y < row, a);
+        load_block<TV>(vals, q1.x, q1.y < row, b);
+        TV x0 = (TV)xv(3 * (size_t)q0.y), x1 = (TV)xv(3 * (size_t)q0.y + 1), x2 = (TV)xv(3 * (size_t)q0.y + 2);
+        TV y0 = (TV)xv(3 * (size_t)q1.y), y1 = (TV)xv(3 * (size_t)q1.y + 1), y2 = (TV)xv(3 * (size_t)q1.y + 2);
         s0 += a[0] * x0 + a[3] * x1 + a[6] * x2;
         s1 += a[1] * x0 + a[4] * x1 + a[7] * x2;
         s2 += a[2] * x0 + a[5] * x1 + a[8] * x2;
+        s0 += b[0] * y0 + b[3] * y1 + b[6] * y2;
+        s1 += b[1] * y0 + b[4] * y1 + b[7] * y2;
+        s2 += b[2] * y0 + b[5] * y1 + b[8] * y2;
+        e += 2 * G;
+    }
+    for (; e < end; e += G) {
+        int2 q = ent[e];
+        block_term<TV>(vals, q.x, q.y, q.y < row, xv, s0, s1, s2);
     }
 }
 

@@ -180,6 +180,19 @@ class NewtonSystem:
         self.build_precond()
         return self.solve(**solve_kw)
 
+    def energies(self):
+        """FP64 line-search energies at the current x: kinetic, elastic
+        (dt^2-scaled SNK1) and barrier (current contact pairs)."""
+        s = self.sc
+        ek = C.c_double(0.0); ef = C.c_double(0.0); eb = C.c_double(0.0)
+        self.ctx.call("sgipc_kinetic_energy", self.n, _p(self.x), _p(self.x_tilde), _p(self.mass), C.byref(ek))
+        self.ctx.call("sgipc_fem_energy", self.T, _p(self.x), _p(self.tets), _p(self.DmInv), _p(self.vol),
+                      _p(self.mu), _p(self.lam), float(s.dt), C.byref(ef))
+        if self.n_pairs:
+            self.ctx.call("sgipc_barrier_energy", self.n_pairs, _p(self.pairs), _p(self.x), _p(self.x_rest),
+                          float(s.dhat), float(s.kappa), C.byref(eb))
+        return {"kinetic": ek.value, "elastic": ef.value, "barrier": eb.value}
+
     # ---- inspection -----------------------------------------------------------
     def triplets(self):
         torch.cuda.synchronize(self.dev)

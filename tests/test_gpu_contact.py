@@ -98,3 +98,23 @@ def test_contact_newton_step_matches_oracle(torch_cuda, oracle, twobox8):
     A = dense_from_bcoo(kr, kc, kv, sc.n)
     x = sysm.dx.cpu().numpy()
     assert st.converged and np.linalg.norm(A @ x - ref["b"]) <= 1.5e-6 * np.linalg.norm(ref["b"])
+
+
+def test_convert_plan_reuse_is_bitexact(torch_cuda, oracle, twobox8):
+    """Convert keeps the sort of the FEM + mass segment between calls; the
+    BCOO must equal the oracle's full sort bit for bit when the plan is
+    built, reused, and reused after the contact prefix changed length."""
+    from sgipc.system import NewtonSystem
+    sc, pt, ee = twobox8
+    sysm = NewtonSystem(sc, "cuda:0")
+    for step, (p, e) in enumerate([(pt, ee), (pt, ee), (pt[: len(pt) // 2], ee[: len(ee) // 3]), (pt, ee)]):
+        print("case: convert plan step %d (%d PT, %d EE candidates)" % (step, len(p), len(e)), flush=True)
+        sysm.setup_contacts(p, e)
+        sysm.assemble()
+        R, Cc, V = sysm.triplets()
+        K = sysm.convert()
+        kr, kc, kv = oracle.convert(R, Cc, V)
+        gr, gc, gv = sysm.bcoo()
+        assert K == len(kr)
+        assert np.array_equal(gr, kr) and np.array_equal(gc, kc)
+        assert np.array_equal(gv, kv)
