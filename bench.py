@@ -53,11 +53,12 @@ def copy_calibration(mib):
 
 
 def spmv_alg_bytes(K, nb, fp32):
-    # upper BCOO: 9 values (72 B FP64 / 36 B FP32) + 8 B (row, col) per
-    # block; x gathered once and y written once per block row
+    # the PCG SpMV kernel (direction update fused): upper BCOO, 9 values
+    # (72 B FP64 / 36 B FP32) + 8 B (row, col) per block; per block row z
+    # read once, p and Ap each read and written once
     vb = 36.0 if fp32 else 72.0
     xb = 12.0 if fp32 else 24.0
-    return (vb + 8.0) * K + 2.0 * xb * nb
+    return (vb + 8.0) * K + 5.0 * xb * nb
 
 
 def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5):
@@ -184,8 +185,8 @@ def main():
                          "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, 1 thread, "
                          "%.1f s" % (csteps, cit, csecs)}
 
-    kname = "k_spmv_pap<8, float, float> (PCG SpMV, FP32 values)" if fp32 else \
-        "k_spmv_pap<8, double, double> (PCG SpMV, FP64)"
+    kname = "k_spmv_pap<8, float, float> (PCG SpMV + direction update, FP32 values)" if fp32 else \
+        "k_spmv_pap<8, double, double> (PCG SpMV + direction update, FP64)"
     out = {
         "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 true rel. residual)",
         "value": round(r["value"], 4),

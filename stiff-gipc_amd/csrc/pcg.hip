@@ -43,21 +43,35 @@ __global__ void __launch_bounds__(kThreads) k_store_sum(const double* __restrict
     if (threadIdx.x == 0) sc[idx] = s;
 }
 
-// Ap = A p (symmetric, upper BCOO + transposed references) with
-// per-workgroup partials of p.Ap.  First kernel of an iteration: enforces
-// the iteration budget (iteration k runs while k < max_iter).
+// Direction update fused with the SpMV.  With beta = rz_(k-1) / rz_(k-2)
+// (0 at k = 1): p_k = z + beta p_(k-1) and, by linearity,
+// A p_k = A z + beta A p_(k-1), so the kernel gathers only z (symmetric
+// SpMV over the upper BCOO + transposed references), forms p and Ap row by
+// row in place, and writes per-workgroup partials of p.Ap.  First kernel of
+// an iteration: enforces the iteration budget (iteration k runs while
+// k < max_iter).
 template <int G, typename MV, typename V>
 __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
-                                                       const MV* __restrict__ val, const V* __restrict__ p,
-                                                       V* __restrict__ Ap, double* __restrict__ part,
-                                                       int* __restrict__ ctrl) {
+                                                       const MV* __restrict__ val, const V* __restrict__ z,
+                                                       V* __restrict__ p, V* __restrict__ Ap,
+                                                       const double* __restrict__ rz_last,
+                                                       const double* __restrict__ rz_prev, int np_rz,
+                                                       double* __restrict__ part, int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
-    if (ctrl[C_ITER] >= ctrl[C_MAXIT]) {
+    const int iter = ctrl[C_ITER];
+    if (iter >= ctrl[C_MAXIT]) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
         return;
     }
     __shared__ double sh[kThreads / 64];
-    auto pv = [p](size_t i) { return p[i]; };
+    const bool first = iter == 1;
+    double be = 0.0;
+    if (!first) {
+        const double rn = sum_partials(rz_last, np_rz, sh);
+        const double ro = sum_partials(rz_prev, np_rz, sh);
+        be = rn / ro;
+    }
+    auto zv = [z](size_t i) { return z[i]; };
     constexpr int RPB = kThreads / G;
     const int nchunks = (nb + RPB - 1) / RPB;
     ChunkWalk w = xcd_chunks(nchunks);
@@ -65,13 +79,22 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
     for (int ch = w.first; ch < w.end; ch += w.step) {
         int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
         MV s0 = 0, s1 = 0, s2 = 0;
-        if (row < nb) sym_row_lane<G, MV>(row, sub, eptr, ent, val, pv, s0, s1, s2);
-        group_sum3<G, MV>(s0, s1, s2);
-        if (sub == 0 && row < nb) {
-            size_t o = 3 * (size_t)row;
-            V a0 = (V)s0, a1 = (V)s1, a2 = (V)s2;
-            Ap[o] = a0; Ap[o + 1] = a1; Ap[o + 2] = a2;
-            contrib += (double)p[o] * (double)a0 + (double)p[o + 1] * (double)a1 + (double)p[o + 2] * (double)a2;
+        if (row < nb) sym_row_lane<G, MV>(row, sub, eptr, ent, val, zv, s0, s1, s2);
+        group_sum3<G, MV>(s0, s1, s2);  // every lane of the group holds the row sums
+        if (sub < 3 && row < nb) {        // lane q forms component q
+            const size_t o = 3 * (size_t)row + sub;
+            const V aw = (V)(sub == 0 ? s0 : (sub == 1 ? s1 : s2));
+            V pq, aq;
+            if (first) {
+                pq = z[o];
+                aq = aw;
+            } else {
+                pq = (V)((double)z[o] + be * (double)p[o]);
+                aq = (V)((double)aw + be * (double)Ap[o]);
+            }
+            p[o] = pq;
+            Ap[o] = aq;
+            contrib += (double)pq * (double)aq;
         }
     }
     write_partial(contrib, part, sh);
@@ -184,20 +207,6 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
     write_partial(v, part_rz, sh);
 }
 
-// beta = rz_new / rz_old ; p = z + beta p
-template <typename V>
-__global__ void __launch_bounds__(kThreads) k_update_p(long long N, V* __restrict__ p, const V* __restrict__ z,
-                                                       const double* __restrict__ part_new, const double* __restrict__ part_old,
-                                                       int np, const int* __restrict__ ctrl) {
-    if (ctrl[C_STOP]) return;
-    __shared__ double sh[kThreads / 64];
-    double rn = sum_partials(part_new, np, sh);
-    double ro = sum_partials(part_old, np, sh);
-    double be = rn / ro;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-        p[i] = (V)((double)z[i] + be * (double)p[i]);
-}
-
 template <typename TD, typename TS>
 __global__ void k_copy_cast(long long N, TD* __restrict__ dst, const TS* __restrict__ src) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
@@ -281,10 +290,10 @@ struct Engine {
         hipStream_t s = c.stream;
         const SymBcoo& S = c.bsr;
         double* rz_last = part_rz[par ^ 1];  // rz_(k-1)
-        double* rz_this = part_rz[par];      // receives rz_k
+        double* rz_this = part_rz[par];      // holds rz_(k-2); receives rz_k
         prof_begin(c, PROF_SPMV);
         hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val,
-                           (const V*)p, Ap, part_pap, ctrl);
+                           (const V*)z, p, Ap, (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
         if (mas) {
@@ -295,9 +304,6 @@ struct Engine {
             SGIPC_LAUNCH_CHECK();
         }
         precond_dot(rz_this, rz_last, rule);
-        hipLaunchKernelGGL((k_update_p<V>), dim3(gN), dim3(kThreads), 0, s, N, p, (const V*)z, (const double*)rz_this,
-                           (const double*)rz_last, gN, (const int*)ctrl);
-        SGIPC_LAUNCH_CHECK();
     }
 
     // r holds the right-hand side, x the initial guess is zero.  Returns the
@@ -324,8 +330,6 @@ struct Engine {
         if (mas) mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl);
         precond_dot(part_rz[0], part_rz[1], -1);
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
-        SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_copy_cast<V, V>), dim3(gN), dim3(kThreads), 0, s, N, p, (const V*)z);
         SGIPC_LAUNCH_CHECK();
         double rz0 = 0.0;
         SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
