@@ -130,17 +130,24 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
 // rule 0 (reference): |rz_prev| <= tol * rz0 on the pre-update rz;
 // rule 1: ||r|| <= tol * ||b|| on the updated residual.  rule < 0: none.
 __device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, int np_rz,
-                                 const double* __restrict__ part_rr, int np_rr, const double* __restrict__ sc,
+                                 const double* __restrict__ part_rr, int np_rr, double* __restrict__ sc,
                                  int* __restrict__ ctrl, double* sh) {
-    bool stop = false;
+    if (rule < 0) return false;
+    bool stop;
+    double q;  // progress: the ratio compared with the tolerance
     if (rule == 0) {
-        double rz = sum_partials(part_rz_prev, np_rz, sh);
+        const double rz = sum_partials(part_rz_prev, np_rz, sh);
         stop = fabs(rz) <= sc[S_TOL] * sc[S_RZ0];
-    } else if (rule == 1) {
-        double rr = sum_partials(part_rr, np_rr, sh);
+        q = fabs(rz) / sc[S_RZ0];
+    } else {
+        const double rr = sum_partials(part_rr, np_rr, sh);
         stop = sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB]);
+        q = sqrt(rr) / sqrt(sc[S_BB]);
     }
-    if (stop && blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_STOP] = ctrl[C_ITER] - 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc[S_PROG] = q;
+        if (stop) ctrl[C_STOP] = ctrl[C_ITER] - 1;
+    }
     return stop;
 }
 
@@ -152,12 +159,11 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
                                                           const T* __restrict__ Z, double* __restrict__ part_rz,
                                                           const double* __restrict__ part_rz_prev, int np_rz,
                                                           const double* __restrict__ part_rr, int np_rr,
-                                                          const double* __restrict__ sc, int* __restrict__ ctrl,
-                                                          int rule) {
+                                                          double* __restrict__ sc, int* __restrict__ ctrl, int rule) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
-    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     double v = 0.0;
+    // z first (its loads do not depend on the stop test), then the test
     for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nb; row += gridDim.x * blockDim.x) {
         size_t o = 3 * (size_t)row;
         V r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
@@ -176,6 +182,7 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
         z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
         v += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
     }
+    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     write_partial(v, part_rz, sh);
 }
 
@@ -185,11 +192,10 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
                                                                  const double* __restrict__ Dinv, double* __restrict__ part_rz,
                                                                  const double* __restrict__ part_rz_prev, int np_rz,
                                                                  const double* __restrict__ part_rr, int np_rr,
-                                                                 const double* __restrict__ sc, int* __restrict__ ctrl,
+                                                                 double* __restrict__ sc, int* __restrict__ ctrl,
                                                                  int rule) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[kThreads / 64];
-    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         V zi;
@@ -204,6 +210,7 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
         z[i] = zi;
         v += (double)r[i] * (double)zi;
     }
+    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
     write_partial(v, part_rz, sh);
 }
 
@@ -264,6 +271,7 @@ struct Engine {
     const double* Dinv;
     int rule;
     int maxit = 0;
+    double tol_now = 0.0;  // tolerance of the current run (progress prediction)
     int gU = 1;  // workgroups (and r.r partials) of the x/r update
 
     static constexpr int G = 8;
@@ -311,6 +319,7 @@ struct Engine {
     double init(double tol, int max_iter) {
         hipStream_t s = c.stream;
         maxit = max_iter;
+        tol_now = tol;
         if (mas) {
             int g1, gx;
             mas_update_layout(c, g1, gx);
@@ -392,11 +401,27 @@ struct Engine {
         }
         int hc[C_NUM] = {0};
         if (use_graph) {
+            // Poll the stop word; after two polls the next one is placed where
+            // the measured convergence rate predicts the stop (at most twice
+            // the nominal interval ahead), so few polls are needed and few
+            // iterations run past the stop.
+            int npairs = pairs_per_check, k_prev = 0;
+            double q_prev = 0.0, q = 0.0;
             for (;;) {
-                for (int q = 0; q < pairs_per_check; ++q) SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
+                for (int j = 0; j < npairs; ++j) SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
                 SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipMemcpyAsync(&q, sc + S_PROG, sizeof(double), hipMemcpyDeviceToHost, s));
                 SGIPC_HIP(hipStreamSynchronize(s));
                 if (hc[C_STOP]) break;
+                const int k = hc[C_ITER] - 1;  // iterations done; q is the ratio after iteration k
+                npairs = pairs_per_check;
+                if (k_prev > 0 && k > k_prev && q > 0.0 && q_prev > q && tol_now > 0.0) {
+                    const double rate = std::log(q / q_prev) / (double)(k - k_prev);  // < 0
+                    const double need = std::log(tol_now / q) / rate;                  // iterations left
+                    npairs = std::max(1, std::min(2 * pairs_per_check, (int)std::ceil(need / 2.0)));
+                }
+                k_prev = k;
+                q_prev = q;
             }
         } else {
             // timed (eager) mode: same batching; the event records of launches
@@ -452,7 +477,7 @@ void precond_apply(Ctx& c, const double* r, double* z) {
     fill<int>(ctrl, 0, C_NUM, s);
     hipLaunchKernelGGL((k_simple_precond_dot<double>), dim3(grid_for(N)), dim3(kThreads), 0, s, N, r, z,
                        c.precond_kind == 1 ? (const double*)c.pcg.Dinv.p : (const double*)nullptr, part, part, 1, part,
-                       1, (const double*)nullptr, ctrl, -1);
+                       1, (double*)nullptr, ctrl, -1);
     SGIPC_LAUNCH_CHECK();
 }
 

@@ -6,8 +6,9 @@
 
 namespace sgipc {
 
-// device scalars: rz0, b.b, tolerance
-enum { S_RZ0 = 0, S_BB, S_TOL, S_NUM };
+// device scalars: rz0, b.b, tolerance, progress of the stop test (the
+// ratio compared against the tolerance at the last test)
+enum { S_RZ0 = 0, S_BB, S_TOL, S_PROG, S_NUM };
 // control words: C_STOP = iteration at which the loop stopped (0 = running),
 // C_ITER = current iteration (from 1), C_MAXIT = iteration budget,
 // C_NOCONV = stopped by the budget
