@@ -55,7 +55,8 @@ def copy_calibration(mib):
 def spmv_alg_bytes(K, nb, fp32):
     # the PCG SpMV kernel (direction update fused): upper BCOO, 9 values
     # (72 B FP64 / 36 B FP32) + 8 B (row, col) per block; per block row z
-    # read once, p and Ap each read and written once
+    # read once, p and Ap each read and written once.  (The FP32 copy is
+    # stored padded to 48 B per block; the padding is not counted.)
     vb = 36.0 if fp32 else 72.0
     xb = 12.0 if fp32 else 24.0
     return (vb + 8.0) * K + 5.0 * xb * nb
@@ -185,8 +186,10 @@ def main():
                          "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, 1 thread, "
                          "%.1f s" % (csteps, cit, csecs)}
 
-    kname = "k_spmv_pap<8, float, float> (PCG SpMV + direction update, FP32 values)" if fp32 else \
-        "k_spmv_pap<8, double, double> (PCG SpMV + direction update, FP64)"
+    g = os.environ.get("SGIPC_SPMV_G", "8")
+    g = g if g in ("4", "8", "16") else "8"   # lanes per block row, as the library reads it
+    kname = ("k_spmv_pap<%s, float, float> (PCG SpMV + direction update, FP32 values)" % g) if fp32 else \
+        ("k_spmv_pap<%s, double, double> (PCG SpMV + direction update, FP64)" % g)
     out = {
         "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 true rel. residual)",
         "value": round(r["value"], 4),

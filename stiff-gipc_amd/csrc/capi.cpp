@@ -1,4 +1,5 @@
 // extern "C" boundary of libsgipc (declarations: include/sgipc.h).
+#include <cstdlib>
 #include <vector>
 #include "../../include/sgipc.h"
 #include "context.hpp"
@@ -77,6 +78,10 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         // callers (and torch's default stream) may use on the same buffers
         SGIPC_HIP(hipStreamCreate(&c->own_stream));
         c->stream = c->own_stream;
+        if (const char* g = std::getenv("SGIPC_SPMV_G")) {
+            int v = std::atoi(g);
+            if (v == 4 || v == 8 || v == 16) c->spmv_group = v;
+        }
     });
     if (rc != SGIPC_OK) { delete c; return rc; }
     *out = c;

@@ -21,7 +21,7 @@ struct SymBcoo {
     DBuf<int2> lref;    // nlow   (build scratch: lower references)
     DBuf<int> eptr;     // nb + 1
     DBuf<int2> ent;     // nlow + K
-    DBuf<float> valf;   // 9 * K FP32 copy of the BCOO values (mixed precision)
+    DBuf<float> valf;   // 12 * K FP32 copy of the BCOO values, padded columns (mixed precision)
     bool valf_ok = false;
 };
 
@@ -136,6 +136,7 @@ struct Ctx {
     DBuf<int> dev_int;    // small device ints (totals)
     SymBcoo bsr;
     bool have_bsr = false;
+    int spmv_group = 8;  // lanes per block row in the PCG SpMV (env SGIPC_SPMV_G: 4, 8, 16)
 
     // preconditioner
     int precond_kind = 0;

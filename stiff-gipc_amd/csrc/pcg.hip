@@ -274,7 +274,7 @@ struct Engine {
     double tol_now = 0.0;  // tolerance of the current run (progress prediction)
     int gU = 1;  // workgroups (and r.r partials) of the x/r update
 
-    static constexpr int G = 8;
+    int G = 8;  // lanes per block row in the SpMV (4, 8 or 16)
     static constexpr bool kFp32Mas = sizeof(T) == 4;
 
     // z = M^-1 r fused with the r.z partials (gN workgroups) and the stop test
@@ -300,8 +300,13 @@ struct Engine {
         double* rz_last = part_rz[par ^ 1];  // rz_(k-1)
         double* rz_this = part_rz[par];      // holds rz_(k-2); receives rz_k
         prof_begin(c, PROF_SPMV);
-        hipLaunchKernelGGL((k_spmv_pap<G, MV, V>), dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val,
-                           (const V*)z, p, Ap, (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val, (const V*)z, p, Ap,
+                               (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
+        };
+        if (G == 4) launch(k_spmv_pap<4, MV, V>);
+        else if (G == 16) launch(k_spmv_pap<16, MV, V>);
+        else launch(k_spmv_pap<8, MV, V>);
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
         if (mas) {
@@ -350,7 +355,7 @@ struct Engine {
         size_t h = 1469598103934665603ull;
         auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
         const SymBcoo& S = c.bsr;
-        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)gU); mix((size_t)S.nb); mix((size_t)S.K);
+        mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)G); mix((size_t)gU); mix((size_t)S.nb); mix((size_t)S.K);
         mix((size_t)S.nlow); mix((size_t)S.eptr.p); mix((size_t)S.ent.p); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
@@ -508,8 +513,9 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
     int* ctrl = w.ctrl.reserve(C_NUM + 8);
     const int max_iter = o.max_iter > 0 ? o.max_iter : (int)(o.max_iter_ratio * (double)N);
     const int gN = grid_for(N);
-    // SpMV: chunks of kThreads / 8 block rows, a multiple of 8 workgroups (one range per XCD)
-    const int nchunks = (int)div_up((long long)c.nb, kThreads / 8);
+    // SpMV: chunks of kThreads / G block rows, a multiple of 8 workgroups (one range per XCD)
+    const int G = c.spmv_group;
+    const int nchunks = (int)div_up((long long)c.nb, kThreads / G);
     const int gS = std::min(kPart, std::max(8, (int)div_up(nchunks, 8) * 8));
     st.refinements = 0;
     if (o.precision == 0) {
@@ -524,6 +530,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
                                         mas ? (T*)(fp32mas ? (void*)c.mas.R32.p : (void*)c.mas.R64.p) : nullptr,
                                         mas ? (T*)(fp32mas ? (void*)c.mas.Z32.p : (void*)c.mas.Z64.p) : nullptr,
                                         c.precond_kind == 1 ? w.Dinv.p : nullptr, o.stop_rule};
+            E.G = G;
             hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, E.r, b);
             SGIPC_LAUNCH_CHECK();
             double rz0 = E.init(o.tol, max_iter);
@@ -554,6 +561,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
                                       parts + 4 * kPart, sc, ctrl, bsr_values_f32(c), mas,
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
                                       c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
+        E.G = G;
         fill<double>(x, 0.0, N, s);
         hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();

@@ -83,9 +83,15 @@ __global__ void k_entries(int nb, const int* __restrict__ lptr, const int2* __re
     for (int k = ustart[j]; k < ustart[j + 1]; ++k) ent[base + nl + (k - ustart[j])] = make_int2(k, ucol[k]);
 }
 
-__global__ void k_to_f32(long long n, const double* __restrict__ s, float* __restrict__ d) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        d[i] = (float)s[i];
+// FP32 copy of the BCOO values, 12 floats per block: column q in lanes
+// 4q .. 4q+2, lane 4q+3 zero
+__global__ void k_to_f32(long long K, const double* __restrict__ s, float* __restrict__ d) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K * kF32BlockStride;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long k = i / kF32BlockStride;
+        const int e = (int)(i % kF32BlockStride), q = e >> 2, p = e & 3;
+        d[i] = (p < 3) ? (float)s[9 * k + p + 3 * q] : 0.0f;
+    }
 }
 
 }  // namespace
@@ -134,11 +140,11 @@ void bsr_build(Ctx& c) {
 const float* bsr_values_f32(Ctx& c) {
     if (!c.have_bsr) bsr_build(c);
     SymBcoo& S = c.bsr;
-    float* vf = S.valf.reserve(9 * (size_t)S.K + 1);
+    float* vf = S.valf.reserve((size_t)kF32BlockStride * S.K + 4);
     if (!S.valf_ok && S.K > 0) {
-        long long n = 9LL * S.K;
+        long long n = (long long)kF32BlockStride * S.K;
         int g = (int)std::min<long long>(div_up(n, 256), 8192);
-        hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, c.stream, n, (const double*)c.t_vals, vf);
+        hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, c.stream, (long long)S.K, (const double*)c.t_vals, vf);
         SGIPC_LAUNCH_CHECK();
     }
     S.valf_ok = true;

@@ -34,13 +34,39 @@ __device__ inline ChunkWalk xcd_chunks(int nchunks) {
     return w;
 }
 
+// Block k of the value array, column-major a[p + 3q] = A(p, q), transposed
+// in registers when tr.  FP64: the BCOO itself (72 B per block, 8-byte
+// aligned), read as four 16-byte and one 8-byte load.  FP32: the padded copy
+// (48 B per block, columns in float4 lanes 0-2), three aligned 16-byte loads.
+typedef double double2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+constexpr int kF32BlockStride = 12;  // floats per block in the FP32 copy
+
 template <typename TV>
-__device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, TV a[9]) {
-    const TV* A = vals + 9 * (size_t)k;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) a[i] = A[i];
-    if (tr) {  // transpose in registers: a[p + 3q] <-> a[q + 3p]
-        TV t;
+__device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, TV a[9]);
+
+template <>
+__device__ inline void load_block<double>(const double* __restrict__ vals, int k, bool tr, double a[9]) {
+    const double* A = vals + 9 * (size_t)k;
+    const double2_a8* A2 = reinterpret_cast<const double2_a8*>(A);
+    double2_a8 u0 = A2[0], u1 = A2[1], u2 = A2[2], u3 = A2[3];
+    a[0] = u0.x; a[1] = u0.y; a[2] = u1.x; a[3] = u1.y; a[4] = u2.x; a[5] = u2.y; a[6] = u3.x; a[7] = u3.y;
+    a[8] = A[8];
+    if (tr) {  // a[p + 3q] <-> a[q + 3p]
+        double t;
+        t = a[1]; a[1] = a[3]; a[3] = t;
+        t = a[2]; a[2] = a[6]; a[6] = t;
+        t = a[5]; a[5] = a[7]; a[7] = t;
+    }
+}
+
+template <>
+__device__ inline void load_block<float>(const float* __restrict__ vals, int k, bool tr, float a[9]) {
+    const float4* A4 = reinterpret_cast<const float4*>(vals + kF32BlockStride * (size_t)k);
+    float4 c0 = A4[0], c1 = A4[1], c2 = A4[2];
+    a[0] = c0.x; a[1] = c0.y; a[2] = c0.z; a[3] = c1.x; a[4] = c1.y; a[5] = c1.z; a[6] = c2.x; a[7] = c2.y;
+    a[8] = c2.z;
+    if (tr) {
+        float t;
         t = a[1]; a[1] = a[3]; a[3] = t;
         t = a[2]; a[2] = a[6]; a[6] = t;
         t = a[5]; a[5] = a[7]; a[7] = t;
