@@ -33,6 +33,19 @@ def cube4():
     return scene.cube_scene(4, seed=7)
 
 
+@pytest.fixture(scope="session")
+def cube10_b32():
+    """cube10 partitioned into 32-vertex banks (2x4x4 vertex blocks)"""
+    from sgipc import scene
+    return scene.cube_scene(10, seed=1, bank=32)
+
+
+@pytest.fixture(scope="session")
+def cube4_b32():
+    from sgipc import scene
+    return scene.cube_scene(4, seed=7, bank=32)
+
+
 def oracle_system(O, sc):
     """Assemble + convert one Newton-step system with the oracle."""
     shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)

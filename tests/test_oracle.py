@@ -211,30 +211,33 @@ def _mesh_edges(ns, nn, nl):
     return e
 
 
-def test_mas_tables_vs_bruteforce(oracle, cube10):
-    sc = cube10
+@pytest.mark.parametrize("bank", [16, 32])
+def test_mas_tables_vs_bruteforce(oracle, request, bank):
+    sc = request.getfixturevalue("cube10" if bank == 16 else "cube10_b32")
     ref = oracle_system(oracle, sc)
     ns, nn, nl = oracle.vertex_neighbors(sc.n, sc.tets)
-    pm, rm = oracle.bank_map(sc.part, 16)
-    mas = oracle.Mas(sc.n, pm, rm, ns, nn, nl, 16)
+    pm, rm = oracle.bank_map(sc.part, bank)
+    mas = oracle.Mas(sc.n, pm, rm, ns, nn, nl, bank)
     kr, kc, kv = ref["bcoo"]
     mas.build(kr, kc, kv)
     t = mas.tables()
     L = mas.levelnum
-    counts, table = brute_force_tables(sc.n, pm, rm, _mesh_edges(ns, nn, nl), 16, L)
+    assert L >= 3
+    counts, table = brute_force_tables(sc.n, pm, rm, _mesh_edges(ns, nn, nl), bank, L)
     for l in range(1, L):
         assert t["level_size"][l, 0] == counts[l - 1]
         assert np.array_equal(t["coarse_space"][l - 1], table[l - 1])
 
 
-def test_mas_apply_dense(oracle, cube4):
-    sc = cube4
+@pytest.mark.parametrize("bank", [16, 32])
+def test_mas_apply_dense(oracle, request, bank):
+    sc = request.getfixturevalue("cube4" if bank == 16 else "cube4_b32")
     ref = oracle_system(oracle, sc)
     kr, kc, kv = ref["bcoo"]
     A = dense_from_bcoo(kr, kc, kv, sc.n)
     ns, nn, nl = oracle.vertex_neighbors(sc.n, sc.tets)
-    pm, rm = oracle.bank_map(sc.part, 16)
-    mas = oracle.Mas(sc.n, pm, rm, ns, nn, nl, 16)
+    pm, rm = oracle.bank_map(sc.part, bank)
+    mas = oracle.Mas(sc.n, pm, rm, ns, nn, nl, bank)
     tot = mas.build(kr, kc, kv)
     ct = mas.tables()["coarse_table"]
     L = mas.levelnum
@@ -247,8 +250,8 @@ def test_mas_apply_dense(oracle, cube4):
             R[3 * nodes[v]:3 * nodes[v] + 3, 3 * v:3 * v + 3] = np.eye(3)
         Al = R @ A @ R.T
         Dinv = np.zeros_like(Al)
-        for b in np.unique(nodes // 16):
-            s = slice(48 * b, 48 * b + 48)
+        for b in np.unique(nodes // bank):
+            s = slice(3 * bank * b, 3 * bank * (b + 1))
             blk = Al[s, s].copy()
             dg = np.diag(blk).copy(); dg[dg == 0] = 1; np.fill_diagonal(blk, dg)
             Dinv[s, s] = np.linalg.inv(blk)
