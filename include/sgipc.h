@@ -120,6 +120,48 @@ int sgipc_mass_triplets(sgipc_ctx* ctx, int n, const double* d_mass, int index_o
 int sgipc_fem_rhs(sgipc_ctx* ctx, int n, const double* d_fb, const double* d_shape_grad,
                   const int* d_btype, double* d_b);
 
+/* ---- ground (half-space) barrier -------------------------------------------
+ * Plane n.x = offset with a unit normal (GIPC::_groundNormal/_groundOffset,
+ * StiffGIPC/GIPC.cu:8539-8549); dhat squared, as GIPC::dHat.  Vertex lists
+ * are uint32 vertex ids (the reference's _surfVerts /
+ * _environment_collisionPair). */
+/* Close vertices: ids of d_cand whose squared distance is <= dhat, in
+ * candidate order; _GroundCollisionDetect (StiffGIPC/GIPC.cu:6433-6450).
+ * d_out holds n_cand entries; *h_n_close receives the count (synchronises). */
+int sgipc_ground_detect(sgipc_ctx* ctx, int n_cand, const uint32_t* d_cand, const double* d_x,
+                        const double normal[3], double offset, double dhat, uint32_t* d_out, int* h_n_close);
+/* Barrier gradient (atomically added into d_fb) and one 3x3 Hessian triplet
+ * per close vertex at triplet_offset (blocks of pinned vertices zeroed);
+ * _computeGroundGradientAndHessian (StiffGIPC/GIPC.cu:6463-6514). */
+int sgipc_assemble_ground(sgipc_ctx* ctx, int n_close, const uint32_t* d_close, const double* d_x,
+                          const double normal[3], double offset, double dhat, double kappa, const int* d_btype,
+                          int index_offset, long long triplet_offset, double* d_fb);
+/* kappa * sum of the barrier over the listed vertices closer than dhat
+ * (FP64, fixed order); _computeGroundEnergy_Reduction
+ * (StiffGIPC/GIPC.cu:6837-6900, :10555). */
+int sgipc_ground_energy(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, const double* d_x, const double normal[3],
+                        double offset, double dhat, double kappa, double* h_energy);
+/* Largest step alpha <= 1 along -d_dir keeping every listed vertex at least
+ * (1 - slack) of its distance away from the plane;
+ * GIPC::ground_largestFeasibleStepSize (StiffGIPC/GIPC.cu:8986-9030). */
+int sgipc_ground_step_bound(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, const double* d_x,
+                            const double* d_dir, const double normal[3], double offset, double slack,
+                            double* h_alpha);
+
+/* ---- differentiation -------------------------------------------------------
+ * No reference counterpart (the reference has no adjoint); these serve an
+ * implicit-function adjoint of the converged Newton step. */
+/* on != 0 (default): PSD-projected SNK1 Hessian blocks, as the reference;
+ * on == 0: the exact second derivative, for adjoint solves at a converged
+ * iterate. */
+int sgipc_set_fem_projection(sgipc_ctx* ctx, int on);
+/* Per tet t: d_dmu[t] = d(adj . g)/d mu_t and d_dlam[t] = d(adj . g)/d lam_t
+ * for the dt^2-scaled SNK1 gradient g of sgipc_assemble_fem (adjoint of
+ * pinned vertices taken as zero). */
+int sgipc_fem_param_adjoint(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d_tets,
+                            const double* d_DmInv, const double* d_vol, double dt, const int* d_btype,
+                            const double* d_adj, double* d_dmu, double* d_dlam);
+
 /* Narrow phase: classify candidate point-triangle (p, t0, t1, t2) and
  * edge-edge (a0, a1, b0, b1) vertex quadruples into the reference's int4 pair
  * encoding, keeping pairs with squared distance < dhat; PT candidates first,

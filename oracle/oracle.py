@@ -75,6 +75,17 @@ def lib():
         L.or_barrier_energy.restype = C.c_double
         L.or_classify.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double, i32p, i32p, i32p]
         L.or_classify.restype = C.c_int
+        L.or_set_fem_projection.argtypes = [C.c_int]
+        L.or_ground_detect.argtypes = [C.c_int, u32p, f64p, f64p, C.c_double, C.c_double, u32p]
+        L.or_ground_detect.restype = C.c_int
+        L.or_ground_assemble.argtypes = [C.c_int, u32p, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                         C.c_int, f64p, f64p, i32p, i32p]
+        L.or_ground_energy.argtypes = [C.c_int, u32p, f64p, f64p, C.c_double, C.c_double, C.c_double]
+        L.or_ground_energy.restype = C.c_double
+        L.or_ground_step_bound.argtypes = [C.c_int, u32p, f64p, f64p, f64p, C.c_double, C.c_double]
+        L.or_ground_step_bound.restype = C.c_double
+        L.or_fem_param_adjoint.argtypes = [C.c_int, f64p, u32p, f64p, f64p, C.c_double, C.c_void_p, f64p, f64p,
+                                           f64p]
         _lib = L
     return _lib
 
@@ -288,3 +299,59 @@ def classify(pt, ee, x, xrest, dhat):
                               _c(x, np.float64).reshape(-1), _c(xrest, np.float64).reshape(-1), dhat,
                               pairs.reshape(-1), mi, counts)
     return pairs[:npair].copy(), mi[:npair].copy(), counts
+
+
+# ---- ground plane and parameter adjoint ---------------------------------------
+class fem_projection:
+    """context manager: SNK1 Hessian projection on/off in the oracle"""
+
+    def __init__(self, on):
+        self.on = on
+
+    def __enter__(self):
+        lib().or_set_fem_projection(int(self.on))
+
+    def __exit__(self, *a):
+        lib().or_set_fem_projection(1)
+
+
+def ground_detect(cand, x, normal, offset, dhat):
+    cand = _c(cand, np.uint32)
+    out = np.zeros(max(len(cand), 1), np.uint32)
+    c = lib().or_ground_detect(len(cand), cand, _c(x, np.float64).reshape(-1), _c(normal, np.float64), offset,
+                               dhat, out)
+    return out[:c]
+
+
+def ground_assemble(gp, x, normal, offset, dhat, kappa, btype=None, index_offset=0, fb=None):
+    gp = _c(gp, np.uint32)
+    x = _c(x, np.float64).reshape(-1)
+    fb = np.zeros(x.shape[0]) if fb is None else _c(fb, np.float64).reshape(-1).copy()
+    m = len(gp)
+    vals = np.zeros(9 * max(m, 1)); rows = np.zeros(max(m, 1), np.int32); cols = np.zeros(max(m, 1), np.int32)
+    bt = None if btype is None else _c(btype, np.int32)
+    lib().or_ground_assemble(m, gp, x, _c(normal, np.float64), offset, dhat, kappa,
+                             None if bt is None else bt.ctypes.data, index_offset, fb, vals, rows, cols)
+    return fb, vals[:9 * m].reshape(m, 9), rows[:m], cols[:m]
+
+
+def ground_energy(ids, x, normal, offset, dhat, kappa):
+    ids = _c(ids, np.uint32)
+    return lib().or_ground_energy(len(ids), ids, _c(x, np.float64).reshape(-1), _c(normal, np.float64), offset,
+                                  dhat, kappa)
+
+
+def ground_step_bound(ids, x, d, normal, offset, slack):
+    ids = _c(ids, np.uint32)
+    return lib().or_ground_step_bound(len(ids), ids, _c(x, np.float64).reshape(-1), _c(d, np.float64).reshape(-1),
+                                      _c(normal, np.float64), offset, slack)
+
+
+def fem_param_adjoint(x, tets, DmInv, vol, dt, adj, btype=None):
+    T = tets.shape[0]
+    dmu = np.zeros(max(T, 1)); dlam = np.zeros(max(T, 1))
+    bt = None if btype is None else _c(btype, np.int32)
+    lib().or_fem_param_adjoint(T, _c(x, np.float64).reshape(-1), _c(tets, np.uint32).reshape(-1),
+                               _c(DmInv, np.float64).reshape(-1), _c(vol, np.float64),
+                               dt, None if bt is None else bt.ctypes.data, _c(adj, np.float64).reshape(-1), dmu, dlam)
+    return dmu[:T], dlam[:T]

@@ -16,6 +16,10 @@
  *   or_mas_*         tables: brute-force connected components per bank;
  *                    apply: explicit dense R_l / D_l^{-1} construction
  *   or_pcg           scipy direct solve of the same system
+ *   or_ground_*      energy closed form; gradient vs central FD of the
+ *                    energy; Hessian vs central FD of the gradient
+ *   or_fem_param_adjoint  central FD of the assembled gradient in mu, lam
+ *   exact SNK1 mode  (projection off) Hessian vs central FD of the gradient
  */
 #include "sgipc_oracle.h"
 #include <float.h>
@@ -281,6 +285,11 @@ static void tet_Bm(const double DmInv[9], double Bm[12]) {
     }
 }
 
+/* 1: clamp negative modes (the reference's PSD Newton Hessian); 0: exact
+ * second derivative (all nine modes), as the library's adjoint mode */
+int or_fem_project = 1;
+void or_set_fem_projection(int on) { or_fem_project = on ? 1 : 0; }
+
 /* Analytic eigensystem of the SNK1 dPsi/dF^2 with negative modes clamped
  * (S/femEnergy.cu:1593-1717).  vec index = i + 3 j (column-major). */
 void or_snk1_hq(const double F[9], double mu, double lam, double H81[81]) {
@@ -320,7 +329,7 @@ void or_snk1_hq(const double F[9], double mu, double lam, double H81[81]) {
                 Q[6 + k][r + 3 * c] = s;
             }
     }
-    for (int k = 0; k < 9; ++k) if (ev[k] < 0.0) ev[k] = 0.0;
+    if (or_fem_project) for (int k = 0; k < 9; ++k) if (ev[k] < 0.0) ev[k] = 0.0;
     for (int a = 0; a < 9; ++a)
         for (int b = 0; b < 9; ++b) {
             double s = 0;

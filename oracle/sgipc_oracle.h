@@ -48,6 +48,19 @@ void or_fem_precompute(int n, int T, const double* xrest, const uint32_t* tets,
 /* FEM triplets + shape gradient (S/femEnergy.cu:1719-1857 + boundary mask
  * S/GIPC.cu:10295-10313).  grad (3n) is accumulated (+=) in tet order.
  * Writes 10 triplets per tet at vals+90*t, rows/cols+10*t. */
+extern int or_fem_project;
+void or_set_fem_projection(int on);
+/* ground plane n.x = o (sgipc_oracle_ground.c) */
+int or_ground_detect(int m, const uint32_t* cand, const double* x, const double nrm[3], double o, double dhat,
+                     uint32_t* out);
+void or_ground_assemble(int np, const uint32_t* gp, const double* x, const double nrm[3], double o, double dhat,
+                        double kappa, const int* btype, int off, double* fb, double* vals, int* rows, int* cols);
+double or_ground_energy(int m, const uint32_t* ids, const double* x, const double nrm[3], double o, double dhat,
+                        double kappa);
+double or_ground_step_bound(int m, const uint32_t* ids, const double* x, const double* dir, const double nrm[3],
+                            double o, double slack);
+void or_fem_param_adjoint(int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                          double dt, const int* btype, const double* adj, double* dmu, double* dlam);
 void or_fem_assemble(int n, int T, const double* x, const uint32_t* tets,
                      const double* DmInv, const double* vol, const double* mu,
                      const double* lam, double dt, const int* btype,

@@ -206,6 +206,68 @@ int sgipc_fem_rhs(sgipc_ctx* c, int n, const double* fb, const double* sh, const
     });
 }
 
+int sgipc_ground_detect(sgipc_ctx* c, int m, const uint32_t* cand, const double* x, const double normal[3],
+                        double offset, double dhat, uint32_t* out, int* n_close) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (m < 0 || !normal || !n_close || (m > 0 && (!cand || !x || !out)))
+            throw sgipc::arg_error("ground_detect: invalid argument");
+        set_device(c);
+        *n_close = sgipc::ground_detect(*c, m, cand, x, normal, offset, dhat, out);
+    });
+}
+
+int sgipc_assemble_ground(sgipc_ctx* c, int np, const uint32_t* gp, const double* x, const double normal[3],
+                          double offset, double dhat, double kappa, const int* bt, int index_offset,
+                          long long triplet_offset, double* fb) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (np < 0 || !normal || (np > 0 && (!gp || !x || !fb)) || !(dhat > 0.0))
+            throw sgipc::arg_error("assemble_ground: invalid argument");
+        set_device(c);
+        sgipc::ground_assemble(*c, np, gp, x, normal, offset, dhat, kappa, bt, index_offset, triplet_offset, fb);
+    });
+}
+
+int sgipc_ground_energy(sgipc_ctx* c, int m, const uint32_t* ids, const double* x, const double normal[3],
+                        double offset, double dhat, double kappa, double* e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (m < 0 || !normal || !e || (m > 0 && (!ids || !x)) || !(dhat > 0.0))
+            throw sgipc::arg_error("ground_energy: invalid argument");
+        set_device(c);
+        *e = sgipc::ground_energy(*c, m, ids, x, normal, offset, dhat, kappa);
+    });
+}
+
+int sgipc_ground_step_bound(sgipc_ctx* c, int m, const uint32_t* ids, const double* x, const double* dir,
+                            const double normal[3], double offset, double slack, double* alpha) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (m < 0 || !normal || !alpha || (m > 0 && (!ids || !x || !dir)))
+            throw sgipc::arg_error("ground_step_bound: invalid argument");
+        set_device(c);
+        *alpha = sgipc::ground_step_bound(*c, m, ids, x, dir, normal, offset, slack);
+    });
+}
+
+int sgipc_set_fem_projection(sgipc_ctx* c, int on) {
+    CHECK_CTX(c);
+    return guard(c, [&] { c->fem_project = on ? 1 : 0; });
+}
+
+int sgipc_fem_param_adjoint(sgipc_ctx* c, int T, const double* x, const uint32_t* tets, const double* DmInv,
+                            const double* vol, double dt, const int* bt, const double* adj, double* dmu,
+                            double* dlam) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (T < 0 || (T > 0 && (!x || !tets || !DmInv || !vol || !adj || !dmu || !dlam)))
+            throw sgipc::arg_error("fem_param_adjoint: invalid argument");
+        set_device(c);
+        sgipc::fem_param_adjoint(*c, T, x, tets, DmInv, vol, dt, bt, adj, dmu, dlam);
+    });
+}
+
 int sgipc_classify_contacts(sgipc_ctx* c, int npt, const int* pt, int nee, const int* ee, const double* x,
                             const double* xr, double dhat, int* pairs, int* mi, int capacity, int* n_pairs,
                             int* counts) {

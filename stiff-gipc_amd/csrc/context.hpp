@@ -119,6 +119,7 @@ struct Ctx {
     DBuf<int> inc_start;  // n+1
     DBuf<int> inc_list;   // 4T entries: 4*t + corner, ascending t per vertex
     DBuf<double> tet_grad; // 12*T
+    int fem_project = 1;   // 1: PSD-projected SNK1 Hessian (Newton); 0: exact (adjoint solves)
 
     // converted system
     int K = 0;            // unique upper blocks
@@ -167,6 +168,17 @@ void fem_assemble(Ctx& c, int T, const double* x, const uint32_t* tets, const do
 void fem_mass_triplets(Ctx& c, int n, const double* mass, int index_offset,
                        long long triplet_offset);
 void fem_rhs(Ctx& c, int n, const double* fb, const double* shape, const int* btype, double* b);
+void fem_param_adjoint(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                       double dt, const int* btype, const double* adj, double* dmu, double* dlam);
+int ground_detect(Ctx& c, int m, const unsigned* cand, const double* x, const double* normal, double offset,
+                  double dhat, unsigned* out);
+void ground_assemble(Ctx& c, int np, const unsigned* gp, const double* x, const double* normal, double offset,
+                     double dhat, double kappa, const int* btype, int index_offset, long long triplet_offset,
+                     double* fb);
+double ground_energy(Ctx& c, int m, const unsigned* ids, const double* x, const double* normal, double offset,
+                     double dhat, double kappa);
+double ground_step_bound(Ctx& c, int m, const unsigned* ids, const double* x, const double* dir,
+                         const double* normal, double offset, double slack);
 
 int bcoo_convert(Ctx& c, long long nt, int nblockrows);
 void bsr_build(Ctx& c);

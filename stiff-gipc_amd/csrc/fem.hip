@@ -236,8 +236,8 @@ __device__ inline double det3(const double* F) {
 // packed row-wise into 45 entries.
 __device__ inline int up9(int a, int b) { return a * 9 - a * (a - 1) / 2 + (b - a); }
 
-__device__ inline void accum_mode(double* Hq, double lam, const double* q) {
-    if (lam <= 0.0) return;
+__device__ inline void accum_mode(double* Hq, double lam, const double* q, bool project) {
+    if (project && lam <= 0.0) return;
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
         double la = lam * q[a];
@@ -246,7 +246,9 @@ __device__ inline void accum_mode(double* Hq, double lam, const double* q) {
     }
 }
 
-__device__ void snk1_hq(const double* F, double mu, double lam, double* Hq) {
+// project: clamp negative modes (the PSD Newton Hessian); otherwise the
+// exact second derivative (all nine modes), used by adjoint solves
+__device__ void snk1_hq(const double* F, double mu, double lam, double* Hq, bool project) {
     double U[9], S[3], V[9];
     svd3(F, U, S, V);
     double J = det3(F);
@@ -268,8 +270,8 @@ __device__ void snk1_hq(const double* F, double mu, double lam, double* Hq) {
                 tw[r + 3 * c] = b - a;
                 fl[r + 3 * c] = a + b;
             }
-        accum_mode(Hq, mu + S[t] * sc, tw);
-        accum_mode(Hq, mu - S[t] * sc, fl);
+        accum_mode(Hq, mu + S[t] * sc, tw, project);
+        accum_mode(Hq, mu - S[t] * sc, fl, project);
     }
     // scaling modes: 3x3 block in singular-value space
     double s00 = S[0] * S[0], s11 = S[1] * S[1], s22 = S[2] * S[2];
@@ -281,7 +283,7 @@ __device__ void snk1_hq(const double* F, double mu, double lam, double* Hq) {
     syev3(Am, w, E);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        if (w[k] <= 0.0) continue;
+        if (project && w[k] <= 0.0) continue;
         double q[9];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
@@ -289,7 +291,7 @@ __device__ void snk1_hq(const double* F, double mu, double lam, double* Hq) {
             for (int c = 0; c < 3; ++c)
                 q[r + 3 * c] = U[3 * r + 0] * E[0 + k] * V[3 * c + 0] + U[3 * r + 1] * E[3 + k] * V[3 * c + 1] +
                                U[3 * r + 2] * E[6 + k] * V[3 * c + 2];
-        accum_mode(Hq, w[k], q);
+        accum_mode(Hq, w[k], q, project);
     }
 }
 
@@ -303,7 +305,8 @@ __global__ void __launch_bounds__(128) k_fem_snk1(int T, const double* __restric
                                                   const double* __restrict__ lam, double dt2,
                                                   const int* __restrict__ btype, int index_offset,
                                                   double* __restrict__ tvals, int* __restrict__ trows,
-                                                  int* __restrict__ tcols, double* __restrict__ tet_grad) {
+                                                  int* __restrict__ tcols, double* __restrict__ tet_grad,
+                                                  int project) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     uint4 tv = tets[t];
@@ -354,7 +357,7 @@ __global__ void __launch_bounds__(128) k_fem_snk1(int T, const double* __restric
             }
     }
     double Hq[45];
-    snk1_hq(F, m, l, Hq);
+    snk1_hq(F, m, l, Hq, project != 0);
     bool pin[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) pin[a] = btype ? (btype[vid[a]] != 0) : false;
@@ -387,6 +390,72 @@ __global__ void __launch_bounds__(128) k_fem_snk1(int T, const double* __restric
             trows[base + kk] = tr ? c : r;
             tcols[base + kk] = tr ? r : c;
         }
+}
+
+// Parameter Jacobian of the SNK1 gradient contracted with an adjoint vector:
+// the tet gradient is g_a = dt^2 vol Bm(a,:) P^T with
+// P = mu F + (lam (J - 1) - mu) cof(F), linear in (mu, lam), so
+//   d(adj . g)/d mu  = dt^2 vol <G, F - cof(F)>
+//   d(adj . g)/d lam = dt^2 vol (J - 1) <G, cof(F)>
+// with G(k, j) = sum_a adj(v_a, k) Bm(a, j); pinned vertices contribute no
+// gradient rows (their right-hand side is zeroed), so their adjoint is
+// taken as zero.  No reference counterpart (the reference has no adjoint);
+// the gradient it differentiates is _calculate_fem_gradient_hessian
+// (StiffGIPC/femEnergy.cu:1719-1857).
+__global__ void __launch_bounds__(128) k_fem_param_adjoint(int T, const double* __restrict__ x,
+                                                           const uint4* __restrict__ tets,
+                                                           const double* __restrict__ DmInv,
+                                                           const double* __restrict__ vol, double dt2,
+                                                           const int* __restrict__ btype,
+                                                           const double* __restrict__ adj,
+                                                           double* __restrict__ dmu, double* __restrict__ dlam) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    uint4 tv = tets[t];
+    const unsigned vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    double X[12], A[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        bool pin = btype ? (btype[vid[a]] != 0) : false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            X[3 * a + k] = x[3 * (size_t)vid[a] + k];
+            A[3 * a + k] = pin ? 0.0 : adj[3 * (size_t)vid[a] + k];
+        }
+    }
+    double Dm[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Dm[i] = DmInv[9 * (size_t)t + i];
+    double Bm[12];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        Bm[j] = -(Dm[j] + Dm[3 + j] + Dm[6 + j]);
+        Bm[3 + j] = Dm[j];
+        Bm[6 + j] = Dm[3 + j];
+        Bm[9 + j] = Dm[6 + j];
+    }
+    double F[9], G[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            F[3 * i + j] = X[i] * Bm[j] + X[3 + i] * Bm[3 + j] + X[6 + i] * Bm[6 + j] + X[9 + i] * Bm[9 + j];
+            G[3 * i + j] = A[i] * Bm[j] + A[3 + i] * Bm[3 + j] + A[6 + i] * Bm[6 + j] + A[9 + i] * Bm[9 + j];
+        }
+    double cof[9];
+    cof[0] = F[4] * F[8] - F[5] * F[7]; cof[1] = F[5] * F[6] - F[3] * F[8]; cof[2] = F[3] * F[7] - F[4] * F[6];
+    cof[3] = F[7] * F[2] - F[8] * F[1]; cof[4] = F[8] * F[0] - F[6] * F[2]; cof[5] = F[6] * F[1] - F[7] * F[0];
+    cof[6] = F[1] * F[5] - F[4] * F[2]; cof[7] = F[2] * F[3] - F[0] * F[5]; cof[8] = F[0] * F[4] - F[1] * F[3];
+    double J = det3(F);
+    double gf = 0.0, gc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        gf += G[i] * F[i];
+        gc += G[i] * cof[i];
+    }
+    double s = dt2 * vol[t];
+    dmu[t] = s * (gf - gc);
+    dlam[t] = s * (J - 1.0) * gc;
 }
 
 // shape[v] += sum over incident tets (ascending tet id) of the tet gradient
@@ -472,11 +541,19 @@ void fem_assemble(Ctx& c, int T, const double* x, const uint32_t* tets, const do
     prof_begin(c, PROF_FEM);
     hipLaunchKernelGGL(k_fem_snk1, dim3(div_up(T, 128)), dim3(128), 0, c.stream, T, x, (const uint4*)tets, DmInv,
                        vol, mu, lam, dt * dt, btype, index_offset, c.t_vals + 9 * triplet_offset,
-                       c.t_rows + triplet_offset, c.t_cols + triplet_offset, c.tet_grad.p);
+                       c.t_rows + triplet_offset, c.t_cols + triplet_offset, c.tet_grad.p, c.fem_project);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_FEM);
     hipLaunchKernelGGL(k_gather_tet_grad, dim3(div_up(c.fem_n, 256)), dim3(256), 0, c.stream, c.fem_n,
                        c.inc_start.p, c.inc_list.p, c.tet_grad.p, shape);
+    SGIPC_LAUNCH_CHECK();
+}
+
+void fem_param_adjoint(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                       double dt, const int* btype, const double* adj, double* dmu, double* dlam) {
+    if (T <= 0) return;
+    hipLaunchKernelGGL(k_fem_param_adjoint, dim3(div_up(T, 128)), dim3(128), 0, c.stream, T, x, (const uint4*)tets,
+                       DmInv, vol, dt * dt, btype, adj, dmu, dlam);
     SGIPC_LAUNCH_CHECK();
 }
 

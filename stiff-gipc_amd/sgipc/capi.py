@@ -54,6 +54,12 @@ SIGNATURES = {
     "sgipc_assemble_fem": ([P, I, P, P, P, P, P, P, D, P, I, LL, P], I),
     "sgipc_mass_triplets": ([P, I, P, I, LL], I),
     "sgipc_fem_rhs": ([P, I, P, P, P, P], I),
+    "sgipc_ground_detect": ([P, I, P, P, P, D, D, P, C.POINTER(I)], I),
+    "sgipc_assemble_ground": ([P, I, P, P, P, D, D, D, P, I, LL, P], I),
+    "sgipc_ground_energy": ([P, I, P, P, P, D, D, D, C.POINTER(D)], I),
+    "sgipc_ground_step_bound": ([P, I, P, P, P, P, D, D, C.POINTER(D)], I),
+    "sgipc_set_fem_projection": ([P, I], I),
+    "sgipc_fem_param_adjoint": ([P, I, P, P, P, P, D, P, P, P, P], I),
     "sgipc_classify_contacts": ([P, I, P, I, P, P, P, D, P, P, I, C.POINTER(I), P], I),
     "sgipc_assemble_barrier": ([P, I, P, P, P, P, P, D, D, P, I, LL, P], I),
     "sgipc_convert": ([P, LL, I, C.POINTER(I)], I),

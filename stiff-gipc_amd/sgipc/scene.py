@@ -305,3 +305,49 @@ def facing_contact_candidates(sc, h=None):
     pt = np.asarray(out_pt, np.int32).reshape(-1, 4)
     ee = np.asarray(out_ee, np.int32).reshape(-1, 4)
     return pt, ee
+
+
+def lame_coefficients(nu):
+    """(c_mu, c_lam) with (mu', lambda') = E * (c_mu, c_lam): the SNK1
+    parameters are linear in Young's modulus at fixed Poisson ratio."""
+    mu, lam = lame_snk(1.0, nu)
+    return float(mu), float(lam)
+
+
+def cantilever_scene(nx=60, ny=12, nz=12, seed=4, length=1.0, E_mean=1e5, E_logstd=0.2, nu=0.4, density=1000.0,
+                     dt=0.01, gravity=-9.8, clearance=0.05, rel_dhat=1e-2, kappa=1e6, vel=0.1, target_std=0.01,
+                     bank=16):
+    """Cantilever bar of nx x ny x nz cells (edge length/nx, 5 tets/cell)
+    clamped at its x = 0 face, its lower face `clearance` above the ground
+    plane y = 0.  Per-tet ln E ~ N(ln E_mean, E_logstd^2), seeded uniform
+    initial velocities in [-vel, vel] on the free vertices, and a seeded
+    Gaussian target displacement field (std target_std) for the loss.  The
+    ground barrier uses dhat = rel_dhat * length (a distance; stored
+    squared, as the reference's dHat) and stiffness kappa.  Returns the
+    Scene at rest (x = x_rest); meta carries log_E, v0, target, the surface
+    vertex ids and the ground plane."""
+    h = length / nx
+    verts, tets, ijk = tet_grid(nx, ny, nz, h, (0.0, clearance, 0.0))
+    part = block_partition(ijk, bank)
+    perm, inv, part_sorted = renumber_by_part(part)
+    x_rest = verts[perm]
+    tets = inv[tets.astype(np.int64)].astype(np.uint32)
+    ijk = ijk[perm]
+    btype = (ijk[:, 0] == 0).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    log_E = rng.normal(np.log(E_mean), E_logstd, size=tets.shape[0])
+    DmInv, vol, mass, mu, lam = rest_precompute(x_rest, tets, np.exp(log_E), nu, density)
+    v0 = rng.uniform(-vel, vel, size=x_rest.shape)
+    v0[btype != 0] = 0.0
+    target = x_rest + rng.normal(0.0, target_std, size=x_rest.shape)
+    surf = np.unique(boundary_triangles(tets).reshape(-1)).astype(np.uint32)
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = x_rest + dt * v0 + dt * dt * g
+    x_tilde[btype != 0] = x_rest[btype != 0]
+    dhat = rel_dhat * length
+    return Scene("cantilever%dx%dx%d" % (nx, ny, nz), x_rest, x_rest.copy(), x_tilde, tets,
+                 part_sorted.astype(np.int32), btype, DmInv, vol, mass, mu, lam, dt, dhat=dhat * dhat, kappa=kappa,
+                 bank=bank,
+                 meta=dict(cells=(nx, ny, nz), nu=nu, density=density, gravity=gravity, seed=seed, log_E=log_E,
+                           v0=v0, target=target, surf=surf, ground_normal=np.array([0.0, 1.0, 0.0]),
+                           ground_offset=0.0, dhat_dist=dhat))
