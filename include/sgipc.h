@@ -207,6 +207,29 @@ int sgipc_precond_apply(sgipc_ctx* ctx, const double* d_r, double* d_z);
 int sgipc_solve(sgipc_ctx* ctx, const double* d_b, double* d_x, const sgipc_solve_opts* opts,
                 sgipc_solve_stats* stats);
 
+/* ---- batched scenes ----------------------------------------------------------
+ * No reference counterpart (the reference solves one scene at a time with
+ * PCGSolver::pcg, linear_system/solver/pcg_solver.cu:174-267).  Independent
+ * scenes are assembled into one triplet store at disjoint vertex ranges and
+ * converted together; scene s owns block rows [h_row_start[s],
+ * h_row_start[s+1]) (h_row_start[0] = 0, h_row_start[n_scenes] = block rows),
+ * and no block may couple two scenes (checked).  With the MAS preconditioner
+ * built over all rows (block-diagonal per scene), every scene runs its own
+ * PCG -- own alpha, beta, stop test and iteration count, the reference loop
+ * per scene -- in shared graph-captured launches.  SGIPC_PRECISION_MIXED:
+ * FP32 inner solves with per-scene FP64 refinement (needs
+ * SGIPC_PRECOND_MAS_FP32).  h_stats receives n_scenes entries. */
+int sgipc_solve_batched(sgipc_ctx* ctx, int n_scenes, const int* h_row_start, const double* d_b, double* d_x,
+                        const sgipc_solve_opts* opts, sgipc_solve_stats* h_stats);
+/* Schedule autotuner: times 48 PCG iterations of each candidate schedule on
+ * the converted system (n_scenes > 0: the batched solver, else sgipc_solve)
+ * -- SpMV lanes per block row 4/8/16 and, batched, 2/4/8 iterations per
+ * graph launch -- and keeps the fastest for later solves; d_b is only read. */
+int sgipc_autotune(sgipc_ctx* ctx, const double* d_b, int n_scenes, const int* h_row_start, int precision,
+                   int* h_group, int* h_iters_per_graph);
+/* Set the schedule directly (group 4, 8 or 16; 1..64 iterations per graph). */
+int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
+
 /* ---- FP64 energies (line search) ------------------------------------------ */
 /* Fixed-order FP64 reductions, result on the host (synchronises the stream);
  * the terms of GIPC::computeEnergy (StiffGIPC/GIPC.cu:10520-10579). */

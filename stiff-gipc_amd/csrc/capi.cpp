@@ -346,6 +346,69 @@ int sgipc_precond_apply(sgipc_ctx* c, const double* r, double* z) {
     });
 }
 
+static sgipc::SolveOpts solve_opts_of(const sgipc_solve_opts* o) {
+    sgipc::SolveOpts so;
+    if (o) {
+        so.precision = o->precision;
+        so.stop_rule = o->stop_rule;
+        so.tol = o->tol;
+        so.max_iter_ratio = o->max_iter_ratio;
+        so.max_iter = o->max_iter;
+        so.check_every = o->check_every;
+    }
+    if (so.stop_rule != 0 && so.stop_rule != 1) throw sgipc::arg_error("solve: unknown stop rule");
+    if (so.precision != 0 && so.precision != 1) throw sgipc::arg_error("solve: unknown precision");
+    if (!(so.tol > 0.0)) throw sgipc::arg_error("solve: tolerance must be positive");
+    return so;
+}
+
+int sgipc_solve_batched(sgipc_ctx* c, int n_scenes, const int* row_start, const double* b, double* x,
+                        const sgipc_solve_opts* o, sgipc_solve_stats* st) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n_scenes <= 0 || !row_start || !b || !x || !st) throw sgipc::arg_error("solve_batched: invalid argument");
+        sgipc::SolveOpts so = solve_opts_of(o);
+        set_device(c);
+        std::vector<sgipc::SolveStats> s2(n_scenes);
+        sgipc::prof_begin(*c, sgipc::PROF_SOLVE);
+        sgipc::pcg_solve_batched(*c, n_scenes, row_start, b, x, so, s2.data());
+        sgipc::prof_end(*c, sgipc::PROF_SOLVE);
+        sgipc::prof_flush(*c);
+        for (int i = 0; i < n_scenes; ++i) {
+            st[i].iterations = s2[i].iterations;
+            st[i].converged = s2[i].converged;
+            st[i].refinements = s2[i].refinements;
+            st[i].rel_residual = s2[i].rel_residual;
+            st[i].rz0 = s2[i].rz0;
+        }
+    });
+}
+
+int sgipc_autotune(sgipc_ctx* c, const double* b, int n_scenes, const int* row_start, int precision, int* group,
+                   int* iters_per_graph) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!b || n_scenes < 0 || (n_scenes > 0 && !row_start) || (precision != 0 && precision != 1))
+            throw sgipc::arg_error("autotune: invalid argument");
+        set_device(c);
+        int g = 0, pg = 0;
+        sgipc::autotune(*c, b, n_scenes, row_start, precision, g, pg);
+        if (group) *group = g;
+        if (iters_per_graph) *iters_per_graph = pg;
+    });
+}
+
+int sgipc_set_schedule(sgipc_ctx* c, int group, int iters_per_graph) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (group != 4 && group != 8 && group != 16) throw sgipc::arg_error("set_schedule: group must be 4, 8 or 16");
+        if (iters_per_graph < 1 || iters_per_graph > 64)
+            throw sgipc::arg_error("set_schedule: iterations per graph must be in [1, 64]");
+        c->spmv_group = group;
+        c->batch_iters_per_graph = iters_per_graph;
+    });
+}
+
 int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts* o, sgipc_solve_stats* st) {
     CHECK_CTX(c);
     return guard(c, [&] {

@@ -76,6 +76,23 @@ struct MasState {
 struct PcgGraph {
     hipGraphExec_t exec = nullptr;
     size_t key = 0;
+    PcgGraph() = default;
+    PcgGraph(const PcgGraph&) = delete;
+    PcgGraph& operator=(const PcgGraph&) = delete;
+    ~PcgGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+// Batched PCG over scenes concatenated block-row-wise (pcg_batch.hip)
+struct BatchWork {
+    int nch = 0;                                    // row chunks (never straddling a scene)
+    DBuf<int> chunk_row, chunk_scene, scene_chunk;  // nch + 1, nch, S + 1
+    DBuf<double> part_pap, part_rr, part_rz;        // one partial per chunk
+    DBuf<double> bb, rz, rz0, alpha, beta, tol, segout;  // per scene
+    DBuf<int> stop, iters, noconv;                  // per scene
+    DBuf<int> ctrl, bad, rstart;                    // C_* words; coupling check counter; row starts
+    PcgGraph graph64, graph32, graph_mixed;
 };
 
 struct PcgWork {
@@ -145,6 +162,9 @@ struct Ctx {
     MasState mas;
 
     PcgWork pcg;
+    BatchWork batch;
+    int batch_iters_per_graph = 4;  // batched PCG iterations captured per graph launch
+    DBuf<double> tune_x;            // autotuner output scratch
     DBuf<int> scan_tmp;
     // contact classification workspaces
     DBuf<int> cls_buf, cls_flag, cls_scan, cls_pos;
@@ -238,6 +258,10 @@ struct SolveStats {
     int refinements = 0;
 };
 void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStats& st);
+void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, double* x, const SolveOpts& o,
+                       SolveStats* st);
+void batch_check_coupling(Ctx& c, const std::vector<int>& row_start);
+void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precision, int& group, int& per_graph);
 void precond_apply(Ctx& c, const double* r, double* z);
 
 }  // namespace sgipc

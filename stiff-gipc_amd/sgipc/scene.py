@@ -351,3 +351,68 @@ def cantilever_scene(nx=60, ny=12, nz=12, seed=4, length=1.0, E_mean=1e5, E_logs
                  meta=dict(cells=(nx, ny, nz), nu=nu, density=density, gravity=gravity, seed=seed, log_E=log_E,
                            v0=v0, target=target, surf=surf, ground_normal=np.array([0.0, 1.0, 0.0]),
                            ground_offset=0.0, dhat_dist=dhat))
+
+
+def ground_box_scene(n_target, aspect, E, nu, seed, dt=0.01, density=1000.0, gravity=-9.8, size=0.5,
+                     rel_dhat=1e-3, kappa=1e4, bank=16, origin_xz=(0.0, 0.0)):
+    """One free box of about n_target vertices with side ratios `aspect`
+    (longest side `size`), resting 0.5 dhat above the ground plane y = 0
+    (dhat = rel_dhat * size, stored squared), seeded Gaussian position
+    perturbation (1% of the edge), zero velocity, gravity."""
+    a = np.asarray(aspect, dtype=np.float64)
+    k = (n_target / np.prod(a)) ** (1.0 / 3.0)
+    cells = np.maximum(1, np.round(k * a - 1.0)).astype(int)
+    h = size / cells.max()
+    dhat = rel_dhat * size
+    body = make_box(int(cells[0]), int(cells[1]), int(cells[2]), h, origin=(origin_xz[0], 0.5 * dhat, origin_xz[1]),
+                    bank=bank)
+    rng = np.random.default_rng(seed)
+    x_rest = body.x_rest
+    DmInv, vol, mass, mu, lam = rest_precompute(x_rest, body.tets, E, nu, density)
+    x = x_rest + rng.normal(0.0, 0.01 * h, size=x_rest.shape)
+    low = x_rest[:, 1] < 0.5 * dhat + 1e-12
+    x[low, 1] = x_rest[low, 1]                      # the bottom face keeps its 0.5 dhat gap
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = x + dt * dt * g
+    surf = np.unique(boundary_triangles(body.tets).reshape(-1)).astype(np.uint32)
+    return Scene("box%d" % x.shape[0], x_rest, x, x_tilde, body.tets, body.part, body.btype, DmInv, vol, mass, mu,
+                 lam, dt, dhat=dhat * dhat, kappa=kappa, bank=bank,
+                 meta=dict(cells=tuple(int(c) for c in cells), E=E, nu=nu, seed=seed, surf=surf,
+                           ground_normal=np.array([0.0, 1.0, 0.0]), ground_offset=0.0, gravity=gravity,
+                           dhat_dist=dhat))
+
+
+def batch_scenes(n_scenes=256, seed=5, n_min=500, n_max=20000, bank=16):
+    """Independent scenes of uneven size: vertex counts log-uniform in
+    [n_min, n_max], side ratios uniform in [1, 4], E log-uniform in
+    [1e4, 1e7], nu uniform in [0.3, 0.48], each a free box on the ground
+    plane; scene i is seeded with seed + i."""
+    out = []
+    for i in range(n_scenes):
+        rng = np.random.default_rng(seed + i)
+        n_t = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        aspect = rng.uniform(1.0, 4.0, size=3)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+        nu = float(rng.uniform(0.3, 0.48))
+        out.append(ground_box_scene(n_t, aspect, E, nu, seed + i, bank=bank, origin_xz=(0.6 * (i % 16), 0.6 * (i // 16))))
+    return out
+
+
+def union_scene(scenes):
+    """Concatenate independent scenes (vertex, tet and part ranges in
+    order); returns (Scene, row_start) with row_start[s] the first vertex of
+    scene s.  The scenes share dt, dhat, kappa and the ground plane."""
+    offs = np.cumsum([0] + [s.n for s in scenes])
+    poff = np.cumsum([0] + [int(s.part.max()) + 1 for s in scenes])
+    cat = lambda f: np.concatenate([f(s) for s in scenes])
+    tets = np.concatenate([s.tets.astype(np.int64) + o for s, o in zip(scenes, offs)]).astype(np.uint32)
+    part = np.concatenate([s.part.astype(np.int64) + p for s, p in zip(scenes, poff)]).astype(np.int32)
+    surf = np.concatenate([s.meta["surf"].astype(np.int64) + o for s, o in zip(scenes, offs)]).astype(np.uint32)
+    s0 = scenes[0]
+    sc = Scene("batch%d" % len(scenes), cat(lambda s: s.x_rest), cat(lambda s: s.x), cat(lambda s: s.x_tilde), tets,
+               part, cat(lambda s: s.btype).astype(np.int32), cat(lambda s: s.DmInv), cat(lambda s: s.vol),
+               cat(lambda s: s.mass), cat(lambda s: s.mu), cat(lambda s: s.lam), s0.dt, dhat=s0.dhat,
+               kappa=s0.kappa, bank=s0.bank,
+               meta=dict(surf=surf, ground_normal=s0.meta["ground_normal"], ground_offset=s0.meta["ground_offset"],
+                         gravity=s0.meta["gravity"], nu=s0.meta["nu"], scenes=len(scenes)))
+    return sc, offs.astype(np.int32)
