@@ -138,6 +138,23 @@ public:
         check(sgipc_mass_triplets(ctx_, n, d_mass, 0, trip_.global_triplet_offset), ctx_, "sgipc_mass_triplets");
         trip_.global_triplet_offset += n;
     }
+    // ground plane n.x = offset (GIPC::_groundNormal/_groundOffset):
+    // close vertices of d_cand into d_close (candidate order), then their
+    // barrier gradient and 3x3 blocks (GIPC.cu:6433-6514)
+    int ground_detect(int n_cand, const uint32_t* d_cand, const double* d_x, const double normal[3], double offset,
+                      double dhat, uint32_t* d_close) {
+        int k = 0;
+        check(sgipc_ground_detect(ctx_, n_cand, d_cand, d_x, normal, offset, dhat, d_close, &k), ctx_,
+              "sgipc_ground_detect");
+        return k;
+    }
+    void assemble_ground(int n_close, const uint32_t* d_close, const double* d_x, const double normal[3],
+                         double offset, double dhat, double kappa, const int* d_btype, double* d_fb) {
+        check(sgipc_assemble_ground(ctx_, n_close, d_close, d_x, normal, offset, dhat, kappa, d_btype, 0,
+                                    trip_.global_triplet_offset, d_fb),
+              ctx_, "sgipc_assemble_ground");
+        trip_.global_triplet_offset += n_close;
+    }
     // FEMLinearSubsystem::assemble: b = fb + shape, zero on pinned vertices
     void assemble_rhs(int n, const double* d_fb, const double* d_shape, const int* d_btype, double* d_b) {
         check(sgipc_fem_rhs(ctx_, n, d_fb, d_shape, d_btype, d_b), ctx_, "sgipc_fem_rhs");
@@ -155,6 +172,29 @@ public:
                            cfg_.check_every};
         check(sgipc_solve(ctx_, d_b, d_x, &o, &stats_), ctx_, "sgipc_solve");
         return (std::size_t)stats_.iterations;
+    }
+    // Independent scenes assembled at disjoint vertex ranges into the one
+    // triplet store: scene s owns block rows [row_start[s], row_start[s+1]).
+    // Every scene runs its own PCG in shared launches; per-scene stats in
+    // `stats` (resized to the scene count).  No reference counterpart.
+    void solve_linear_system_batched(const std::vector<int>& row_start, const double* d_b, double* d_x,
+                                     std::vector<sgipc_solve_stats>& stats) {
+        const int S = (int)row_start.size() - 1;
+        if (S <= 0) throw Error(SGIPC_ERR_INVALID_ARG, "solve_linear_system_batched: no scenes");
+        int K = 0;
+        check(sgipc_convert(ctx_, trip_.global_triplet_offset, n_fem_, &K), ctx_, "sgipc_convert");
+        check(sgipc_precond_build(ctx_, precond_, 0, n_fem_, d_pairs_, n_pairs_), ctx_, "sgipc_precond_build");
+        sgipc_solve_opts o{cfg_.precision, cfg_.stop_rule, cfg_.global_tol_rate, cfg_.max_iter_ratio, cfg_.max_iter,
+                           cfg_.check_every};
+        stats.assign(S, sgipc_solve_stats{});
+        check(sgipc_solve_batched(ctx_, S, row_start.data(), d_b, d_x, &o, stats.data()), ctx_,
+              "sgipc_solve_batched");
+    }
+    // time the candidate schedules on the converted system, keep the fastest
+    void autotune(const double* d_b, const std::vector<int>& row_start = {}) {
+        const int S = row_start.empty() ? 0 : (int)row_start.size() - 1;
+        check(sgipc_autotune(ctx_, d_b, S, S ? row_start.data() : nullptr, cfg_.precision, nullptr, nullptr), ctx_,
+              "sgipc_autotune");
     }
     // helpers of the solver contract (i_linear_system_solver.h:43-53)
     void spmv(double a, const double* d_x, double b, double* d_y) {
