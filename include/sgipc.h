@@ -162,6 +162,22 @@ int sgipc_fem_param_adjoint(sgipc_ctx* ctx, int T, const double* d_x, const uint
                             const double* d_DmInv, const double* d_vol, double dt, const int* d_btype,
                             const double* d_adj, double* d_dmu, double* d_dlam);
 
+/* Broad phase: linear BVHs over the surface triangles (3 uint32 per face)
+ * and surface edges (2 uint32 per edge), queried per surface vertex and per
+ * edge with boxes inflated by sqrt(dhat) (dhat squared, as GIPC::dHat);
+ * S/mlbvh.cu _calcMChash / _calcInternalNodes / _calcInternalAABB
+ * (StiffGIPC/mlbvh.cu:1206-1296), _selfQuery_vf (:1306-1406), _selfQuery_ee
+ * (:1509-1632).  Emits candidate int4 quadruples for
+ * sgipc_classify_contacts: (p, f0, f1, f2) with p not in the face, and
+ * (a0, a1, b0, b1) for edges a < b sharing no vertex; pairs whose four
+ * vertices all have btype >= 2, or that join two vertices of the same body
+ * (d_body_id, NULL = all -1), are skipped.  Order: query index, then
+ * traversal order (deterministic).  *h_n_pt / *h_n_ee receive the counts;
+ * SGIPC_ERR_CAPACITY (nothing written) when they exceed the capacities. */
+int sgipc_broad_phase(sgipc_ctx* ctx, int n_surf_verts, const uint32_t* d_surf_verts, int n_faces,
+                      const uint32_t* d_faces, int n_edges, const uint32_t* d_edges, const double* d_x,
+                      const int* d_btype, const int* d_body_id, double dhat, int* d_pt_out, int pt_capacity,
+                      int* h_n_pt, int* d_ee_out, int ee_capacity, int* h_n_ee);
 /* Narrow phase: classify candidate point-triangle (p, t0, t1, t2) and
  * edge-edge (a0, a1, b0, b1) vertex quadruples into the reference's int4 pair
  * encoding, keeping pairs with squared distance < dhat; PT candidates first,

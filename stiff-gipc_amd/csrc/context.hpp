@@ -84,6 +84,18 @@ struct PcgGraph {
     }
 };
 
+// Linear BVH over surface primitives (lbvh.hip)
+struct LbvhTree {
+    int m = 0;
+    DBuf<char> leaf, scene, nodes, nbox, sort_tmp;  // boxes / nodes as raw bytes (types private to lbvh.hip)
+    DBuf<unsigned> code, idx, flags;
+    DBuf<unsigned long long> key;
+};
+struct Lbvh {
+    LbvhTree tri, edge;
+    DBuf<int> cnt, off, err;
+};
+
 // Batched PCG over scenes concatenated block-row-wise (pcg_batch.hip)
 struct BatchWork {
     int nch = 0;                                    // row chunks (never straddling a scene)
@@ -163,6 +175,7 @@ struct Ctx {
 
     PcgWork pcg;
     BatchWork batch;
+    Lbvh lbvh;
     int batch_iters_per_graph = 4;  // batched PCG iterations captured per graph launch
     DBuf<double> tune_x;            // autotuner output scratch
     DBuf<int> scan_tmp;
@@ -262,6 +275,10 @@ void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, d
                        SolveStats* st);
 void batch_check_coupling(Ctx& c, const std::vector<int>& row_start);
 void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precision, int& group, int& per_graph);
+// candidate counts in n_pt / n_ee; quadruples written only when they fit the capacities
+void broad_phase(Ctx& c, int nsv, const unsigned* sv, int nf, const unsigned* faces, int ne, const unsigned* edges,
+                 const double* x, const int* btype, const int* body, double dhat, int* pt_out, int pt_cap, int& n_pt,
+                 int* ee_out, int ee_cap, int& n_ee);
 void precond_apply(Ctx& c, const double* r, double* z);
 
 }  // namespace sgipc

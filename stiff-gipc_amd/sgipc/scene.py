@@ -416,3 +416,15 @@ def union_scene(scenes):
                meta=dict(surf=surf, ground_normal=s0.meta["ground_normal"], ground_offset=s0.meta["ground_offset"],
                          gravity=s0.meta["gravity"], nu=s0.meta["nu"], scenes=len(scenes)))
     return sc, offs.astype(np.int32)
+
+
+def surface_primitives(tets):
+    """Surface triangles (boundary faces, as stored in the tet), surface
+    edges (unique, sorted (lo, hi), lexicographic order) and surface
+    vertices (ascending) of a tet mesh, as uint32 arrays -- the reference's
+    _faces / _edges / _surfVerts inputs of the broad phase."""
+    faces = np.ascontiguousarray(boundary_triangles(tets), np.uint32)
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [0, 2]]]).astype(np.int64)
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    verts = np.unique(faces.reshape(-1))
+    return faces, np.ascontiguousarray(e, np.uint32), np.ascontiguousarray(verts, np.uint32)

@@ -469,3 +469,38 @@ def test_barrier_assemble_layout(oracle, twobox5):
     # each pair's nonzero blocks are the upper blocks of its dense Hessian
     E = oracle.barrier_energy(pairs, sc.x, sc.x_rest, sc.dhat, sc.kappa)
     assert E > 0 and np.isfinite(fb).all()
+
+
+def test_broad_phase_reference_is_complete(oracle):
+    """The brute-force box candidates lose no pair the narrow phase keeps:
+    classifying every point-triangle and edge-edge pair of the surfaces
+    gives the same contact pairs as classifying only the box candidates."""
+    from sgipc import scene
+    from broad_phase_ref import candidates
+    sc = scene.two_box_scene(4, seed=2)
+    faces, edges, surf = scene.surface_primitives(sc.tets)
+    pt, ee = candidates(sc.x, faces, edges, surf, sc.dhat)
+    assert len(pt) and len(ee)
+    f = faces.astype(np.int64); e = edges.astype(np.int64)
+    P, F = np.meshgrid(surf.astype(np.int64), np.arange(len(f)), indexing="ij")
+    allpt = np.stack([P.ravel(), f[F.ravel(), 0], f[F.ravel(), 1], f[F.ravel(), 2]], 1)
+    allpt = allpt[(allpt[:, 0] != allpt[:, 1]) & (allpt[:, 0] != allpt[:, 2]) & (allpt[:, 0] != allpt[:, 3])]
+    I, J = np.triu_indices(len(e), 1)
+    allee = np.concatenate([e[I], e[J]], 1)
+    share = (allee[:, 0] == allee[:, 2]) | (allee[:, 0] == allee[:, 3]) | (allee[:, 1] == allee[:, 2]) | \
+        (allee[:, 1] == allee[:, 3])
+    allee = allee[~share]
+    pa, _, ca = oracle.classify(allpt.astype(np.int32), allee.astype(np.int32), sc.x, sc.x_rest, sc.dhat)
+    pb, _, cb = oracle.classify(pt, ee, sc.x, sc.x_rest, sc.dhat)
+    assert len(pa) > 0 and np.array_equal(ca, cb)
+    assert set(map(tuple, _canon(pa, sc.n))) == set(map(tuple, _canon(pb, sc.n)))
+    assert len(pt) < len(allpt) // 10                      # and the boxes do prune
+
+
+def _canon(pairs, n):
+    """encoded pairs with negative fourth entries made uniform: the barrier
+    reads only their sign, and parallel edge pairs store -(candidate)-2
+    there, which depends on the candidate list"""
+    q = pairs.copy()
+    q[q[:, 3] < 0, 3] = -1
+    return q
