@@ -243,6 +243,12 @@ int sgipc_solve_batched(sgipc_ctx* ctx, int n_scenes, const int* h_row_start, co
  * graph launch -- and keeps the fastest for later solves; d_b is only read. */
 int sgipc_autotune(sgipc_ctx* ctx, const double* d_b, int n_scenes, const int* h_row_start, int precision,
                    int* h_group, int* h_iters_per_graph);
+/* on != 0: the MAS build sums every coarse-level bank entry in a fixed
+ * order (records sorted by entry, then by BCOO block) instead of with FP64
+ * atomics, so preconditioners -- and with them PCG iterates -- repeat
+ * bitwise from run to run (slower build).  Default 0.  The reference
+ * projects with atomics (MASPreconditioner.cu:1816-2093). */
+int sgipc_set_deterministic(sgipc_ctx* ctx, int on);
 /* Set the schedule directly (group 4, 8 or 16; 1..64 iterations per graph). */
 int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
 

@@ -138,6 +138,22 @@ public:
         check(sgipc_mass_triplets(ctx_, n, d_mass, 0, trip_.global_triplet_offset), ctx_, "sgipc_mass_triplets");
         trip_.global_triplet_offset += n;
     }
+    // collision detection (GIPC::buildCP, mlbvh.cu): LBVH candidates, then
+    // the narrow phase into reference-encoded pairs and their class counts;
+    // the caller sizes the buffers (SGIPC_ERR_CAPACITY reports the need)
+    int detect_contacts(int n_surf, const uint32_t* d_surf, int n_faces, const uint32_t* d_faces, int n_edges,
+                        const uint32_t* d_edges, const double* d_x, const double* d_xrest, const int* d_btype,
+                        double dhat, int* d_pt, int* d_ee, int cand_capacity, int* d_pairs, int* d_mat_index,
+                        int pair_capacity, int counts[3]) {
+        int npt = 0, nee = 0, np = 0;
+        check(sgipc_broad_phase(ctx_, n_surf, d_surf, n_faces, d_faces, n_edges, d_edges, d_x, d_btype, nullptr, dhat,
+                                d_pt, cand_capacity, &npt, d_ee, cand_capacity, &nee),
+              ctx_, "sgipc_broad_phase");
+        check(sgipc_classify_contacts(ctx_, npt, d_pt, nee, d_ee, d_x, d_xrest, dhat, d_pairs, d_mat_index,
+                                      pair_capacity, &np, counts),
+              ctx_, "sgipc_classify_contacts");
+        return np;
+    }
     // ground plane n.x = offset (GIPC::_groundNormal/_groundOffset):
     // close vertices of d_cand into d_close (candidate order), then their
     // barrier gradient and 3x3 blocks (GIPC.cu:6433-6514)

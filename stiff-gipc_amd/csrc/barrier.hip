@@ -11,6 +11,7 @@
 // s = sqrt(d / dhat)); the non-mollified Hessian is lambda0 g g^T, the
 // mollified one a PSD-projected 2x2 Gauss-Newton form in (c, s).
 #include "context.hpp"
+#include <rocprim/device/device_radix_sort.hpp>
 #include "pcg_dev.hpp"
 
 namespace sgipc {
@@ -125,7 +126,9 @@ __global__ void __launch_bounds__(128) k_barrier(int np, const int4* __restrict_
                                                  const double* __restrict__ xr, double dhat, double kappa,
                                                  const int* __restrict__ btype, int index_offset,
                                                  double* __restrict__ fb, double* __restrict__ tv,
-                                                 int* __restrict__ tr, int* __restrict__ tc) {
+                                                 int* __restrict__ tr, int* __restrict__ tc,
+                                                 double* __restrict__ gscr, unsigned* __restrict__ gkey,
+                                                 unsigned* __restrict__ grec, unsigned gsentinel) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     Pair P = decode(pairs[p]);
@@ -200,11 +203,22 @@ __global__ void __launch_bounds__(128) k_barrier(int np, const int4* __restrict_
             F.m00 = m00; F.m01 = m01; F.m11 = m11;
         }
     }
-    // gradient (FP64 atomics, as the reference)
-    if (active)
+    // gradient: FP64 atomics, as the reference; or, in the ordered mode, one
+    // record per pair vertex for a sorted per-vertex sum (k_grad_gather)
+    if (gscr) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const bool use = active && a < P.nv;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) gscr[12 * (size_t)p + 3 * a + k] = use ? grad[3 * a + k] : 0.0;
+            gkey[4 * (size_t)p + a] = use ? (unsigned)P.v[a] : gsentinel;
+            grec[4 * (size_t)p + a] = 4u * (unsigned)p + (unsigned)a;
+        }
+    } else if (active) {
         for (int a = 0; a < P.nv; ++a)
 #pragma unroll
             for (int k = 0; k < 3; ++k) atomicAdd(fb + 3 * (size_t)P.v[a] + k, grad[3 * a + k]);
+    }
     // triplets: full nv x nv block grid, then the upper/boundary remap
     long long off;
     int mi = mat_index[p];
@@ -387,6 +401,24 @@ __global__ void k_class_scatter(int m, const int4* __restrict__ enc, const int* 
     mat_index[o] = pos[(size_t)(k - 1) * (m + 1) + i];
 }
 
+// fb[v] += the sorted records of vertex v, summed in pair order (one head
+// thread per vertex)
+__global__ void k_grad_gather(long long m, const unsigned* __restrict__ key, const unsigned* __restrict__ rec,
+                              const double* __restrict__ gscr, unsigned sentinel, double* __restrict__ fb) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const unsigned v = key[i];
+    if (v == sentinel || (i > 0 && key[i - 1] == v)) return;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (long long j = i; j < m && key[j] == v; ++j) {
+        const double* g = gscr + 3 * (size_t)rec[j];  // record 4 p + a -> gradient row 3 (4 p + a)
+        g0 += g[0]; g1 += g[1]; g2 += g[2];
+    }
+    fb[3 * (size_t)v] += g0;
+    fb[3 * (size_t)v + 1] += g1;
+    fb[3 * (size_t)v + 2] += g2;
+}
+
 }  // namespace
 
 void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,
@@ -397,10 +429,31 @@ void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, co
     if (triplet_offset < 0 || triplet_offset + need > c.t_cap) throw capacity_error("barrier: triplet capacity exceeded");
     if (np <= 0) return;
     prof_begin(c, PROF_BARRIER);
+    double* gscr = nullptr;
+    unsigned *gkey = nullptr, *grec = nullptr;
+    const unsigned sentinel = 0xFFFFFFFFu;  // sorts after every vertex id
+    const long long m = 4LL * np;
+    if (c.deterministic) {
+        gscr = c.grad_scr.reserve(12 * (size_t)np);
+        gkey = c.grad_key.reserve(2 * (size_t)m);
+        grec = c.grad_rec.reserve(2 * (size_t)m);
+    }
     hipLaunchKernelGGL(k_barrier, dim3(div_up(np, 128)), dim3(128), 0, c.stream, np, (const int4*)pairs, mat_index,
                        counts[0], counts[1], counts[2], x, xrest, dhat, kappa, btype, index_offset, fb,
-                       c.t_vals + 9 * triplet_offset, c.t_rows + triplet_offset, c.t_cols + triplet_offset);
+                       c.t_vals + 9 * triplet_offset, c.t_rows + triplet_offset, c.t_cols + triplet_offset, gscr,
+                       gkey, grec, sentinel);
     SGIPC_LAUNCH_CHECK();
+    if (c.deterministic) {
+        size_t bytes = 0;
+        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32,
+                                            c.stream));
+        void* tmp = c.grad_sort_tmp.reserve(bytes + 1);
+        SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32, c.stream));
+        hipLaunchKernelGGL(k_grad_gather, dim3(div_up(m, 256)), dim3(256), 0, c.stream, m,
+                           (const unsigned*)(gkey + m), (const unsigned*)(grec + m), (const double*)gscr, sentinel,
+                           fb);
+        SGIPC_LAUNCH_CHECK();
+    }
     prof_end(c, PROF_BARRIER);
 }
 

@@ -273,6 +273,11 @@ int sgipc_broad_phase(sgipc_ctx* c, int n_surf_verts, const uint32_t* surf_verts
     });
 }
 
+int sgipc_set_deterministic(sgipc_ctx* c, int on) {
+    CHECK_CTX(c);
+    return guard(c, [&] { c->deterministic = on ? 1 : 0; });
+}
+
 int sgipc_set_fem_projection(sgipc_ctx* c, int on) {
     CHECK_CTX(c);
     return guard(c, [&] { c->fem_project = on ? 1 : 0; });

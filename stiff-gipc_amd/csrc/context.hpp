@@ -57,6 +57,8 @@ struct MasState {
     DBuf<int> coarse_space;  // L*n
     DBuf<int> going_next;    // cap
     DBuf<unsigned> comp_mask; // cap: bank lanes of each coarse node's component (levels >= 1)
+    DBuf<unsigned> proj_key, proj_rec;  // ordered projection: records (entry, block/level), double-buffered
+    DBuf<char> proj_sort_tmp;
     size_t going_cap = 0;
     DBuf<int> coarse_table;  // 6*n (itable layout)
     DBuf<int> node_of;       // L*n : global node id of vertex at level l (l=0: slot)
@@ -167,6 +169,10 @@ struct Ctx {
     SymBcoo bsr;
     bool have_bsr = false;
     int spmv_group = 8;  // lanes per block row in the PCG SpMV (env SGIPC_SPMV_G: 4, 8, 16)
+    int deterministic = 0;  // 1: MAS projection and barrier gradient summed in a fixed order (bitwise repeatable)
+    DBuf<double> grad_scr;               // ordered barrier gradient: per-pair rows (12 doubles per pair)
+    DBuf<unsigned> grad_key, grad_rec;   // (vertex, record) pairs, double-buffered for the sort
+    DBuf<char> grad_sort_tmp;
 
     // preconditioner
     int precond_kind = 0;

@@ -15,6 +15,7 @@
 // a deterministic per-bank component sum (no FP atomics); the apply is a
 // bank-per-workgroup dense GEMV on the symmetric inverse.
 #include "context.hpp"
+#include <rocprim/device/device_radix_sort.hpp>
 #include <utility>
 #include "pcg_dev.hpp"
 #include <algorithm>
@@ -452,6 +453,70 @@ __global__ void __launch_bounds__(512) k_project_banked(
     }
     double* out = top_scratch + (size_t)b * top_nb * NB;
     for (int q = t; q < top_nb * NB; q += blockDim.x) out[q] = top[q];
+}
+
+// Deterministic projection (opt-in): every BCOO block writes its level-0
+// target directly and, for each coarse level where its node pair shares a
+// bank, one (entry, block) record; a stable radix sort by entry groups the
+// records in block order and one thread per entry sums its group
+// sequentially.  Same result on every run, for any bank size or numbering.
+__global__ void k_proj_records(int K, const int* __restrict__ rows, const int* __restrict__ cols,
+                               const double* __restrict__ vals, int off, int n, int B, int L,
+                               const int* __restrict__ node_of, double* __restrict__ bank_upper, unsigned sentinel,
+                               unsigned* __restrict__ key, unsigned* __restrict__ rec) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int npack = B * (B + 1) / 2;
+    const int r = rows[k] - off, c = cols[k] - off;
+    const bool in = r >= 0 && c >= 0 && r < n && c < n;
+    for (int l = 1; l < L; ++l) {
+        unsigned kk = sentinel;
+        if (in) {
+            const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
+            if (I / B == J / B) {
+                const int li = I % B, lj = J % B;
+                kk = (unsigned)(I / B) * (unsigned)npack + (unsigned)(li <= lj ? pack_index(B, li, lj) : pack_index(B, lj, li));
+            }
+        }
+        key[(size_t)k * (L - 1) + (l - 1)] = kk;
+        rec[(size_t)k * (L - 1) + (l - 1)] = (unsigned)k * 8u + (unsigned)l;
+    }
+    if (!in) return;
+    const int I = node_of[r], J = node_of[c];
+    if (I / B != J / B) return;
+    double H[9], blk[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
+    const int pidx = project_block(B, I % B, J % B, r == c, H, blk);
+    double* dst = bank_upper + ((size_t)(I / B) * npack + pidx) * 9;  // level 0: unique per BCOO block
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dst[e] += blk[e];
+}
+
+__global__ void k_proj_sum(long long m, const unsigned* __restrict__ key, const unsigned* __restrict__ rec,
+                           const int* __restrict__ rows, const int* __restrict__ cols, const double* __restrict__ vals,
+                           int off, int n, int B, const int* __restrict__ node_of, unsigned sentinel,
+                           double* __restrict__ bank_upper) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const unsigned kk = key[i];
+    if (kk == sentinel || (i > 0 && key[i - 1] == kk)) return;  // one thread per entry: its group's head
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long j = i; j < m && key[j] == kk; ++j) {
+        const unsigned q = rec[j];
+        const int k = (int)(q >> 3), l = (int)(q & 7u);
+        const int r = rows[k] - off, c = cols[k] - off;
+        const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
+        double H[9], blk[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
+        (void)project_block(B, I % B, J % B, r == c, H, blk);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += blk[e];
+    }
+    double* dst = bank_upper + (size_t)kk * 9;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dst[e] = acc[e];
 }
 
 // top-level bank entries = sum of the per-workgroup partials, fixed order
@@ -1071,7 +1136,36 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
     }
     fill<double>(m.bank_upper.p, 0.0, (long long)m.nbanks * npack * 9, s);
     const bool banked = (B == 16) && m.slot_ordered && m.nb1 > 0 && L >= 2;
-    if (c.K > 0 && banked) {
+    if (c.K > 0 && c.deterministic) {
+        if ((long long)c.K >= (1LL << 29)) throw capacity_error("MAS build: too many blocks for the ordered projection");
+        const unsigned sentinel = (unsigned)((long long)m.nbanks * npack);
+        int bits = 1;
+        while (bits < 32 && (1ULL << bits) <= (unsigned long long)sentinel) ++bits;
+        const long long mrec = (long long)c.K * std::max(L - 1, 0);
+        if (L > 1) {
+            unsigned* key = m.proj_key.reserve(2 * (size_t)mrec);
+            unsigned* rec = m.proj_rec.reserve(2 * (size_t)mrec);
+            hipLaunchKernelGGL(k_proj_records, dim3(div_up(c.K, 256)), dim3(256), 0, s, c.K, (const int*)c.t_rows,
+                               (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, B, L,
+                               (const int*)m.node_of.p, m.bank_upper.p, sentinel, key, rec);
+            SGIPC_LAUNCH_CHECK();
+            size_t bytes = 0;
+            SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
+            void* tmp = m.proj_sort_tmp.reserve(bytes + 1);
+            SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
+            hipLaunchKernelGGL(k_proj_sum, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec,
+                               (const unsigned*)(key + mrec), (const unsigned*)(rec + mrec), (const int*)c.t_rows,
+                               (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, B,
+                               (const int*)m.node_of.p, sentinel, m.bank_upper.p);
+            SGIPC_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_proj_records, dim3(div_up(c.K, 256)), dim3(256), 0, s, c.K, (const int*)c.t_rows,
+                               (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, B, L,
+                               (const int*)m.node_of.p, m.bank_upper.p, sentinel, (unsigned*)nullptr,
+                               (unsigned*)nullptr);
+            SGIPC_LAUNCH_CHECK();
+        }
+    } else if (c.K > 0 && banked) {
         // top levels: the highest levels whose banks fit kProjTopBanks
         int top_lvl = L, top_lo = m.nbanks, top_nb = 0;
         for (int l = L - 1; l >= 1; --l) {

@@ -59,6 +59,7 @@ SIGNATURES = {
     "sgipc_ground_energy": ([P, I, P, P, P, D, D, D, C.POINTER(D)], I),
     "sgipc_ground_step_bound": ([P, I, P, P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_set_fem_projection": ([P, I], I),
+    "sgipc_set_deterministic": ([P, I], I),
     "sgipc_broad_phase": ([P, I, P, I, P, I, P, P, P, P, D, P, I, C.POINTER(I), P, I, C.POINTER(I)], I),
     "sgipc_solve_batched": ([P, I, P, P, P, P, P], I),
     "sgipc_autotune": ([P, P, I, P, I, C.POINTER(I), C.POINTER(I)], I),

@@ -39,6 +39,9 @@ class GroundSim:
         self.surf_np = np.ascontiguousarray(m["surf"], np.uint32)
         self.sys = NewtonSystem(scene, device, precond=capi.PRECOND_MAS, extra_triplets=len(self.surf_np))
         s = self.sys
+        # bitwise-repeatable solves: finite differences of a rollout then see
+        # no run-to-run round-off
+        s.ctx.call("sgipc_set_deterministic", 1)
         self.dev = s.dev
         self.n, self.T = s.n, s.T
         self.dt = float(scene.dt)
@@ -247,6 +250,11 @@ def loss_only(sim, log_E, v0, target, steps):
     return float(((xT.reshape(-1, 3) - target) ** 2).sum())
 
 
+def final_positions(sim, log_E, v0, steps):
+    xT, _ = sim.rollout(log_E, v0, steps, ckpt=max(steps, 1))
+    return xT.reshape(-1, 3)
+
+
 def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0):
     """Central differences of L along n_dir seeded unit directions in
     (ln E, free v0) space against the adjoint gradient.  Returns a list of
@@ -261,9 +269,12 @@ def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0):
         nrm = math.sqrt(float((dE * dE).sum() + (dv * dv).sum()))
         dE /= nrm
         dv /= nrm
-        lp = loss_only(sim, log_E + h * dE, v0 + h * dv, target, steps)
-        lm = loss_only(sim, log_E - h * dE, v0 - h * dv, target, steps)
-        fd = (lp - lm) / (2 * h)
+        xp = final_positions(sim, log_E + h * dE, v0 + h * dv, steps)
+        xm = final_positions(sim, log_E - h * dE, v0 - h * dv, steps)
+        # (L+ - L-) / 2h with L = |x_T - target|^2, summed as
+        # (x+ - x-) . (x+ + x- - 2 target) so that no two O(L) numbers are
+        # subtracted (their round-off would swamp derivatives far below L)
+        fd = float(((xp - xm) * (xp + xm - 2.0 * target)).sum()) / (2 * h)
         ad = float((gE * dE).sum() + (gv * dv).sum())
         out.append((fd, ad, abs(fd - ad) / max(abs(ad), 1e-300)))
     return out, L
