@@ -59,6 +59,7 @@ struct MasState {
     DBuf<unsigned> comp_mask; // cap: bank lanes of each coarse node's component (levels >= 1)
     DBuf<unsigned> proj_key, proj_rec;  // ordered projection: records (entry, block/level), double-buffered
     DBuf<char> proj_sort_tmp;
+    DBuf<double> proj_contrib;  // ordered projection: per-record contributions, then piece sums
     size_t going_cap = 0;
     DBuf<int> coarse_table;  // 6*n (itable layout)
     DBuf<int> node_of;       // L*n : global node id of vertex at level l (l=0: slot)

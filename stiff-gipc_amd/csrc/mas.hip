@@ -458,8 +458,9 @@ __global__ void __launch_bounds__(512) k_project_banked(
 // Deterministic projection (opt-in): every BCOO block writes its level-0
 // target directly and, for each coarse level where its node pair shares a
 // bank, one (entry, block) record; a stable radix sort by entry groups the
-// records in block order and one thread per entry sums its group
-// sequentially.  Same result on every run, for any bank size or numbering.
+// records in block order, and a fixed-order segmented sum (tiles of
+// kProjTile records, see below) adds each group.  Same result on every run,
+// for any bank size or numbering.
 __global__ void k_proj_records(int K, const int* __restrict__ rows, const int* __restrict__ cols,
                                const double* __restrict__ vals, int off, int n, int B, int L,
                                const int* __restrict__ node_of, double* __restrict__ bank_upper, unsigned sentinel,
@@ -493,27 +494,58 @@ __global__ void k_proj_records(int K, const int* __restrict__ rows, const int* _
     for (int e = 0; e < 9; ++e) dst[e] += blk[e];
 }
 
-__global__ void k_proj_sum(long long m, const unsigned* __restrict__ key, const unsigned* __restrict__ rec,
-                           const int* __restrict__ rows, const int* __restrict__ cols, const double* __restrict__ vals,
-                           int off, int n, int B, const int* __restrict__ node_of, unsigned sentinel,
-                           double* __restrict__ bank_upper) {
+// contribution (9 doubles) of every sorted record
+__global__ void k_proj_contrib(long long m, const unsigned* __restrict__ key, const unsigned* __restrict__ rec,
+                               const int* __restrict__ rows, const int* __restrict__ cols,
+                               const double* __restrict__ vals, int off, int n, int B, const int* __restrict__ node_of,
+                               unsigned sentinel, double* __restrict__ contrib) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || key[i] == sentinel) return;
+    const unsigned q = rec[i];
+    const int k = (int)(q >> 3), l = (int)(q & 7u);
+    const int r = rows[k] - off, c = cols[k] - off;
+    const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
+    double H[9], blk[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
+    (void)project_block(B, I % B, J % B, r == c, H, blk);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) contrib[9 * (size_t)i + e] = blk[e];
+}
+
+// Fixed-order segmented sum: records are cut into tiles of kProjTile; the
+// first record of each (entry, tile) piece sums the piece in record order,
+// then the first record of each entry sums its pieces in tile order.
+constexpr int kProjTile = 256;
+__global__ void k_proj_pieces(long long m, const unsigned* __restrict__ key, unsigned sentinel,
+                              double* __restrict__ contrib) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const unsigned kk = key[i];
-    if (kk == sentinel || (i > 0 && key[i - 1] == kk)) return;  // one thread per entry: its group's head
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (long long j = i; j < m && key[j] == kk; ++j) {
-        const unsigned q = rec[j];
-        const int k = (int)(q >> 3), l = (int)(q & 7u);
-        const int r = rows[k] - off, c = cols[k] - off;
-        const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
-        double H[9], blk[9];
+    if (kk == sentinel || (i % kProjTile != 0 && key[i - 1] == kk)) return;
+    const long long tend = min(m, (i / kProjTile + 1) * kProjTile);
+    double acc[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
-        (void)project_block(B, I % B, J % B, r == c, H, blk);
+    for (int e = 0; e < 9; ++e) acc[e] = contrib[9 * (size_t)i + e];
+    for (long long j = i + 1; j < tend && key[j] == kk; ++j)
 #pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += blk[e];
-    }
+        for (int e = 0; e < 9; ++e) acc[e] += contrib[9 * (size_t)j + e];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) contrib[9 * (size_t)i + e] = acc[e];  // the piece sum, at the piece head
+}
+
+__global__ void k_proj_entries(long long m, const unsigned* __restrict__ key, unsigned sentinel,
+                               const double* __restrict__ contrib, double* __restrict__ bank_upper) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const unsigned kk = key[i];
+    if (kk == sentinel || (i > 0 && key[i - 1] == kk)) return;
+    double acc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = contrib[9 * (size_t)i + e];
+    for (long long j = (i / kProjTile + 1) * kProjTile; j < m && key[j] == kk; j += kProjTile)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += contrib[9 * (size_t)j + e];
     double* dst = bank_upper + (size_t)kk * 9;
 #pragma unroll
     for (int e = 0; e < 9; ++e) dst[e] = acc[e];
@@ -1153,10 +1185,16 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
             SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
             void* tmp = m.proj_sort_tmp.reserve(bytes + 1);
             SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
-            hipLaunchKernelGGL(k_proj_sum, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec,
-                               (const unsigned*)(key + mrec), (const unsigned*)(rec + mrec), (const int*)c.t_rows,
-                               (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, B,
-                               (const int*)m.node_of.p, sentinel, m.bank_upper.p);
+            double* contrib = m.proj_contrib.reserve(9 * (size_t)mrec);
+            const unsigned* skey = key + mrec;
+            hipLaunchKernelGGL(k_proj_contrib, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec, skey,
+                               (const unsigned*)(rec + mrec), (const int*)c.t_rows, (const int*)c.t_cols,
+                               (const double*)c.t_vals, sub_offset, n, B, (const int*)m.node_of.p, sentinel, contrib);
+            SGIPC_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_proj_pieces, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec, skey, sentinel, contrib);
+            SGIPC_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_proj_entries, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec, skey, sentinel,
+                               (const double*)contrib, m.bank_upper.p);
             SGIPC_LAUNCH_CHECK();
         } else {
             hipLaunchKernelGGL(k_proj_records, dim3(div_up(c.K, 256)), dim3(256), 0, s, c.K, (const int*)c.t_rows,

@@ -8,9 +8,9 @@
 //
 // Layout: block rows are cut into chunks that never straddle a scene (1024/G
 // rows for the SpMV with G lanes per row); one workgroup per chunk writes one
-// partial per dot product, and a single-workgroup scalar kernel sums each
-// scene's partials in chunk order (deterministic, no atomics) into per-scene
-// alpha / beta / stop words.  A stopped scene's rows are left untouched by
+// partial per dot product, and a scalar kernel (one wave per scene) sums
+// each scene's partials in chunk order (deterministic, no FP atomics) into
+// per-scene alpha / beta / stop words.  A stopped scene's rows are left untouched by
 // every later kernel.  The iteration sequence
 //   SpMV (Ap, p.Ap) -> alpha -> x, r update (r.r) -> MAS restrict + bank solve
 //   -> collect (z, r.z) -> beta + stop tests -> p update
@@ -29,7 +29,7 @@ namespace sgipc {
 namespace {
 
 constexpr int kBT = 256;       // threads of the row-wise batched kernels
-constexpr int kScalT = 1024;   // the scalar kernel: one workgroup, one wave per scene at a time
+constexpr int kScalT = 1024;   // the scalar kernel: one wave per scene, 16 scenes per workgroup
 
 enum BPhase { B_INIT_BB = 0, B_INIT_RZ, B_ALPHA, B_BETA, B_SEGSUM };
 
@@ -157,7 +157,14 @@ struct BScal {
     int *stop, *iters, *noconv;
 };
 
-// Per-scene scalars; one workgroup, each wave takes scenes w, w + 16, ...
+// extra control words of the batched solver (after the shared C_* words)
+constexpr int kCActive = C_NUM;      // active scenes counted by the workgroups of a scalar kernel
+constexpr int kCTicket = C_NUM + 1;  // workgroups of that kernel done
+
+// Per-scene scalars: one wave per scene, 16 scenes per workgroup.  The
+// phases that decide the stop (INIT_RZ, BETA) count active scenes per
+// workgroup; the last workgroup to finish sets the stop word and advances
+// the iteration counter (every workgroup has read both before it finishes).
 __global__ void __launch_bounds__(kScalT) k_bscalars(int phase, int S, const int* __restrict__ scene_chunk,
                                                      const double* __restrict__ pa, const double* __restrict__ pb,
                                                      BScal q, int rule, int* __restrict__ ctrl) {
@@ -166,7 +173,7 @@ __global__ void __launch_bounds__(kScalT) k_bscalars(int phase, int S, const int
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int k = ctrl[C_ITER];
     int active = 0;
-    for (int s = w; s < S; s += kScalT / 64) {
+    for (int s = blockIdx.x * (kScalT / 64) + w; s < S && s < (blockIdx.x + 1) * (kScalT / 64); s += kScalT / 64) {
         const int lo = scene_chunk[s], hi = scene_chunk[s + 1];
         if (phase == B_SEGSUM) {
             double v = wave_sum_partials(pa, lo, hi);
@@ -227,13 +234,21 @@ __global__ void __launch_bounds__(kScalT) k_bscalars(int phase, int S, const int
     if (threadIdx.x == 0) {
         int tot = 0;
         for (int i = 0; i < kScalT / 64; ++i) tot += act[i];
-        if (phase == B_INIT_RZ) {
-            ctrl[C_ITER] = 1;
-            ctrl[C_STOP] = tot == 0 ? 1 : 0;
-            ctrl[C_NOCONV] = 0;
-        } else {
-            ctrl[C_ITER] = k + 1;
-            if (tot == 0) ctrl[C_STOP] = 1;
+        if (tot) atomicAdd(ctrl + kCActive, tot);
+        __threadfence();
+        if (atomicAdd(ctrl + kCTicket, 1) == (int)gridDim.x - 1) {  // last workgroup
+            __threadfence();
+            const int all = atomicAdd(ctrl + kCActive, 0);
+            ctrl[kCActive] = 0;
+            ctrl[kCTicket] = 0;
+            if (phase == B_INIT_RZ) {
+                ctrl[C_ITER] = 1;
+                ctrl[C_STOP] = all == 0 ? 1 : 0;
+                ctrl[C_NOCONV] = 0;
+            } else {
+                ctrl[C_ITER] = k + 1;
+                if (all == 0) ctrl[C_STOP] = 1;
+            }
         }
     }
 }
@@ -243,6 +258,8 @@ __global__ void k_bctrl_init(int* ctrl, int maxit) {
     ctrl[C_ITER] = 1;
     ctrl[C_MAXIT] = maxit;
     ctrl[C_NOCONV] = 0;
+    ctrl[kCActive] = 0;
+    ctrl[kCTicket] = 0;
 }
 
 template <typename TD, typename TS>
@@ -297,8 +314,8 @@ struct BEngine {
     T* Zm() { return fp32mas ? (T*)(void*)c.mas.Z32.p : (T*)(void*)c.mas.Z64.p; }
 
     void scalars(int phase, const double* pa, const double* pb) {
-        hipLaunchKernelGGL(k_bscalars, dim3(1), dim3(kScalT), 0, c.stream, phase, S, (const int*)w.scene_chunk.p, pa,
-                           pb, scal(), rule, w.ctrl.p);
+        hipLaunchKernelGGL(k_bscalars, dim3(div_up(S, kScalT / 64)), dim3(kScalT), 0, c.stream, phase, S,
+                           (const int*)w.scene_chunk.p, pa, pb, scal(), rule, w.ctrl.p);
         SGIPC_LAUNCH_CHECK();
     }
 
@@ -415,7 +432,8 @@ struct BEngine {
 
 // per-scene sums of a chunk-partial array, to the host
 void seg_to_host(Ctx& c, BatchWork& w, int S, const double* part, std::vector<double>& out) {
-    hipLaunchKernelGGL(k_bscalars, dim3(1), dim3(kScalT), 0, c.stream, (int)B_SEGSUM, S, (const int*)w.scene_chunk.p,
+    hipLaunchKernelGGL(k_bscalars, dim3(div_up(S, kScalT / 64)), dim3(kScalT), 0, c.stream, (int)B_SEGSUM, S,
+                       (const int*)w.scene_chunk.p,
                        part, (const double*)nullptr,
                        BScal{w.bb.p, w.rz.p, w.rz0.p, w.alpha.p, w.beta.p, w.tol.p, w.segout.p, w.stop.p, w.iters.p,
                              w.noconv.p},

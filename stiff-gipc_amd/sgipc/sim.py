@@ -255,10 +255,13 @@ def final_positions(sim, log_E, v0, steps):
     return xT.reshape(-1, 3)
 
 
-def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0):
+def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, progress=None, richardson_above=None):
     """Central differences of L along n_dir seeded unit directions in
     (ln E, free v0) space against the adjoint gradient.  Returns a list of
-    (fd, adjoint, relative error) and the loss."""
+    (fd, adjoint, relative error) and the loss.  With richardson_above set,
+    directions whose error exceeds it are also differenced with h/2 and the
+    Richardson value (4 fd(h/2) - fd(h)) / 3, which cancels the O(h^2)
+    truncation term, is compared too: entries then carry (fd_r, rel_r)."""
     L, gE, gv = loss_and_gradient(sim, log_E, v0, target, steps, ckpt)
     rng = np.random.default_rng(seed)
     free = sim.free.reshape(-1, 3)
@@ -269,12 +272,21 @@ def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0):
         nrm = math.sqrt(float((dE * dE).sum() + (dv * dv).sum()))
         dE /= nrm
         dv /= nrm
-        xp = final_positions(sim, log_E + h * dE, v0 + h * dv, steps)
-        xm = final_positions(sim, log_E - h * dE, v0 - h * dv, steps)
-        # (L+ - L-) / 2h with L = |x_T - target|^2, summed as
-        # (x+ - x-) . (x+ + x- - 2 target) so that no two O(L) numbers are
-        # subtracted (their round-off would swamp derivatives far below L)
-        fd = float(((xp - xm) * (xp + xm - 2.0 * target)).sum()) / (2 * h)
+        def central(hh):
+            xp = final_positions(sim, log_E + hh * dE, v0 + hh * dv, steps)
+            xm = final_positions(sim, log_E - hh * dE, v0 - hh * dv, steps)
+            # (L+ - L-) / 2h with L = |x_T - target|^2, summed as
+            # (x+ - x-) . (x+ + x- - 2 target) so that no two O(L) numbers
+            # are subtracted (their round-off would swamp small derivatives)
+            return float(((xp - xm) * (xp + xm - 2.0 * target)).sum()) / (2 * hh)
+        fd = central(h)
         ad = float((gE * dE).sum() + (gv * dv).sum())
-        out.append((fd, ad, abs(fd - ad) / max(abs(ad), 1e-300)))
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
+        if richardson_above is not None and rel > richardson_above:
+            fr = (4.0 * central(0.5 * h) - fd) / 3.0
+            out.append((fd, ad, rel, fr, abs(fr - ad) / max(abs(ad), 1e-300)))
+        else:
+            out.append((fd, ad, rel))
+        if progress:
+            progress(len(out), *out[-1][:3])
     return out, L
