@@ -32,6 +32,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=40)
+    ap.add_argument("--bank", type=int, choices=[16, 32], default=16, help="MAS subdomain size (vertices)")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["contact", "cantilever_adjoint", "batched"], default="contact",
@@ -42,6 +43,9 @@ def parse():
     ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
     ap.add_argument("--fd-dirs", type=int, default=32, help="cantilever_adjoint: FD directions")
     ap.add_argument("--rollout-steps", type=int, default=20, help="cantilever_adjoint: time steps")
+    ap.add_argument("--newton-tol", type=float, default=1e-13,
+                    help="cantilever_adjoint: Newton stop on |dx|_inf (and round-off stall below 1e3 x this)")
+    ap.add_argument("--pcg-tol", type=float, default=1e-12, help="cantilever_adjoint: PCG true rel. residual")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -202,7 +206,7 @@ def main():
     from sgipc import scene, capi
     from sgipc.system import NewtonSystem
 
-    sc = scene.two_box_scene(a.cells)
+    sc = scene.two_box_scene(a.cells, bank=a.bank)
     pt, ee = scene.facing_contact_candidates(sc)
     sysm = NewtonSystem(sc, "cuda:0", precond=capi.PRECOND_MAS)
     sysm.setup_contacts(pt, ee)  # narrow phase once per Newton iterate, outside the timed hot path
@@ -263,7 +267,7 @@ def main():
             "contact_pairs": int(sysm.n_pairs),
             "upper_blocks": int(sysm.K),
             "precision": main_mode,
-            "preconditioner": "MAS, 16-vertex banks, %d levels" % sysm.mas_levels(),
+            "preconditioner": "MAS, %d-vertex banks, %d levels" % (a.bank, sysm.mas_levels()),
             "stop": "true relative residual 1e-6",
             "parallelism": "single GPU",
         },
@@ -300,7 +304,8 @@ def cantilever_adjoint(a):
     from sgipc import scene
     from sgipc.sim import GroundSim, loss_and_gradient, fd_check
     sc = scene.cantilever_scene(60, 12, 12, seed=4)
-    sim = GroundSim(sc, "cuda:0")
+    sim = GroundSim(sc, "cuda:0", newton_tol=a.newton_tol, stall_tol=1e3 * a.newton_tol, pcg_tol=a.pcg_tol,
+                    adj_tol=a.pcg_tol)
     m = sc.meta
     dev = "cuda:0"
     logE = torch.tensor(m["log_E"], dtype=torch.float64, device=dev)
@@ -345,6 +350,7 @@ def cantilever_adjoint(a):
                                "barrier, %d implicit steps, dt=%g, ln E ~ N(ln 1e5, 0.2^2) per tet" %
                                (sc.n, sc.T, steps, sc.dt),
                    "checkpoint_every": 5, "fd_h": 1e-6, "fd_directions": a.fd_dirs,
+                   "newton_tol": a.newton_tol, "pcg_tol": a.pcg_tol,
                    "parameters": "per-tet ln E (%d) and free initial velocities" % sc.T},
         "loss": L,
         "grad_norm_logE": float(gE.norm()),

@@ -33,7 +33,7 @@ from .system import NewtonSystem, _p
 
 class GroundSim:
     def __init__(self, scene, device="cuda:0", newton_tol=1e-11, max_newton=60, pcg_tol=1e-10, adj_tol=1e-11,
-                 slack=0.8, max_pcg=20000):
+                 slack=0.8, max_pcg=20000, stall_tol=1e-9):
         m = scene.meta
         self.sc = scene
         self.surf_np = np.ascontiguousarray(m["surf"], np.uint32)
@@ -50,7 +50,7 @@ class GroundSim:
         self.dhat, self.kappa = float(scene.dhat), float(scene.kappa)
         self.c_mu, self.c_lam = lame_coefficients(m["nu"])
         self.grav = torch.tensor([0.0, m["gravity"], 0.0], dtype=torch.float64, device=self.dev)
-        self.newton_tol, self.max_newton = newton_tol, max_newton
+        self.newton_tol, self.max_newton, self.stall_tol = newton_tol, max_newton, stall_tol
         self.pcg_tol, self.adj_tol, self.slack, self.max_pcg = pcg_tol, adj_tol, slack, max_pcg
         i32 = dict(dtype=torch.int32, device=self.dev)
         self.surf = torch.tensor(self.surf_np.astype(np.int32), **i32)     # uint32 ids, int32 storage
@@ -152,7 +152,7 @@ class GroundSim:
                     self.stats["line_search"] += 1
             else:
                 s.x.sub_(alpha * s.dx)
-            if alpha == 1.0 and (dxn <= self.newton_tol or (dxn < 1e-9 and dxn > 0.5 * prev)):
+            if alpha == 1.0 and (dxn <= self.newton_tol or (dxn < self.stall_tol and dxn > 0.5 * prev)):
                 break                       # converged, or stalled at round-off
             prev = dxn
         else:
