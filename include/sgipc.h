@@ -178,6 +178,24 @@ int sgipc_broad_phase(sgipc_ctx* ctx, int n_surf_verts, const uint32_t* d_surf_v
                       const uint32_t* d_faces, int n_edges, const uint32_t* d_edges, const double* d_x,
                       const int* d_btype, const int* d_body_id, double dhat, int* d_pt_out, int pt_capacity,
                       int* h_n_pt, int* d_ee_out, int ee_capacity, int* h_n_ee);
+/* The same queries with every box swept from x to x - alpha * d_dir (the
+ * search direction of a line search): the candidates of the CCD step bound;
+ * StiffGIPC/mlbvh.cu _calcLeafBvs_ccd (:1172-1204), _selfQuery_vf_ccd
+ * (:1408-1507), _selfQuery_ee_ccd (:1634-1760), built by GIPC::buildFullCP
+ * (StiffGIPC/GIPC.cu:9085). */
+int sgipc_broad_phase_ccd(sgipc_ctx* ctx, int n_surf_verts, const uint32_t* d_surf_verts, int n_faces,
+                          const uint32_t* d_faces, int n_edges, const uint32_t* d_edges, const double* d_x,
+                          const double* d_dir, double alpha, const int* d_btype, const int* d_body_id, double dhat,
+                          int* d_pt_out, int pt_capacity, int* h_n_pt, int* d_ee_out, int ee_capacity, int* h_n_ee);
+/* Largest feasible step alpha <= 1 along -d_dir for the candidate
+ * point-triangle (p, t0, t1, t2) and edge-edge (a0, a1, b0, b1) quadruples:
+ * additive CCD per candidate with eta = 1 - slackness and zero thickness,
+ * minimum over candidates (order-independent); ACCD.cu point_triangle_ccd
+ * (:456-534), edge_edge_ccd (:373-454) as used by
+ * GIPC::self_largestFeasibleStepSize (StiffGIPC/GIPC.cu:8880-8915,
+ * _reduct_min_selfTimeStep_to_double :7052-7150).  Synchronises. */
+int sgipc_ccd_step_bound(sgipc_ctx* ctx, int n_pt, const int* d_pt, int n_ee, const int* d_ee, const double* d_x,
+                         const double* d_dir, double slackness, double* h_alpha);
 /* Narrow phase: classify candidate point-triangle (p, t0, t1, t2) and
  * edge-edge (a0, a1, b0, b1) vertex quadruples into the reference's int4 pair
  * encoding, keeping pairs with squared distance < dhat; PT candidates first,

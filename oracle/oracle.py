@@ -76,6 +76,10 @@ def lib():
         L.or_classify.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double, i32p, i32p, i32p]
         L.or_classify.restype = C.c_int
         L.or_set_fem_projection.argtypes = [C.c_int]
+        L.or_accd.argtypes = [C.c_int, f64p, f64p, C.c_double]
+        L.or_accd.restype = C.c_double
+        L.or_ccd_step_bound.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double]
+        L.or_ccd_step_bound.restype = C.c_double
         L.or_ground_detect.argtypes = [C.c_int, u32p, f64p, f64p, C.c_double, C.c_double, u32p]
         L.or_ground_detect.restype = C.c_int
         L.or_ground_assemble.argtypes = [C.c_int, u32p, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_void_p,
@@ -355,3 +359,14 @@ def fem_param_adjoint(x, tets, DmInv, vol, dt, adj, btype=None):
                                _c(DmInv, np.float64).reshape(-1), _c(vol, np.float64),
                                dt, None if bt is None else bt.ctypes.data, _c(adj, np.float64).reshape(-1), dmu, dlam)
     return dmu[:T], dlam[:T]
+
+
+def accd(pt, v, u, eta):
+    """time of impact (fraction of the step) of 4 vertices v moving by u"""
+    return lib().or_accd(int(pt), _c(v, np.float64).reshape(12), _c(u, np.float64).reshape(12), eta)
+
+
+def ccd_step_bound(pt, ee, x, d, slack):
+    pt = _c(pt, np.int32).reshape(-1, 4); ee = _c(ee, np.int32).reshape(-1, 4)
+    return lib().or_ccd_step_bound(pt.shape[0], pt.reshape(-1), ee.shape[0], ee.reshape(-1),
+                                   _c(x, np.float64).reshape(-1), _c(d, np.float64).reshape(-1), slack)

@@ -19,6 +19,8 @@
  *   or_ground_*      energy closed form; gradient vs central FD of the
  *                    energy; Hessian vs central FD of the gradient
  *   or_fem_param_adjoint  central FD of the assembled gradient in mu, lam
+ *   or_accd          head-on approach: (1 - eta) of the exact time of impact;
+ *                    no penetration over [0, toc] on random motions
  *   exact SNK1 mode  (projection off) Hessian vs central FD of the gradient
  */
 #include "sgipc_oracle.h"

@@ -48,6 +48,9 @@ void or_fem_precompute(int n, int T, const double* xrest, const uint32_t* tets,
 /* FEM triplets + shape gradient (S/femEnergy.cu:1719-1857 + boundary mask
  * S/GIPC.cu:10295-10313).  grad (3n) is accumulated (+=) in tet order.
  * Writes 10 triplets per tet at vals+90*t, rows/cols+10*t. */
+double or_accd(int pt, const double* v, const double* u, double eta);
+double or_ccd_step_bound(int npt, const int* pt, int nee, const int* ee, const double* x, const double* dir,
+                         double slack);
 extern int or_fem_project;
 void or_set_fem_projection(int on);
 /* ground plane n.x = o (sgipc_oracle_ground.c) */

@@ -376,3 +376,82 @@ int or_classify(int npt, const int* pt, int nee, const int* ee, const double* x,
 #undef EMIT
     return np;
 }
+
+/* ======================================================================== */
+/* Additive CCD (Li et al. 2021, "Codimensional incremental potential       */
+/* contact"): ACCD.cu point_triangle_ccd :456-534, edge_edge_ccd :373-454,   */
+/* thickness 0; step bound as GIPC::self_largestFeasibleStepSize            */
+/* (GIPC.cu:8880-8915) with eta = 1 - slackness.                            */
+/* ======================================================================== */
+static double ccd_d2_pt(const double* p, const double* a, const double* b, const double* c) {
+    switch (dtype_pt(p, a, b, c)) {
+        case 0: return d_pp(p, a, NULL);
+        case 1: return d_pp(p, b, NULL);
+        case 2: return d_pp(p, c, NULL);
+        case 3: return d_pe(p, a, b, NULL);
+        case 4: return d_pe(p, b, c, NULL);
+        case 5: return d_pe(p, c, a, NULL);
+        default: return d_pt(p, a, b, c, NULL);
+    }
+}
+static double ccd_d2_ee(const double* a0, const double* a1, const double* b0, const double* b1) {
+    double d;
+    switch (dtype_ee(a0, a1, b0, b1)) {
+        case 0: d = d_pp(a0, b0, NULL); break;
+        case 1: d = d_pp(a0, b1, NULL); break;
+        case 2: d = d_pe(a0, b0, b1, NULL); break;
+        case 3: d = d_pp(a1, b0, NULL); break;
+        case 4: d = d_pp(a1, b1, NULL); break;
+        case 5: d = d_pe(a1, b0, b1, NULL); break;
+        case 6: d = d_pe(b0, a0, a1, NULL); break;
+        case 7: d = d_pe(b1, a0, a1, NULL); break;
+        default: d = d_ee(a0, a1, b0, b1, NULL); break;
+    }
+    if (d <= 0.0) {
+        double m = d_pp(a0, b0, NULL), t;
+        t = d_pp(a0, b1, NULL); if (t < m) m = t;
+        t = d_pp(a1, b0, NULL); if (t < m) m = t;
+        t = d_pp(a1, b1, NULL); if (t < m) m = t;
+        d = m;
+    }
+    return d;
+}
+
+/* v: 4 vertices (12), u: their displacements (12); pt != 0: point-triangle */
+double or_accd(int pt, const double* v_in, const double* u_in, double eta) {
+    double v[12], u[12], mov[3];
+    for (int k = 0; k < 3; ++k) mov[k] = -0.25 * (u_in[k] + u_in[3 + k] + u_in[6 + k] + u_in[9 + k]);
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 3; ++k) { v[3 * i + k] = v_in[3 * i + k]; u[3 * i + k] = u_in[3 * i + k] + mov[k]; }
+    double n0 = dot3(u, u), n1 = dot3(u + 3, u + 3), n2 = dot3(u + 6, u + 6), n3 = dot3(u + 9, u + 9);
+    double maxd;
+    if (pt) maxd = sqrt(n0) + sqrt(fmax(n1, fmax(n2, n3)));
+    else maxd = sqrt(fmax(n0, n1)) + sqrt(fmax(n2, n3));
+    if (maxd == 0.0) return 1.0;
+    double dd = pt ? ccd_d2_pt(v, v + 3, v + 6, v + 9) : ccd_d2_ee(v, v + 3, v + 6, v + 9);
+    double dist = sqrt(dd), gap = eta * dd / dist, toc = 0.0;
+    for (int it = 0; it < 50000; ++it) {
+        double lb = (1.0 - eta) * dd / (dist * maxd);
+        for (int i = 0; i < 12; ++i) v[i] += lb * u[i];
+        dd = pt ? ccd_d2_pt(v, v + 3, v + 6, v + 9) : ccd_d2_ee(v, v + 3, v + 6, v + 9);
+        dist = sqrt(dd);
+        if (toc != 0.0 && dd / dist < gap) return toc;
+        toc += lb;
+        if (toc > 1.0) return 1.0;
+    }
+    return toc;
+}
+
+/* min over candidates of the time of impact along -dir, at most 1 */
+double or_ccd_step_bound(int npt, const int* pt, int nee, const int* ee, const double* x, const double* dir,
+                         double slack) {
+    double t = 1.0, v[12], u[12];
+    for (int i = 0; i < npt + nee; ++i) {
+        const int* q = i < npt ? pt + 4 * i : ee + 4 * (i - npt);
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 3; ++k) { v[3 * a + k] = x[3 * (size_t)q[a] + k]; u[3 * a + k] = -dir[3 * (size_t)q[a] + k]; }
+        double s = or_accd(i < npt, v, u, 1.0 - slack);
+        if (s < t) t = s;
+    }
+    return t;
+}

@@ -401,6 +401,122 @@ __global__ void k_class_scatter(int m, const int4* __restrict__ enc, const int* 
     mat_index[o] = pos[(size_t)(k - 1) * (m + 1) + i];
 }
 
+// ---- additive continuous collision detection ---------------------------------
+// Squared distance of the closest feature pair (the classification above).
+__device__ inline double dist2_pt_any(V3 p, V3 a, V3 b, V3 c) {
+    switch (dtype_pt(p, a, b, c)) {
+        case 0: return d_pp(p, a, nullptr);
+        case 1: return d_pp(p, b, nullptr);
+        case 2: return d_pp(p, c, nullptr);
+        case 3: return d_pe(p, a, b, nullptr);
+        case 4: return d_pe(p, b, c, nullptr);
+        case 5: return d_pe(p, c, a, nullptr);
+        default: return d_pt(p, a, b, c, nullptr);
+    }
+}
+__device__ inline double dist2_ee_any(V3 a0, V3 a1, V3 b0, V3 b1) {
+    switch (dtype_ee(a0, a1, b0, b1)) {
+        case 0: return d_pp(a0, b0, nullptr);
+        case 1: return d_pp(a0, b1, nullptr);
+        case 2: return d_pe(a0, b0, b1, nullptr);
+        case 3: return d_pp(a1, b0, nullptr);
+        case 4: return d_pp(a1, b1, nullptr);
+        case 5: return d_pe(a1, b0, b1, nullptr);
+        case 6: return d_pe(b0, a0, a1, nullptr);
+        case 7: return d_pe(b1, a0, a1, nullptr);
+        default: return d_ee(a0, a1, b0, b1, nullptr);
+    }
+}
+__device__ inline double dist2_ee_safe(V3 a0, V3 a1, V3 b0, V3 b1) {
+    double d = dist2_ee_any(a0, a1, b0, b1);
+    if (d <= 0.0)  // touching or degenerate: the closest endpoint pair
+        d = fmin(fmin(d_pp(a0, b0, nullptr), d_pp(a0, b1, nullptr)), fmin(d_pp(a1, b0, nullptr), d_pp(a1, b1, nullptr)));
+    return d;
+}
+constexpr int kAccdMaxSteps = 50000;
+
+// Additive CCD (Li et al. 2021, "Codimensional incremental potential
+// contact"): advance by conservative steps (1 - eta) d / (max relative
+// displacement) until the distance falls below eta of its start; time of
+// impact as a fraction of the step, capped at 1.  Four vertices v[0..3]
+// moving by u[0..3]; pt selects point-triangle (else edge-edge).
+// Reference: ACCD.cu point_triangle_ccd :456-534, edge_edge_ccd :373-454,
+// thickness 0 as GIPC::self_largestFeasibleStepSize uses them.
+__device__ double accd(bool pt, V3 v[4], V3 u[4], double eta) {
+    const V3 mov = -0.25 * (u[0] + u[1] + u[2] + u[3]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = u[i] + mov;
+    double maxd;
+    if (pt) {
+        maxd = sqrt(dot(u[0], u[0])) + sqrt(fmax(dot(u[1], u[1]), fmax(dot(u[2], u[2]), dot(u[3], u[3]))));
+    } else {
+        maxd = sqrt(fmax(dot(u[0], u[0]), dot(u[1], u[1]))) + sqrt(fmax(dot(u[2], u[2]), dot(u[3], u[3])));
+    }
+    if (maxd == 0.0) return 1.0;
+    auto d2 = [&](void) {
+        return pt ? dist2_pt_any(v[0], v[1], v[2], v[3]) : dist2_ee_safe(v[0], v[1], v[2], v[3]);
+    };
+    double dd = d2();
+    double dist = sqrt(dd);
+    const double gap = eta * dd / dist;
+    double toc = 0.0;
+    for (int it = 0; it < kAccdMaxSteps; ++it) {
+        const double lb = (1.0 - eta) * dd / (dist * maxd);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] + lb * u[i];
+        dd = d2();
+        dist = sqrt(dd);
+        if (toc != 0.0 && dd / dist < gap) return toc;
+        toc += lb;
+        if (toc > 1.0) return 1.0;
+    }
+    return toc;
+}
+
+// per candidate time of impact along -dir; block minima (1 if none)
+__global__ void __launch_bounds__(256) k_ccd(int npt, const int4* __restrict__ pt, int nee, const int4* __restrict__ ee,
+                                             const double* __restrict__ x, const double* __restrict__ dir, double eta,
+                                             double* __restrict__ part) {
+    __shared__ double sh[256 / 64];
+    double t = 1.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npt + nee; i += gridDim.x * blockDim.x) {
+        const bool isp = i < npt;
+        const int4 q = isp ? pt[i] : ee[i - npt];
+        const int id[4] = {q.x, q.y, q.z, q.w};
+        V3 v[4], u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = ld(x, id[k]);
+            u[k] = -1.0 * ld(dir, id[k]);
+        }
+        t = fmin(t, accd(isp, v, u, eta));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t = fmin(t, __shfl_xor(t, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sh[0];
+        for (int w = 1; w < 4; ++w) m = fmin(m, sh[w]);
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_min_total(const double* __restrict__ part, int np, double* __restrict__ out) {
+    __shared__ double sh[256 / 64];
+    double t = 1.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) t = fmin(t, part[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t = fmin(t, __shfl_xor(t, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sh[0];
+        for (int w = 1; w < 4; ++w) m = fmin(m, sh[w]);
+        *out = m;
+    }
+}
+
 // fb[v] += the sorted records of vertex v, summed in pair order (one head
 // thread per vertex)
 __global__ void k_grad_gather(long long m, const unsigned* __restrict__ key, const unsigned* __restrict__ rec,
@@ -486,6 +602,25 @@ int classify_contacts(Ctx& c, int npt, const int* pt, int nee, const int* ee, co
     counts[1] = hcnt[2];
     counts[2] = hcnt[3];
     return hcnt[0];
+}
+
+double ccd_step_bound(Ctx& c, int npt, const int* pt, int nee, const int* ee, const double* x, const double* dir,
+                      double slack) {
+    if (!(slack > 0.0 && slack < 1.0)) throw arg_error("ccd_step_bound: slackness must be in (0, 1)");
+    const int m = npt + nee;
+    if (m <= 0) return 1.0;
+    double* part = energy_partials(c);
+    const int g = energy_grid(m);
+    hipLaunchKernelGGL(k_ccd, dim3(g), dim3(256), 0, c.stream, npt, (const int4*)pt, nee, (const int4*)ee, x, dir,
+                       1.0 - slack, part);
+    SGIPC_LAUNCH_CHECK();
+    double* out = part + g;
+    hipLaunchKernelGGL(k_min_total, dim3(1), dim3(256), 0, c.stream, (const double*)part, g, out);
+    SGIPC_LAUNCH_CHECK();
+    double h = 1.0;
+    SGIPC_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    return h;
 }
 
 double barrier_energy(Ctx& c, int np, const int* pairs, const double* x, const double* xr, double dhat,

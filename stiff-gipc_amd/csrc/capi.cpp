@@ -273,6 +273,41 @@ int sgipc_broad_phase(sgipc_ctx* c, int n_surf_verts, const uint32_t* surf_verts
     });
 }
 
+int sgipc_broad_phase_ccd(sgipc_ctx* c, int n_surf_verts, const uint32_t* surf_verts, int n_faces,
+                          const uint32_t* faces, int n_edges, const uint32_t* edges, const double* x, const double* dir,
+                          double alpha, const int* btype, const int* body_id, double dhat, int* pt_out,
+                          int pt_capacity, int* n_pt, int* ee_out, int ee_capacity, int* n_ee) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!dir || !(alpha >= 0.0)) throw sgipc::arg_error("broad_phase_ccd: a direction and alpha >= 0 are required");
+        if (n_surf_verts < 0 || n_faces < 0 || n_edges < 0 || pt_capacity < 0 || ee_capacity < 0 || !n_pt || !n_ee ||
+            !x || (n_surf_verts > 0 && !surf_verts) || (n_faces > 0 && !faces) || (n_edges > 0 && !edges) ||
+            (pt_capacity > 0 && !pt_out) || (ee_capacity > 0 && !ee_out))
+            throw sgipc::arg_error("broad_phase_ccd: invalid argument");
+        set_device(c);
+        int a = 0, b = 0;
+        sgipc::broad_phase(*c, n_surf_verts, surf_verts, n_faces, faces, n_edges, edges, x, btype, body_id, dhat,
+                           pt_out, pt_capacity, a, ee_out, ee_capacity, b, dir, alpha);
+        *n_pt = a;
+        *n_ee = b;
+        if (a > pt_capacity || b > ee_capacity)
+            throw sgipc::capacity_error("broad_phase_ccd: " + std::to_string(a) + " point-triangle / " +
+                                        std::to_string(b) + " edge-edge candidates exceed the capacities (" +
+                                        std::to_string(pt_capacity) + " / " + std::to_string(ee_capacity) + ")");
+    });
+}
+
+int sgipc_ccd_step_bound(sgipc_ctx* c, int n_pt, const int* pt, int n_ee, const int* ee, const double* x,
+                         const double* dir, double slackness, double* alpha) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n_pt < 0 || n_ee < 0 || !alpha || !x || !dir || (n_pt > 0 && !pt) || (n_ee > 0 && !ee))
+            throw sgipc::arg_error("ccd_step_bound: invalid argument");
+        set_device(c);
+        *alpha = sgipc::ccd_step_bound(*c, n_pt, pt, n_ee, ee, x, dir, slackness);
+    });
+}
+
 int sgipc_set_deterministic(sgipc_ctx* c, int on) {
     CHECK_CTX(c);
     return guard(c, [&] { c->deterministic = on ? 1 : 0; });

@@ -282,10 +282,13 @@ void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, d
                        SolveStats* st);
 void batch_check_coupling(Ctx& c, const std::vector<int>& row_start);
 void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precision, int& group, int& per_graph);
+double ccd_step_bound(Ctx& c, int npt, const int* pt, int nee, const int* ee, const double* x, const double* dir,
+                      double slack);
 // candidate counts in n_pt / n_ee; quadruples written only when they fit the capacities
+// (dir non-null: boxes swept from x to x - alpha dir, the CCD candidates)
 void broad_phase(Ctx& c, int nsv, const unsigned* sv, int nf, const unsigned* faces, int ne, const unsigned* edges,
                  const double* x, const int* btype, const int* body, double dhat, int* pt_out, int pt_cap, int& n_pt,
-                 int* ee_out, int ee_cap, int& n_ee);
+                 int* ee_out, int ee_cap, int& n_ee, const double* dir = nullptr, double alpha = 0.0);
 void precond_apply(Ctx& c, const double* r, double* z);
 
 }  // namespace sgipc

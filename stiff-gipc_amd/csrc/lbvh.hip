@@ -2,7 +2,10 @@
 // over the surface edges, queried per surface vertex (point-triangle) and
 // per edge (edge-edge) for boxes within sqrt(dhat) of each other.  The
 // output is the candidate quadruples that the narrow phase
-// (sgipc_classify_contacts) classifies and encodes.
+// (sgipc_classify_contacts) classifies and encodes.  With a search
+// direction the boxes are swept from x to x - alpha dir (the reference's
+// _calcLeafBvs_ccd / _selfQuery_vf_ccd / _selfQuery_ee_ccd, mlbvh.cu
+// :1172-1204, :1408-1507, :1634-1760): the candidates of the CCD step bound.
 //
 // Reference: S/mlbvh.cu -- leaf boxes _calcLeafBvs (:1147-1170), scene box
 // _reduct_max_box (:1062-1145), Morton keys _calcMChash (:1206-1225, 10 bits
@@ -38,6 +41,18 @@ __device__ inline void grow(Box& b, const double* __restrict__ x, unsigned v) {
     b.lx = fmin(b.lx, px); b.ly = fmin(b.ly, py); b.lz = fmin(b.lz, pz);
     b.ux = fmax(b.ux, px); b.uy = fmax(b.uy, py); b.uz = fmax(b.uz, pz);
 }
+// the vertex at x and, when swept, at x - alpha dir
+__device__ inline void grow_swept(Box& b, const double* __restrict__ x, const double* __restrict__ dir, double alpha,
+                                  unsigned v) {
+    grow(b, x, v);
+    if (dir) {
+        double px = x[3 * (size_t)v] - alpha * dir[3 * (size_t)v];
+        double py = x[3 * (size_t)v + 1] - alpha * dir[3 * (size_t)v + 1];
+        double pz = x[3 * (size_t)v + 2] - alpha * dir[3 * (size_t)v + 2];
+        b.lx = fmin(b.lx, px); b.ly = fmin(b.ly, py); b.lz = fmin(b.lz, pz);
+        b.ux = fmax(b.ux, px); b.uy = fmax(b.uy, py); b.uz = fmax(b.uz, pz);
+    }
+}
 __device__ inline Box merge(const Box& a, const Box& b) {
     return Box{fmin(a.lx, b.lx), fmin(a.ly, b.ly), fmin(a.lz, b.lz), fmax(a.ux, b.ux), fmax(a.uy, b.uy),
                fmax(a.uz, b.uz)};
@@ -56,12 +71,12 @@ struct Node {
 
 template <int K>
 __global__ void k_leaf_boxes(int m, const unsigned* __restrict__ prim, const double* __restrict__ x,
-                             Box* __restrict__ box) {
+                             const double* __restrict__ dir, double alpha, Box* __restrict__ box) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     Box b = empty_box();
 #pragma unroll
-    for (int k = 0; k < K; ++k) grow(b, x, prim[K * (size_t)i + k]);
+    for (int k = 0; k < K; ++k) grow_swept(b, x, dir, alpha, prim[K * (size_t)i + k]);
     box[i] = b;
 }
 
@@ -216,14 +231,15 @@ __device__ inline bool other_body(const int* __restrict__ body, unsigned a, unsi
 // pass 0 counts hits per query, pass 1 writes them at the scanned offsets
 template <int PASS>
 __global__ void k_query_vf(int nq, const unsigned* __restrict__ sv, const unsigned* __restrict__ faces,
-                           const double* __restrict__ x, QueryCtx Q, const int* __restrict__ bt,
+                           const double* __restrict__ x, const double* __restrict__ dir, double alpha, QueryCtx Q,
+                           const int* __restrict__ bt,
                            const int* __restrict__ body, int* __restrict__ cnt, const int* __restrict__ off,
                            int4* __restrict__ out, int cap, int* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     const unsigned p = sv[i];
     Box q = empty_box();
-    grow(q, x, p);
+    grow_swept(q, x, dir, alpha, p);
     int k = 0;
     const int base = PASS ? off[i] : 0;
     bool ok = traverse(Q, q, [&](unsigned f) {
@@ -263,12 +279,13 @@ __global__ void k_query_ee(int ne, const unsigned* __restrict__ edges, const Box
 
 // Builds the tree over m primitives of K vertices into t (boxes, nodes).
 template <int K>
-static void lbvh_build(Ctx& c, LbvhTree& t, int m, const unsigned* prim, const double* x) {
+static void lbvh_build(Ctx& c, LbvhTree& t, int m, const unsigned* prim, const double* x, const double* dir,
+                       double alpha) {
     hipStream_t s = c.stream;
     t.m = m;
     if (m <= 0) return;
     Box* leaf = (Box*)t.leaf.reserve((size_t)m * sizeof(Box));
-    hipLaunchKernelGGL(k_leaf_boxes<K>, dim3(div_up(m, 256)), dim3(256), 0, s, m, prim, x, leaf);
+    hipLaunchKernelGGL(k_leaf_boxes<K>, dim3(div_up(m, 256)), dim3(256), 0, s, m, prim, x, dir, alpha, leaf);
     SGIPC_LAUNCH_CHECK();
     const int gb = std::max(1, std::min(256, (int)div_up(m, 256)));
     Box* part = (Box*)t.scene.reserve((size_t)(gb + 1) * sizeof(Box));
@@ -304,7 +321,7 @@ static void lbvh_build(Ctx& c, LbvhTree& t, int m, const unsigned* prim, const d
 
 void broad_phase(Ctx& c, int nsv, const unsigned* sv, int nf, const unsigned* faces, int ne, const unsigned* edges,
                  const double* x, const int* btype, const int* body, double dhat, int* pt_out, int pt_cap, int& n_pt,
-                 int* ee_out, int ee_cap, int& n_ee) {
+                 int* ee_out, int ee_cap, int& n_ee, const double* dir, double alpha) {
     if (!(dhat > 0.0)) throw arg_error("broad_phase: dhat must be positive");
     hipStream_t s = c.stream;
     const double gap = sqrt(dhat);
@@ -315,25 +332,25 @@ void broad_phase(Ctx& c, int nsv, const unsigned* sv, int nf, const unsigned* fa
     int hcount[2] = {0, 0};
     // point - triangle
     if (nsv > 0 && nf > 0) {
-        lbvh_build<3>(c, B.tri, nf, faces, x);
+        lbvh_build<3>(c, B.tri, nf, faces, x, dir, alpha);
         QueryCtx Q{(const Node*)B.tri.nodes.p, (const Box*)B.tri.nbox.p, nf, gap};
         int* cnt = B.cnt.reserve((size_t)nsv + 1);
         int* off = B.off.reserve((size_t)nsv + 1);
-        hipLaunchKernelGGL(k_query_vf<0>, dim3(div_up(nsv, 128)), dim3(128), 0, s, nsv, sv, faces, x, Q, btype, body,
-                           cnt, (const int*)nullptr, (int4*)nullptr, 0, err);
+        hipLaunchKernelGGL(k_query_vf<0>, dim3(div_up(nsv, 128)), dim3(128), 0, s, nsv, sv, faces, x, dir, alpha, Q,
+                           btype, body, cnt, (const int*)nullptr, (int4*)nullptr, 0, err);
         SGIPC_LAUNCH_CHECK();
         exclusive_scan(c, cnt, off, nsv, nullptr, s);
         SGIPC_HIP(hipMemcpyAsync(&hcount[0], off + nsv, sizeof(int), hipMemcpyDeviceToHost, s));
         SGIPC_HIP(hipStreamSynchronize(s));
         if (hcount[0] <= pt_cap && hcount[0] > 0) {
-            hipLaunchKernelGGL(k_query_vf<1>, dim3(div_up(nsv, 128)), dim3(128), 0, s, nsv, sv, faces, x, Q, btype,
-                               body, cnt, (const int*)off, (int4*)pt_out, pt_cap, err);
+            hipLaunchKernelGGL(k_query_vf<1>, dim3(div_up(nsv, 128)), dim3(128), 0, s, nsv, sv, faces, x, dir, alpha,
+                               Q, btype, body, cnt, (const int*)off, (int4*)pt_out, pt_cap, err);
             SGIPC_LAUNCH_CHECK();
         }
     }
     // edge - edge
     if (ne > 1) {
-        lbvh_build<2>(c, B.edge, ne, edges, x);
+        lbvh_build<2>(c, B.edge, ne, edges, x, dir, alpha);
         const Box* ebox = (const Box*)B.edge.leaf.p;  // leaf boxes in edge order
         QueryCtx Q{(const Node*)B.edge.nodes.p, (const Box*)B.edge.nbox.p, ne, gap};
         int* cnt = B.cnt.reserve((size_t)ne + 1);

@@ -60,6 +60,8 @@ SIGNATURES = {
     "sgipc_ground_step_bound": ([P, I, P, P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_set_fem_projection": ([P, I], I),
     "sgipc_set_deterministic": ([P, I], I),
+    "sgipc_broad_phase_ccd": ([P, I, P, I, P, I, P, P, P, D, P, P, D, P, I, C.POINTER(I), P, I, C.POINTER(I)], I),
+    "sgipc_ccd_step_bound": ([P, I, P, I, P, P, P, D, C.POINTER(D)], I),
     "sgipc_broad_phase": ([P, I, P, I, P, I, P, P, P, P, D, P, I, C.POINTER(I), P, I, C.POINTER(I)], I),
     "sgipc_solve_batched": ([P, I, P, P, P, P, P], I),
     "sgipc_autotune": ([P, P, I, P, I, C.POINTER(I), C.POINTER(I)], I),

@@ -94,3 +94,48 @@ def test_pairs_from_bvh_cover_grid_candidates(torch_cuda, oracle, twobox8):
     sysm.setup_contacts(gpt, gee)
     grid_pairs = _canon(sysm.pairs.cpu().numpy(), sc.n)
     assert grid_pairs <= bvh_pairs
+
+
+def _broad_ccd(torch, sysm, faces, edges, surf, dhat, d, alpha, cap=(400000, 400000)):
+    from sgipc.system import _p
+    dev = dict(device="cuda:0")
+    f = torch.tensor(faces.astype(np.int32).reshape(-1), dtype=torch.int32, **dev)
+    e = torch.tensor(edges.astype(np.int32).reshape(-1), dtype=torch.int32, **dev)
+    v = torch.tensor(surf.astype(np.int32), dtype=torch.int32, **dev)
+    dd = torch.tensor(np.ascontiguousarray(d).reshape(-1), dtype=torch.float64, **dev)
+    pt = torch.zeros(4 * cap[0], dtype=torch.int32, **dev)
+    ee = torch.zeros(4 * cap[1], dtype=torch.int32, **dev)
+    npt, nee = C.c_int(), C.c_int()
+    sysm.ctx.call("sgipc_broad_phase_ccd", len(surf), _p(v), len(faces), _p(f), len(edges), _p(e), _p(sysm.x), _p(dd),
+                  float(alpha), None, None, float(dhat), _p(pt), cap[0], C.byref(npt), _p(ee), cap[1], C.byref(nee))
+    return pt[:4 * npt.value], ee[:4 * nee.value], dd, npt.value, nee.value
+
+
+def test_ccd_candidates_and_step_bound(torch_cuda, oracle, twobox8):
+    from broad_phase_ref import candidates
+    from sgipc.system import _p
+    sc, sysm, (faces, edges, surf) = twobox8
+    print("case: CCD swept candidates + ACCD step bound", flush=True)
+    rng = np.random.default_rng(7)
+    d = rng.normal(size=(sc.n, 3)) * 0.002
+    d[:, 1] += 0.004 * (np.arange(sc.n) >= sc.meta["n_lower"])   # the upper box moves down onto the lower
+    pt, ee, dd, npt, nee = _broad_ccd(torch_cuda, sysm, faces, edges, surf, sc.dhat, d, 1.0)
+    ptn = pt.cpu().numpy().reshape(-1, 4); een = ee.cpu().numpy().reshape(-1, 4)
+    rpt, ree = candidates(sc.x, faces, edges, surf, sc.dhat, dir=d, alpha=1.0)
+    assert set(map(tuple, ptn)) == set(map(tuple, rpt)) and set(map(tuple, een)) == set(map(tuple, ree))
+    s0, _, _, _, _ = candidates(sc.x, faces, edges, surf, sc.dhat) + (None,) * 3
+    assert npt > len(s0)                                          # sweeping adds candidates
+    a = C.c_double()
+    sysm.ctx.call("sgipc_ccd_step_bound", npt, _p(pt), nee, _p(ee), _p(sysm.x), _p(dd), 0.8, C.byref(a))
+    ao = oracle.ccd_step_bound(ptn, een, sc.x, d, 0.8)
+    assert 0 < a.value < 1 and abs(a.value - ao) <= 1e-12 * ao
+    # conservative advancement keeps every candidate at >= eta^2 of its start
+    # distance over the bounded step (eta = 1 - slackness = 0.2)
+    from test_oracle_ground_adjoint import _pt_d, _seg_seg_d
+    x0 = sc.x; x1 = sc.x - a.value * d
+    for q in ptn[:: max(1, len(ptn) // 400)]:
+        d0 = _pt_d(*x0[q])
+        assert _pt_d(*x1[q]) >= 0.04 * d0 * (1 - 1e-9)
+    for q in een[:: max(1, len(een) // 400)]:
+        d0 = _seg_seg_d(*x0[q])
+        assert _seg_seg_d(*x1[q]) >= 0.04 * d0 * (1 - 1e-9)

@@ -162,3 +162,106 @@ def test_reference_rollout_adjoint_vs_fd(oracle):
         fd = (loss(h) - loss(-h)) / (2 * h)
         ad = gE @ dE + np.sum(gv * dv)
         assert abs(fd - ad) <= 1e-5 * abs(ad)
+
+
+# ---- additive CCD ------------------------------------------------------------
+def _seg_seg_d(a0, a1, b0, b1):
+    """exact distance between two segments (closest points, clamped)"""
+    u, v, w = a1 - a0, b1 - b0, a0 - b0
+    a, b, c, d, e = u @ u, u @ v, v @ v, u @ w, v @ w
+    cands = []
+    D = a * c - b * b
+    if D > 1e-300:
+        s = (b * e - c * d) / D; t = (a * e - b * d) / D
+        if 0 <= s <= 1 and 0 <= t <= 1:
+            cands.append(np.linalg.norm(w + s * u - t * v))
+    for p, q0, q1 in ((a0, b0, b1), (a1, b0, b1), (b0, a0, a1), (b1, a0, a1)):
+        cands.append(_p_seg_d(p, q0, q1))
+    return min(cands)
+
+
+def _p_seg_d(p, q0, q1):
+    r = q1 - q0
+    t = np.clip((p - q0) @ r / (r @ r), 0, 1)
+    return np.linalg.norm(p - q0 - t * r)
+
+
+def _pt_d(p, t0, t1, t2):
+    """exact point-triangle distance"""
+    n = np.cross(t1 - t0, t2 - t0); n2 = n @ n
+    q = p - ((p - t0) @ n) / n2 * n
+    bc = [np.cross(t2 - t1, q - t1) @ n, np.cross(t0 - t2, q - t2) @ n, np.cross(t1 - t0, q - t0) @ n]
+    if min(bc) >= 0:
+        return abs((p - t0) @ n) / np.sqrt(n2)
+    return min(_p_seg_d(p, a, b) for a, b in ((t0, t1), (t1, t2), (t2, t0)))
+
+
+def _rot(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_accd_head_on_is_one_minus_eta_of_impact(oracle):
+    rng = np.random.default_rng(21)
+    for trial in range(20):
+        R, off = _rot(rng), rng.normal(size=3)
+        eta = rng.uniform(0.1, 0.5)
+        d0, speed = rng.uniform(0.01, 0.5), rng.uniform(0.2, 3.0)   # impact at d0 / speed
+        tstar = d0 / speed
+        # point above a triangle's interior, moving straight at it; the
+        # triangle drifts sideways with it (no change in the gap)
+        tri = np.array([[-1.0, -1, 0], [1, -0.5, 0], [0, 1, 0]])
+        p = np.array([0.05, 0.1, d0])
+        drift = np.array([rng.normal(), rng.normal(), 0.0]) * 0.1
+        v = np.concatenate([p, *tri]) .reshape(4, 3) @ R.T + off
+        u = np.stack([drift + [0, 0, -speed]] + [drift] * 3) @ R.T
+        t = oracle.accd(True, v, u, eta)
+        if tstar < 1:
+            assert abs(t - (1 - eta) * tstar) <= 1e-9 * tstar
+        else:
+            assert t == 1.0
+        # crossing edges approaching head-on
+        a = np.array([[-1.0, 0, d0], [1, 0, d0]]); b = np.array([[0.0, -1, 0], [0, 1, 0]])
+        v = np.concatenate([a, b]) @ R.T + off
+        u = np.stack([[0, 0, -0.5 * speed]] * 2 + [[0, 0, 0.5 * speed]] * 2) @ R.T
+        t = oracle.accd(False, v, u, eta)
+        if tstar < 1:
+            assert abs(t - (1 - eta) * tstar) <= 1e-9 * tstar
+        else:
+            assert t == 1.0
+
+
+def test_accd_never_passes_contact(oracle):
+    """motions built to touch at a chosen time t_c (the point reaches a point
+    of the moving triangle; the edges cross): the returned time stays below
+    t_c, and the exact distance stays positive on [0, toc]"""
+    rng = np.random.default_rng(5)
+    for trial in range(120):
+        pt = trial % 2 == 0
+        tc = rng.uniform(0.2, 0.9)
+        if pt:
+            tri0 = rng.normal(size=(3, 3)); ut = rng.normal(size=(3, 3)) * 0.5
+            w = rng.dirichlet([1.0, 1.0, 1.0])
+            c = w @ (tri0 + tc * ut)
+            p0 = c + rng.normal(size=3)
+            v = np.vstack([p0, tri0]); u = np.vstack([(c - p0) / tc, ut])
+            dist = lambda y: _pt_d(*y)
+        else:
+            b0 = rng.normal(size=(2, 3)); ub = rng.normal(size=(2, 3)) * 0.5
+            r, sa = rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9)
+            bt = b0 + tc * ub
+            c = bt[0] + r * (bt[1] - bt[0])
+            a0 = rng.normal(size=3); ua0 = rng.normal(size=3) * 0.5
+            a0t = a0 + tc * ua0
+            a1t = a0t + (c - a0t) / sa
+            a1 = a1t - rng.normal(size=3) * 0.5 * tc
+            v = np.vstack([a0, a1, b0]); u = np.vstack([ua0, (a1t - a1) / tc, ub])
+            dist = lambda y: _seg_seg_d(*y)
+        d0 = dist(v)
+        if d0 < 1e-2:
+            continue
+        assert dist(v + tc * u) < 1e-9                       # the construction touches at t_c
+        toc = oracle.accd(pt, v, u, 0.2)
+        assert 0 < toc < tc
+        for t in np.linspace(0.0, toc, 401):
+            assert dist(v + t * u) > 0
