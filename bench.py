@@ -104,10 +104,32 @@ def broad_phase_timing(sc, sysm, reps=5):
         torch.cuda.synchronize()
         ts.append(1e3 * (time.perf_counter() - t0))
     ts = sorted(ts[1:])
-    return {"ms": round(ts[len(ts) // 2], 3), "surface_vertices": int(len(surf)), "faces": int(len(faces)),
-            "edges": int(len(edges)), "pt_candidates": npt.value, "ee_candidates": nee.value,
-            "pairs": npair.value, "note": "LBVH broad phase + narrow phase; the Newton step uses the grid-cell "
-            "candidate pairs (a subset of these)"}
+    out = {"ms": round(ts[len(ts) // 2], 3), "surface_vertices": int(len(surf)), "faces": int(len(faces)),
+           "edges": int(len(edges)), "pt_candidates": npt.value, "ee_candidates": nee.value,
+           "pairs": npair.value, "note": "LBVH broad phase + narrow phase; the Newton step uses the grid-cell "
+           "candidate pairs (a subset of these)"}
+    # line-search step bound along the Newton direction of this iterate
+    sysm.newton_step(tol=1e-6, stop_rule=1)
+    a = C.c_double(1.0)
+    tc = []
+    ccap = 8 * cap
+    pt = torch.zeros(4 * ccap, **dev); ee = torch.zeros(4 * ccap, **dev)
+    cap = ccap
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sysm.ctx.call("sgipc_broad_phase_ccd", len(surf), _p(v), len(faces), _p(f), len(edges), _p(e), _p(sysm.x),
+                      _p(sysm.dx), 1.0, _p(sysm.btype), None, float(sc.dhat), _p(pt), cap, C.byref(npt), _p(ee), cap,
+                      C.byref(nee))
+        sysm.ctx.call("sgipc_ccd_step_bound", npt.value, _p(pt), nee.value, _p(ee), _p(sysm.x), _p(sysm.dx), 0.8,
+                      C.byref(a))
+        torch.cuda.synchronize()
+        tc.append(1e3 * (time.perf_counter() - t0))
+    tc = sorted(tc[1:])
+    out["ccd"] = {"ms": round(tc[len(tc) // 2], 3), "pt_candidates": npt.value, "ee_candidates": nee.value,
+                  "step_bound": a.value, "note": "swept-box broad phase + additive CCD along the Newton direction, "
+                  "slackness 0.8"}
+    return out
 
 
 def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5):

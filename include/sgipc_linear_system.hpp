@@ -154,6 +154,20 @@ public:
               ctx_, "sgipc_classify_contacts");
         return np;
     }
+    // line search: largest feasible step along -d_dir (GIPC::buildFullCP +
+    // self_largestFeasibleStepSize, GIPC.cu:9085, 8880-8915)
+    double ccd_step_bound(int n_surf, const uint32_t* d_surf, int n_faces, const uint32_t* d_faces, int n_edges,
+                          const uint32_t* d_edges, const double* d_x, const double* d_dir, double dhat,
+                          double slackness, int* d_pt, int* d_ee, int cand_capacity) {
+        int npt = 0, nee = 0;
+        double a = 1.0;
+        check(sgipc_broad_phase_ccd(ctx_, n_surf, d_surf, n_faces, d_faces, n_edges, d_edges, d_x, d_dir, 1.0, nullptr,
+                                    nullptr, dhat, d_pt, cand_capacity, &npt, d_ee, cand_capacity, &nee),
+              ctx_, "sgipc_broad_phase_ccd");
+        check(sgipc_ccd_step_bound(ctx_, npt, d_pt, nee, d_ee, d_x, d_dir, slackness, &a), ctx_,
+              "sgipc_ccd_step_bound");
+        return a;
+    }
     // ground plane n.x = offset (GIPC::_groundNormal/_groundOffset):
     // close vertices of d_cand into d_close (candidate order), then their
     // barrier gradient and 3x3 blocks (GIPC.cu:6433-6514)
