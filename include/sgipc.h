@@ -63,6 +63,15 @@ typedef struct sgipc_solve_stats {
     double rz0;
 } sgipc_solve_stats;
 
+/* The sixteen values of the reference's parameterSetting.txt, in file order
+ * (LoadSettings, StiffGIPC/gl_main.cu:677-726). */
+typedef struct sgipc_parameters {
+    double density, poisson_rate, friction_rate, gd_friction_rate;
+    double cloth_thickness, cloth_youngs_modulus, bend_youngs_modulus, cloth_density;
+    double strain_rate, soft_motion_rate, collision_detection_buff_scale, motion_rate;
+    double dt, pcg_threshold, newton_threshold, relative_dhat;
+} sgipc_parameters;
+
 /* ---- context -------------------------------------------------------------- */
 int sgipc_version(void);
 /* replaces GlobalLinearSystem construction (global_linear_system.h:24) */
@@ -287,6 +296,22 @@ int sgipc_fem_energy(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d
  * __cal_Barrier_energy (:701-957).  dhat squared, as GIPC::dHat. */
 int sgipc_barrier_energy(sgipc_ctx* ctx, int n_pairs, const int* d_pairs, const double* d_x, const double* d_xrest,
                          double dhat, double kappa, double* h_energy);
+
+/* ---- the reference's file formats (host only, no context) ------------------
+ * gmsh 2.2 ASCII tet meshes: tetrahedra_obj::load_tetrahedraMesh
+ * (StiffGIPC/load_mesh.cpp:481-615; vertex ids 0-based on return, positions
+ * untransformed), writer as TetMesh::export_mesh
+ * (MeshProcess/metis_partition/src/mesh.cpp:115-141); .part files,
+ * tetrahedra_obj::load_parts (StiffGIPC/load_mesh.cpp:447-478);
+ * parameterSetting.txt, LoadSettings (StiffGIPC/gl_main.cu:677-726).
+ * Failures return SGIPC_ERR_* with a message in sgipc_io_last_error(). */
+int sgipc_msh_counts(const char* path, int* n_vertices, int* n_tets);
+int sgipc_msh_read(const char* path, double* h_x, int n_vertices, uint32_t* h_tets, int n_tets);
+int sgipc_msh_write(const char* path, int n_vertices, const double* h_x, int n_tets, const uint32_t* h_tets);
+/* *n_read = ids in the file; SGIPC_ERR_CAPACITY if more than capacity */
+int sgipc_part_read(const char* path, int* h_part, int capacity, int* n_read);
+int sgipc_parameters_read(const char* path, sgipc_parameters* params);
+const char* sgipc_io_last_error(void);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* HIP-event timing of selected launches on the context stream (no reference

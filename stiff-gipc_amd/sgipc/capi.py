@@ -39,6 +39,12 @@ _lib = None
 P, I, LL, D, SZ = C.c_void_p, C.c_int, C.c_longlong, C.c_double, C.c_size_t
 SIGNATURES = {
     "sgipc_version": ([], I),
+    "sgipc_msh_counts": ([C.c_char_p, C.POINTER(I), C.POINTER(I)], I),
+    "sgipc_msh_read": ([C.c_char_p, P, I, P, I], I),
+    "sgipc_msh_write": ([C.c_char_p, I, P, I, P], I),
+    "sgipc_part_read": ([C.c_char_p, P, I, C.POINTER(I)], I),
+    "sgipc_parameters_read": ([C.c_char_p, P], I),
+    "sgipc_io_last_error": ([], C.c_char_p),
     "sgipc_ctx_create": ([I, C.POINTER(P)], I),
     "sgipc_ctx_destroy": ([P], I),
     "sgipc_ctx_set_stream": ([P, P], I),

@@ -428,3 +428,26 @@ def surface_primitives(tets):
     e = np.unique(np.sort(e, axis=1), axis=0)
     verts = np.unique(faces.reshape(-1))
     return faces, np.ascontiguousarray(e, np.uint32), np.ascontiguousarray(verts, np.uint32)
+
+
+def mesh_scene(x_rest, tets, part, E=1e5, nu=0.45, density=1000.0, dt=0.01, gravity=-9.8, pinned=None, bank=16):
+    """Scene at rest from a loaded tet mesh (e.g. sgipc.io.read_msh of the
+    reference's *_sorted.16.msh with its .part ids, vertices already in part
+    order): lumped masses, SNK1 parameters, optional pinned-vertex mask."""
+    x_rest = np.ascontiguousarray(x_rest, np.float64).reshape(-1, 3)
+    tets = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+    # positive orientation, as the precomputation expects
+    t = tets.astype(np.int64)
+    d = np.einsum("ij,ij->i", np.cross(x_rest[t[:, 1]] - x_rest[t[:, 0]], x_rest[t[:, 2]] - x_rest[t[:, 0]]),
+                  x_rest[t[:, 3]] - x_rest[t[:, 0]])
+    neg = d < 0
+    tets = tets.copy()
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    DmInv, vol, mass, mu, lam = rest_precompute(x_rest, tets, E, nu, density)
+    btype = np.zeros(x_rest.shape[0], np.int32) if pinned is None else np.asarray(pinned, np.int32)
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = x_rest + dt * dt * g
+    x_tilde[btype != 0] = x_rest[btype != 0]
+    return Scene("mesh%d" % x_rest.shape[0], x_rest, x_rest.copy(), x_tilde, tets,
+                 np.ascontiguousarray(part, np.int32), btype, DmInv, vol, mass, mu, lam, dt, bank=bank,
+                 meta=dict(E=E, nu=nu, density=density, gravity=gravity))
