@@ -157,6 +157,63 @@ int sgipc_ground_step_bound(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, co
                             const double* d_dir, const double normal[3], double offset, double slack,
                             double* h_alpha);
 
+/* ---- affine bodies (ABD) ---------------------------------------------------
+ * Stiff bodies with 12 DOF q = (p, a1, a2, a3) each; body vertex i sits at
+ * x_i = p + (a1.xbar_i, a2.xbar_i, a3.xbar_i) (ABDJacobi,
+ * StiffGIPC/abd_system/abd_jacobi_matrix.h:7-60).  Vertex space: the body
+ * vertices first, grouped by body, then the FEM vertices.  DOF space (block
+ * rows of the system): 4 per body, then one per FEM vertex, i.e. FEM vertex
+ * v (vertex id) sits at block row v - n_body_vertices + 4 n_bodies, the
+ * reference's abd_fem_count_info layout (StiffGIPC/GIPC.cu:10181).  With a
+ * MAS preconditioner over sub_offset = 4 n_bodies, sgipc_precond_build also
+ * builds the bodies' 12x12 block-Jacobi (ABDPreconditioner,
+ * linear_system/preconditioner/abd_preconditioner.cu:15-38). */
+/* Host arrays: h_vstart (n_bodies + 1) body vertex ranges covering
+ * 0 .. n_body_vertices; h_xbar (3 per body vertex) rest coordinates;
+ * h_mass12 (144 per body, row-major) the 12x12 mass matrices; h_volume (one
+ * per body); h_fixed (one per body, may be NULL) nonzero for fixed bodies
+ * (BodyBoundaryType::Fixed).  Replaces the ABDSimData state the reference
+ * builds at load time (abd_system/abd_sim_data.h:40-100). */
+int sgipc_abd_setup(sgipc_ctx* ctx, int n_bodies, int n_body_vertices, const int* h_vstart, const double* h_xbar,
+                    const double* h_mass12, const double* h_volume, const int* h_fixed);
+/* exact == 0 (default): a same-body vertex pair (i, j), i < j, maps to the
+ * upper blocks of J_i^T H J_j, as write_barrier_hessian
+ * (abd_system_function/setup_abd_system_gradient_and_hessian.cu:87-128);
+ * exact != 0: to the upper blocks of J_i^T H J_j + (J_i^T H J_j)^T, the
+ * exact J^T H J (no reference counterpart; for adjoint solves). */
+int sgipc_abd_set_exact_mapping(sgipc_ctx* ctx, int exact);
+/* d_x[i] = J_i q for the body vertices (3 per vertex); applied to a step dq
+ * it gives the vertex step.  cal_x_from_q / cal_dx_from_dq
+ * (abd_system/abd_system_function/cal_x_from_q.cu:5-120). */
+int sgipc_abd_positions(sgipc_ctx* ctx, const double* d_q, double* d_x);
+/* Per body: d_grad_q (12 per body) = M (q - q~) + kappa V dt^2 dS/dq and 10
+ * upper 3x3 triplets of M + PSD(kappa V dt^2 d2S/dq2) at triplet_offset
+ * (fixed bodies: gradient 0, Hessian M); ABDSystem::
+ * _cal_abd_body_gradient_and_hessian (setup_abd_system_gradient_and_hessian.cu:
+ * 201-345) and write_abd_body_hessian (:132-141). */
+int sgipc_abd_assemble(sgipc_ctx* ctx, const double* d_q, const double* d_q_tilde, double kappa, double dt,
+                       long long triplet_offset, double* d_grad_q);
+/* d_grad_q[b] += sum over the body's vertices of J_i^T g_i for a vertex-space
+ * gradient d_vertex_gradient (fixed bodies unchanged), summed in vertex
+ * order; _cal_abd_system_barrier_gradient
+ * (setup_abd_system_gradient_and_hessian.cu:348-422). */
+int sgipc_abd_contact_gradient(sgipc_ctx* ctx, const double* d_vertex_gradient, double* d_grad_q);
+/* Maps the vertex-space contact triplets [src_offset, src_offset + count)
+ * into DOF-space triplets written from dst_offset (the ranges must not
+ * overlap): FEM-FEM blocks shifted, body-FEM blocks as the 4 blocks of
+ * J_i^T H, body-body blocks as J_i^T H J_j (16 blocks; 10 upper ones for a
+ * same-body pair); all-zero blocks and blocks touching a fixed body are
+ * dropped.  *h_n_written receives the count (synchronises);
+ * _setup_abd_system_hessian (setup_abd_system_gradient_and_hessian.cu:
+ * 424-610) and the FEM remap (StiffGIPC/GIPC.cu:10180-10215). */
+int sgipc_abd_map_contacts(sgipc_ctx* ctx, long long src_offset, long long count, long long dst_offset,
+                           long long* h_n_written);
+/* sum over bodies of 0.5 (q - q~)^T M (q - q~) (0 for fixed bodies) plus
+ * kappa V dt^2 S(q) (FP64, fixed order); cal_abd_kinetic_energy /
+ * cal_abd_shape_energy (abd_system_function/cal_abd_energy.cu:6-140). */
+int sgipc_abd_energy(sgipc_ctx* ctx, const double* d_q, const double* d_q_tilde, double kappa, double dt,
+                     double* h_energy);
+
 /* ---- differentiation -------------------------------------------------------
  * No reference counterpart (the reference has no adjoint); these serve an
  * implicit-function adjoint of the converged Newton step. */

@@ -394,8 +394,74 @@ int sgipc_precond_build(sgipc_ctx* c, int kind, int sub_offset, int sub_nb, cons
         c->sub_nb = sub_nb;
         sgipc::prof_begin(*c, sgipc::PROF_PRECOND_BUILD);
         if (kind == SGIPC_PRECOND_DIAG) sgipc::diag_build(*c);
-        if (kind == SGIPC_PRECOND_MAS || kind == SGIPC_PRECOND_MAS_FP32) sgipc::mas_build(*c, sub_offset, sub_nb, pairs, npairs);
+        if (kind == SGIPC_PRECOND_MAS || kind == SGIPC_PRECOND_MAS_FP32) {
+            sgipc::mas_build(*c, sub_offset, sub_nb, pairs, npairs);
+            // affine bodies below the MAS range: their 12x12 block-Jacobi
+            if (c->abd.ready && sub_offset >= 4 * c->abd.nb) sgipc::abd_jacobi_build(*c);
+            else c->abd.inv_ready = false;
+        }
         sgipc::prof_end(*c, sgipc::PROF_PRECOND_BUILD);
+    });
+}
+
+int sgipc_abd_setup(sgipc_ctx* c, int n_bodies, int n_body_vertices, const int* vstart, const double* xbar,
+                    const double* mass12, const double* volume, const int* fixed) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        set_device(c);
+        sgipc::abd_setup(*c, n_bodies, n_body_vertices, vstart, xbar, mass12, volume, fixed);
+    });
+}
+
+int sgipc_abd_set_exact_mapping(sgipc_ctx* c, int exact) {
+    CHECK_CTX(c);
+    return guard(c, [&] { c->abd.exact_same = exact ? 1 : 0; });
+}
+
+int sgipc_abd_positions(sgipc_ctx* c, const double* q, double* x) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!q || !x) throw sgipc::arg_error("abd_positions: null array");
+        set_device(c);
+        sgipc::abd_positions(*c, q, x);
+    });
+}
+
+int sgipc_abd_assemble(sgipc_ctx* c, const double* q, const double* q_tilde, double kappa, double dt,
+                       long long triplet_offset, double* grad_q) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!q || !q_tilde || !grad_q) throw sgipc::arg_error("abd_assemble: null array");
+        set_device(c);
+        sgipc::abd_assemble(*c, q, q_tilde, kappa, dt, triplet_offset, grad_q);
+    });
+}
+
+int sgipc_abd_contact_gradient(sgipc_ctx* c, const double* vertex_gradient, double* grad_q) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!vertex_gradient || !grad_q) throw sgipc::arg_error("abd_contact_gradient: null array");
+        set_device(c);
+        sgipc::abd_contact_gradient(*c, vertex_gradient, grad_q);
+    });
+}
+
+int sgipc_abd_map_contacts(sgipc_ctx* c, long long src_offset, long long count, long long dst_offset,
+                           long long* n_written) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!n_written) throw sgipc::arg_error("abd_map_contacts: null output");
+        set_device(c);
+        *n_written = sgipc::abd_map_contacts(*c, src_offset, count, dst_offset);
+    });
+}
+
+int sgipc_abd_energy(sgipc_ctx* c, const double* q, const double* q_tilde, double kappa, double dt, double* e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!q || !q_tilde || !e) throw sgipc::arg_error("abd_energy: null array");
+        set_device(c);
+        *e = sgipc::abd_energy(*c, q, q_tilde, kappa, dt);
     });
 }
 

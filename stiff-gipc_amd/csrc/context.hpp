@@ -110,6 +110,17 @@ struct BatchWork {
     PcgGraph graph64, graph32, graph_mixed;
 };
 
+// Affine bodies coupled to the FEM system (abd.hip)
+struct AbdState {
+    bool ready = false, inv_ready = false;
+    int nb = 0, nav = 0;      // bodies; body vertices (vertex ids 0 .. nav)
+    int exact_same = 0;       // 1: same-body vertex pairs mapped as X + X^T (exact); 0: reference form
+    DBuf<int> vstart, body_of, fixed;  // nb + 1 vertex ranges; nav body ids; nb fixed flags
+    DBuf<double> xbar, mass, vol;      // 3 nav rest coordinates; 144 nb mass matrices (row-major); nb volumes
+    DBuf<double> inv;                  // 144 nb block-Jacobi inverses (row-major)
+    DBuf<int> cnt, off, total;         // contact mapping counts / offsets / total
+};
+
 struct PcgWork {
     DBuf<double> r, z, p, Ap, xw;   // 3N each
     DBuf<double> r64b, d64;         // mixed precision: FP64 residual and correction
@@ -181,6 +192,7 @@ struct Ctx {
     MasState mas;
 
     PcgWork pcg;
+    AbdState abd;
     BatchWork batch;
     Lbvh lbvh;
     int batch_iters_per_graph = 4;  // batched PCG iterations captured per graph launch
@@ -290,5 +302,15 @@ void broad_phase(Ctx& c, int nsv, const unsigned* sv, int nf, const unsigned* fa
                  const double* x, const int* btype, const int* body, double dhat, int* pt_out, int pt_cap, int& n_pt,
                  int* ee_out, int ee_cap, int& n_ee, const double* dir = nullptr, double alpha = 0.0);
 void precond_apply(Ctx& c, const double* r, double* z);
+
+void abd_setup(Ctx& c, int nb, int nav, const int* vstart, const double* xbar, const double* mass12,
+               const double* vol, const int* fixed);
+void abd_positions(Ctx& c, const double* q, double* x);
+void abd_assemble(Ctx& c, const double* q, const double* qt, double kappa, double dt, long long toff, double* grad_q);
+void abd_contact_gradient(Ctx& c, const double* g, double* grad_q);
+long long abd_map_contacts(Ctx& c, long long src, long long m, long long dst);
+double abd_energy(Ctx& c, const double* q, const double* qt, double kappa, double dt);
+void abd_jacobi_build(Ctx& c);
+void abd_precond_apply(Ctx& c, const double* r, double* z);
 
 }  // namespace sgipc

@@ -151,12 +151,14 @@ __device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_pr
     return stop;
 }
 
-// Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range (identity
-// elsewhere) and partials of r.z; one thread per block row.
+// Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range, the 12x12
+// block-Jacobi of the affine bodies on block rows < abd_rows (when abd_inv is
+// set), identity elsewhere, and partials of r.z; one thread per block row.
 template <typename V, typename T>
 __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __restrict__ r, V* __restrict__ z,
                                                           int sub_off, int n_mas, int L, const int* __restrict__ node_of,
-                                                          const T* __restrict__ Z, double* __restrict__ part_rz,
+                                                          const T* __restrict__ Z, const double* __restrict__ abd_inv,
+                                                          int abd_rows, double* __restrict__ part_rz,
                                                           const double* __restrict__ part_rz_prev, int np_rz,
                                                           const double* __restrict__ part_rr, int np_rr,
                                                           double* __restrict__ sc, int* __restrict__ ctrl, int rule) {
@@ -174,6 +176,17 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
             for (int l = 0; l < L; ++l) {
                 size_t g = 3 * (size_t)node_of[(size_t)l * n_mas + vt];
                 s0 += Z[g]; s1 += Z[g + 1]; s2 += Z[g + 2];
+            }
+            z0 = (V)s0; z1 = (V)s1; z2 = (V)s2;
+        } else if (row < abd_rows) {
+            const int b = row >> 2, k = row & 3;
+            const double* P = abd_inv + 144 * (size_t)b + 36 * k;  // rows 3k .. 3k+2 of the body inverse
+            const V* rb = r + 12 * (size_t)b;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < 12; ++j) {
+                const double rj = (double)rb[j];
+                s0 += P[j] * rj; s1 += P[12 + j] * rj; s2 += P[24 + j] * rj;
             }
             z0 = (V)s0; z1 = (V)s1; z2 = (V)s2;
         } else {
@@ -248,6 +261,12 @@ __global__ void k_diag_inv(int K, const int* __restrict__ rows, const int* __res
     R[2] = C * id; R[5] = -(a * h - b * g) * id; R[8] = (a * e - b * d) * id;
 }
 
+// the affine-body block-Jacobi applies on the body rows below the MAS range
+const double* abd_inv(const Ctx& c) {
+    return (c.abd.inv_ready && c.sub_offset >= 4 * c.abd.nb) ? (const double*)c.abd.inv.p : nullptr;
+}
+int abd_rows(const Ctx& c) { return abd_inv(c) ? 4 * c.abd.nb : 0; }
+
 int grid_for(long long N) {
     int g = div_up(N, kThreads);
     return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
@@ -283,8 +302,8 @@ struct Engine {
         if (mas) {
             mas_bank_solve(c, kFp32Mas, ctrl);
             hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
-                               c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, rz_out, rz_prev, gN, part_rr, gU, sc,
-                               ctrl, rule_now);
+                               c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, abd_inv(c), abd_rows(c), rz_out,
+                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now);
             SGIPC_LAUNCH_CHECK();
         } else {
             hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_out,
@@ -360,6 +379,7 @@ struct Engine {
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
+        mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c));
         if (mas) {
             const MasState& m = c.mas;
             mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
@@ -475,6 +495,7 @@ void precond_apply(Ctx& c, const double* r, double* z) {
         }
         if (c.precond_kind == 2) mas_apply64(c, r + off, z + off);
         else mas_apply32(c, r + off, z + off);
+        if (abd_inv(c)) abd_precond_apply(c, r, z);
         return;
     }
     double* part = c.pcg.partials.reserve(5 * kPart);

@@ -65,6 +65,13 @@ SIGNATURES = {
     "sgipc_ground_energy": ([P, I, P, P, P, D, D, D, C.POINTER(D)], I),
     "sgipc_ground_step_bound": ([P, I, P, P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_set_fem_projection": ([P, I], I),
+    "sgipc_abd_setup": ([P, I, I, P, P, P, P, P], I),
+    "sgipc_abd_set_exact_mapping": ([P, I], I),
+    "sgipc_abd_positions": ([P, P, P], I),
+    "sgipc_abd_assemble": ([P, P, P, D, D, LL, P], I),
+    "sgipc_abd_contact_gradient": ([P, P, P], I),
+    "sgipc_abd_map_contacts": ([P, LL, LL, LL, C.POINTER(LL)], I),
+    "sgipc_abd_energy": ([P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_set_deterministic": ([P, I], I),
     "sgipc_broad_phase_ccd": ([P, I, P, I, P, I, P, P, P, D, P, P, D, P, I, C.POINTER(I), P, I, C.POINTER(I)], I),
     "sgipc_ccd_step_bound": ([P, I, P, I, P, P, P, D, C.POINTER(D)], I),

@@ -451,3 +451,110 @@ def mesh_scene(x_rest, tets, part, E=1e5, nu=0.45, density=1000.0, dt=0.01, grav
     return Scene("mesh%d" % x_rest.shape[0], x_rest, x_rest.copy(), x_tilde, tets,
                  np.ascontiguousarray(part, np.int32), btype, DmInv, vol, mass, mu, lam, dt, bank=bank,
                  meta=dict(E=E, nu=nu, density=density, gravity=gravity))
+
+
+@dataclass
+class HybridScene:
+    """Affine bodies (stiff, 12 DOF each) over a soft FEM body.  Vertex ids:
+    the nav body vertices first (grouped by body), then the FEM vertices of
+    `fem`; DOF block rows: 4 per body, then one per FEM vertex."""
+    fem: Scene                  # soft part (its own vertex numbering)
+    vstart: np.ndarray          # (nb + 1,) body vertex ranges
+    xbar: np.ndarray            # (nav, 3) body-frame rest coordinates (about the mass centre)
+    body_tets: np.ndarray       # (Tb, 4) body tets in body-vertex ids
+    q: np.ndarray               # (nb, 12) current q = (p, a1, a2, a3)
+    q_tilde: np.ndarray         # (nb, 12) inertial target
+    mass12: np.ndarray          # (nb, 144) row-major sum_i m_i J_i^T J_i
+    volume: np.ndarray          # (nb,)
+    fixed: np.ndarray           # (nb,) int32
+    kappa_abd: float
+    faces: np.ndarray           # union surface triangles (vertex space)
+    edges: np.ndarray
+    surf: np.ndarray
+    body_id: np.ndarray         # (nav + nf,) int32: body of a body vertex, -1 for FEM vertices
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nb(self):
+        return self.q.shape[0]
+
+    @property
+    def nav(self):
+        return self.xbar.shape[0]
+
+
+def _rot_y(t):
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def hybrid_scene(cells=(100, 20, 100), lattice=4, seed=3, E=1e5, nu=0.45, density=1000.0, dt=0.005,
+                 gravity=-9.8, size=1.0, cube=0.1, rel_dhat=1e-3, kappa=1e4, kappa_abd=1e11, bank=16,
+                 jitter=0.05, max_turn=0.3):
+    """Hybrid stiff/soft workload: lattice^3 affine cubes (one cell, 8
+    vertices and 5 tets each) over a soft SNK1 slab of cells = (nx, ny, nz)
+    cells (y up, bottom face pinned), slab width `size`.  The cubes sit on a
+    seeded jittered lattice (x/z offsets up to `jitter` * size, turns about y
+    up to `max_turn` rad) in columns of `lattice` layers, each layer resting
+    0.5 dhat above the one below and the bottom layer 0.5 dhat above the
+    slab, so body-slab and body-body contact pairs are active.  dhat =
+    rel_dhat * size (a distance; stored squared).  q starts at (mass centre,
+    rotation rows); xbar about the lumped mass centre (ABDSystem::_setup_J and
+    _calculate_body_mass_center, StiffGIPC/abd_system/abd_system.cu:172-251);
+    masses lumped per tet vertex (_setup_unique_point_mass, :141-170)."""
+    nx, ny, nz = cells
+    h = size / nx
+    thick = ny * h
+    dhat = rel_dhat * size
+    gap = 0.5 * dhat
+    slab = make_box(nx, ny, nz, h, bank=bank, pin_bottom=True)
+    DmInv, vol, mass, mu, lam = rest_precompute(slab.x_rest, slab.tets, E, nu, density)
+    g = np.array([0.0, gravity, 0.0])
+    x_tilde = slab.x_rest + dt * dt * g
+    x_tilde[slab.btype != 0] = slab.x_rest[slab.btype != 0]
+    fem = Scene("slab%dx%dx%d" % (nx, ny, nz), slab.x_rest, slab.x_rest.copy(), x_tilde, slab.tets, slab.part,
+                slab.btype, DmInv, vol, mass, mu, lam, dt, dhat=dhat * dhat, kappa=kappa, bank=bank,
+                meta=dict(E=E, nu=nu, gravity=gravity))
+    # one cube: a single grid cell, lumped masses, mass centre
+    cv, ct, _ = tet_grid(1, 1, 1, cube)
+    _, cvol, cmass, _, _ = rest_precompute(cv, ct, E, nu, density)
+    centre = (cmass[:, None] * cv).sum(0) / cmass.sum()
+    xb = cv - centre
+    J = np.zeros((8, 3, 12))
+    J[:, :, :3] = np.eye(3)
+    for k in range(3):
+        J[:, k, 3 + 3 * k:6 + 3 * k] = xb
+    M1 = np.einsum("i,iak,ial->kl", cmass, J, J)
+    rng = np.random.default_rng(seed)
+    nb = lattice ** 3
+    q = np.zeros((nb, 12))
+    pos = np.zeros((nb * 8, 3))
+    spacing = size / lattice
+    b = 0
+    for layer in range(lattice):
+        y0 = thick + gap + layer * (cube + gap)      # bottom face of this layer
+        for k in range(lattice):
+            for i in range(lattice):
+                R = _rot_y(rng.uniform(-max_turn, max_turn))
+                cx = (i + 0.5) * spacing + rng.uniform(-jitter, jitter) * size
+                cz = (k + 0.5) * spacing + rng.uniform(-jitter, jitter) * size
+                p = np.array([cx, y0 + centre[1], cz])
+                q[b, :3] = p
+                q[b, 3:] = R.reshape(9)               # rows a1, a2, a3
+                pos[8 * b:8 * b + 8] = p + xb @ R.T
+                b += 1
+    vstart = (8 * np.arange(nb + 1)).astype(np.int32)
+    xbar = np.tile(xb, (nb, 1))
+    body_tets = (ct.astype(np.int64)[None, :, :] + 8 * np.arange(nb)[:, None, None]).reshape(-1, 4)
+    nav = 8 * nb
+    all_tets = np.concatenate([body_tets, slab.tets.astype(np.int64) + nav]).astype(np.uint32)
+    faces, edges, surf = surface_primitives(all_tets)
+    qg = np.zeros(12)
+    qg[:3] = g                                        # mass-centred xbar: gravity moves p only
+    q_tilde = q + dt * dt * qg
+    body_id = np.concatenate([np.repeat(np.arange(nb), 8), -np.ones(slab.x_rest.shape[0], np.int64)]).astype(np.int32)
+    return HybridScene(fem, vstart, xbar, body_tets.astype(np.uint32), q, q_tilde,
+                       np.tile(M1.reshape(1, 144), (nb, 1)), np.full(nb, float(cvol.sum())), np.zeros(nb, np.int32),
+                       float(kappa_abd), faces, edges, surf, body_id,
+                       meta=dict(cells=cells, lattice=lattice, seed=seed, dhat_dist=dhat, gap=gap, cube=cube,
+                                 positions=pos, gravity=gravity))
