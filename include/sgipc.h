@@ -176,12 +176,6 @@ int sgipc_ground_step_bound(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, co
  * builds at load time (abd_system/abd_sim_data.h:40-100). */
 int sgipc_abd_setup(sgipc_ctx* ctx, int n_bodies, int n_body_vertices, const int* h_vstart, const double* h_xbar,
                     const double* h_mass12, const double* h_volume, const int* h_fixed);
-/* exact == 0 (default): a same-body vertex pair (i, j), i < j, maps to the
- * upper blocks of J_i^T H J_j, as write_barrier_hessian
- * (abd_system_function/setup_abd_system_gradient_and_hessian.cu:87-128);
- * exact != 0: to the upper blocks of J_i^T H J_j + (J_i^T H J_j)^T, the
- * exact J^T H J (no reference counterpart; for adjoint solves). */
-int sgipc_abd_set_exact_mapping(sgipc_ctx* ctx, int exact);
 /* d_x[i] = J_i q for the body vertices (3 per vertex); applied to a step dq
  * it gives the vertex step.  cal_x_from_q / cal_dx_from_dq
  * (abd_system/abd_system_function/cal_x_from_q.cu:5-120). */
@@ -199,11 +193,13 @@ int sgipc_abd_assemble(sgipc_ctx* ctx, const double* d_q, const double* d_q_tild
  * (setup_abd_system_gradient_and_hessian.cu:348-422). */
 int sgipc_abd_contact_gradient(sgipc_ctx* ctx, const double* d_vertex_gradient, double* d_grad_q);
 /* Maps the vertex-space contact triplets [src_offset, src_offset + count)
- * into DOF-space triplets written from dst_offset (the ranges must not
- * overlap): FEM-FEM blocks shifted, body-FEM blocks as the 4 blocks of
- * J_i^T H, body-body blocks as J_i^T H J_j (16 blocks; 10 upper ones for a
- * same-body pair); all-zero blocks and blocks touching a fixed body are
- * dropped.  *h_n_written receives the count (synchronises);
+ * (upper storage, as sgipc_assemble_barrier writes them) into DOF-space
+ * triplets written from dst_offset (the ranges must not overlap): FEM-FEM
+ * blocks shifted, body-FEM blocks as the 4 blocks of J_i^T H, body-body
+ * blocks as J_i^T H J_j (16 blocks; for a same-body pair the 10 upper blocks
+ * of J_i^T H J_j + (J_i^T H J_j)^T), i.e. the upper blocks of J^T H J, the
+ * matrix the reference builds from its full vertex-space storage; all-zero
+ * blocks and blocks touching a fixed body are dropped.  *h_n_written receives the count (synchronises);
  * _setup_abd_system_hessian (setup_abd_system_gradient_and_hessian.cu:
  * 424-610) and the FEM remap (StiffGIPC/GIPC.cu:10180-10215). */
 int sgipc_abd_map_contacts(sgipc_ctx* ctx, long long src_offset, long long count, long long dst_offset,

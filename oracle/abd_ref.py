@@ -14,10 +14,15 @@ Follows:
   on the 9x9 affine block; fixed bodies G = 0, H = M;
 - write_abd_body_hessian (:132-141): 10 upper 3x3 blocks per body;
 - write_barrier_hessian (:87-128) with write_triplet_cv2 (:40-75): a
-  body-body vertex block (i, j) becomes J_i^T H J_j as 16 blocks, of which a
-  same-body pair keeps the upper ones; the ABD-FEM coupling (:521-580): the 4
-  blocks of J_i^T H; FEM-FEM blocks shifted to row v - nav + 4 nb
-  (StiffGIPC/GIPC.cu:10180-10215); blocks touching fixed bodies zero;
+  body-body vertex block (i, j) becomes J_i^T H J_j as 16 blocks, those
+  below the diagonal dropped; the ABD-FEM coupling (:521-580): the 4 blocks
+  of J_i^T H; FEM-FEM blocks shifted to row v - nav + 4 nb, lower ones zeroed
+  (StiffGIPC/GIPC.cu:10180-10215); blocks touching fixed bodies zero.  The
+  reference applies these to its FULL vertex-space storage (SymGH is off,
+  linear_system/linear_system/global_matrix.h:6: every vertex pair of a
+  contact appears as (i, j) and (j, i)); map_contacts_full_storage restates
+  that, map_contacts maps the upper storage the library uses (a same-body
+  pair i < j -> upper blocks of X + X^T);
 - _cal_abd_system_barrier_gradient (:348-422): grad_q += J_i^T g_i, fixed
   bodies skipped;
 - cal_abd_kinetic_energy / cal_abd_shape_energy (cal_abd_energy.cu:6-140);
@@ -27,8 +32,8 @@ The checks that pin it (tests/test_oracle_abd.py): central differences of
 the energy against the gradient and of the gradient against the Hessian,
 closed forms (S = 0 with zero gradient on rotations, S(sI) = 3 (s^2 - 1)^2),
 the mapped contact blocks against a dense J^T H J built from the assembled
-global J (exact form) and the reference form as that minus the dropped
-lower same-body blocks, and J^T g against the dense J.
+global J, the upper-storage mapping against the reference's full-storage
+procedure, and J^T g against the dense J.
 """
 import numpy as np
 
@@ -140,7 +145,7 @@ def contact_gradient(g, body_of, xbar, nb, fixed=None):
     return out
 
 
-def map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fixed=None, exact=False):
+def map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fixed=None):
     """Vertex-space upper blocks -> DOF-space blocks (rows, cols, (m,3,3)).
     vals are (m, 3, 3) in row-major math orientation (vals[t][a, c])."""
     shift = 4 * nb - nav
@@ -170,11 +175,45 @@ def map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fixed=None, exact=Fal
                 for l in range(4):
                     R.append(4 * bi + k); Cc.append(4 * bj + l); V.append(X[3 * k:3 * k + 3, 3 * l:3 * l + 3])
             continue
-        if exact and r != c:
+        if r != c:
             X = X + X.T
         for k in range(4):
             for l in range(k, 4):
                 R.append(4 * bi + k); Cc.append(4 * bi + l); V.append(X[3 * k:3 * k + 3, 3 * l:3 * l + 3])
+    return np.array(R, np.int64), np.array(Cc, np.int64), np.array(V).reshape(-1, 3, 3)
+
+
+def map_contacts_full_storage(rows, cols, vals, nav, nb, body_of, xbar, fixed=None):
+    """The reference's procedure on full vertex-space storage (each entry
+    (r, c, H) with r, c in any order): FEM-FEM entries shifted, lower ones
+    dropped (zeroed in the reference); body-FEM entries (body row) -> the 4
+    blocks of J_r^T H, FEM-body entries dropped (the lower class); body-body
+    -> the 16 blocks of J_r^T H J_c, those below the diagonal dropped
+    (written as zero blocks in the reference); fixed bodies -> nothing."""
+    shift = 4 * nb - nav
+    fixed = np.zeros(nb, bool) if fixed is None else np.asarray(fixed, bool)
+    R, Cc, V = [], [], []
+    for r, c, H in zip(rows, cols, vals):
+        if r >= nav and c >= nav:
+            if r <= c:
+                R.append(r + shift); Cc.append(c + shift); V.append(H)
+            continue
+        if r >= nav:                          # FEM row, body column: lower class
+            continue
+        if fixed[body_of[r]] or (c < nav and fixed[body_of[c]]):
+            continue
+        Jr = jacobian(xbar[r])
+        if c >= nav:
+            X = Jr.T @ H
+            for k in range(4):
+                R.append(4 * body_of[r] + k); Cc.append(c + shift); V.append(X[3 * k:3 * k + 3])
+            continue
+        X = Jr.T @ H @ jacobian(xbar[c])
+        for k in range(4):
+            for l in range(4):
+                row, col = 4 * body_of[r] + k, 4 * body_of[c] + l
+                if row <= col:
+                    R.append(row); Cc.append(col); V.append(X[3 * k:3 * k + 3, 3 * l:3 * l + 3])
     return np.array(R, np.int64), np.array(Cc, np.int64), np.array(V).reshape(-1, 3, 3)
 
 

@@ -13,7 +13,7 @@
 // 201-345: kinetic M (q - q~), shape term kappa V dt^2 with the 9x9 block
 // made PSD; fixed bodies: G = 0, H = M), write_abd_body_hessian (:132-141,
 // 10 upper blocks), write_barrier_hessian (:87-128, J_i^T H J_j written as
-// 16 blocks, lower blocks of a same-body pair dropped), the ABD-FEM 4-block
+// 16 blocks, those below the diagonal dropped), the ABD-FEM 4-block
 // coupling (:521-580), the barrier gradient J^T g (:348-422, fixed bodies
 // skipped), energies (cal_abd_energy.cu:6-140) and the 12x12 block-Jacobi
 // preconditioner (linear_system/preconditioner/abd_preconditioner.cu:15-38,
@@ -199,15 +199,17 @@ __device__ inline int map_count(int r, int c, const double* H, int nav, const in
 }
 
 // Vertex-space contact triplets -> DOF space. PASS 0 writes the counts,
-// PASS 1 the blocks at the scanned offsets. exact_same: for a same-body pair
-// i != j write the upper blocks of X + X^T (X = J_i^T H J_j, the exact
-// mapping of the symmetric pair); otherwise the upper blocks of X only (the
-// reference's form).
+// PASS 1 the blocks at the scanned offsets.  The input holds each vertex
+// pair once (upper, row <= col); the reference stores the contact blocks in
+// full (both (i, j) and (j, i); SymGH is off,
+// linear_system/linear_system/global_matrix.h:6) and keeps, per entry, the
+// mapped blocks on or above the diagonal.  For a same-body pair i != j that
+// is the upper part of X + X^T (X = J_i^T H J_j; the (j, i) entry maps to
+// X^T), written here from the single upper entry: the same matrix.
 template <int PASS>
 __global__ void k_abd_map(long long m, const double* __restrict__ sv, const int* __restrict__ sr,
                           const int* __restrict__ sc, int nav, int nbody, const int* __restrict__ body_of,
-                          const int* __restrict__ fixed, const double* __restrict__ xbar, int exact_same,
-                          int* __restrict__ cnt, const int* __restrict__ off, double* __restrict__ dv,
+                          const int* __restrict__ fixed, const double* __restrict__ xbar, int* __restrict__ cnt, const int* __restrict__ off, double* __restrict__ dv,
                           int* __restrict__ dr, int* __restrict__ dc) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
@@ -269,7 +271,7 @@ __global__ void k_abd_map(long long m, const double* __restrict__ sv, const int*
         for (int l = k; l < 4; ++l) {
             double B[9];
             jthj_block(xi, xj, H, k, l, B);
-            if (exact_same && r != c) {
+            if (r != c) {
                 double Bt[9];
                 jthj_block(xi, xj, H, l, k, Bt);  // block (l, k) of X, transposed into (k, l)
                 for (int p = 0; p < 3; ++p)
@@ -468,7 +470,7 @@ long long abd_map_contacts(Ctx& c, long long src, long long m, long long dst) {
     int* total_dev = a.total.reserve(2);
     hipLaunchKernelGGL(k_abd_map<0>, dim3(div_up(m, 256)), dim3(256), 0, s, m, (const double*)(c.t_vals + 9 * src),
                        (const int*)(c.t_rows + src), (const int*)(c.t_cols + src), a.nav, a.nb, (const int*)a.body_of.p,
-                       (const int*)a.fixed.p, (const double*)a.xbar.p, a.exact_same, cnt, (const int*)nullptr,
+                       (const int*)a.fixed.p, (const double*)a.xbar.p, cnt, (const int*)nullptr,
                        (double*)nullptr, (int*)nullptr, (int*)nullptr);
     SGIPC_LAUNCH_CHECK();
     exclusive_scan(c, cnt, off, (int)m, total_dev, s);
@@ -480,7 +482,7 @@ long long abd_map_contacts(Ctx& c, long long src, long long m, long long dst) {
     if (total == 0) return 0;
     hipLaunchKernelGGL(k_abd_map<1>, dim3(div_up(m, 256)), dim3(256), 0, s, m, (const double*)(c.t_vals + 9 * src),
                        (const int*)(c.t_rows + src), (const int*)(c.t_cols + src), a.nav, a.nb, (const int*)a.body_of.p,
-                       (const int*)a.fixed.p, (const double*)a.xbar.p, a.exact_same, cnt, (const int*)off,
+                       (const int*)a.fixed.p, (const double*)a.xbar.p, cnt, (const int*)off,
                        c.t_vals + 9 * dst, c.t_rows + dst, c.t_cols + dst);
     SGIPC_LAUNCH_CHECK();
     return total;

@@ -413,11 +413,6 @@ int sgipc_abd_setup(sgipc_ctx* c, int n_bodies, int n_body_vertices, const int* 
     });
 }
 
-int sgipc_abd_set_exact_mapping(sgipc_ctx* c, int exact) {
-    CHECK_CTX(c);
-    return guard(c, [&] { c->abd.exact_same = exact ? 1 : 0; });
-}
-
 int sgipc_abd_positions(sgipc_ctx* c, const double* q, double* x) {
     CHECK_CTX(c);
     return guard(c, [&] {

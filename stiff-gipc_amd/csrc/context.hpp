@@ -114,7 +114,6 @@ struct BatchWork {
 struct AbdState {
     bool ready = false, inv_ready = false;
     int nb = 0, nav = 0;      // bodies; body vertices (vertex ids 0 .. nav)
-    int exact_same = 0;       // 1: same-body vertex pairs mapped as X + X^T (exact); 0: reference form
     DBuf<int> vstart, body_of, fixed;  // nb + 1 vertex ranges; nav body ids; nb fixed flags
     DBuf<double> xbar, mass, vol;      // 3 nav rest coordinates; 144 nb mass matrices (row-major); nb volumes
     DBuf<double> inv;                  // 144 nb block-Jacobi inverses (row-major)
@@ -174,6 +173,10 @@ struct Ctx {
     DBuf<int> dyn_ustart, dyn_ucol, dyn_urun, merge_row;  // contact segment runs, merged rows
     DBuf<int4> merge_desc;                                 // per merged block: contact / static runs
     DBuf<int> long_rows, dev_int2;                         // rows sorted by one workgroup; their count
+    DBuf<int> vlong_rows, vl_rows, vl_lens, vl_flag, vl_sel, vl_val;  // rows sorted by a device radix sort
+    DBuf<long long> vl_off;
+    DBuf<unsigned long long> vl_key, vl_key2;
+    DBuf<char> vl_tmp;
     DBuf<double> energy_part;                              // energy partials (+ total)
     long long static_begin = -1;  // first triplet of the FEM + mass segment (set by FEM assembly)
     ConvertPlan cplan;

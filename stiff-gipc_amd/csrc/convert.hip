@@ -17,11 +17,17 @@
 // a full sort.
 #include "context.hpp"
 
+#include <algorithm>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 namespace sgipc {
 namespace {
 
 constexpr int kSortCap = 256;    // one wavefront per row, LDS: rows up to this many triplets
-constexpr int kLongCap = 4096;   // one workgroup per row, LDS: longer rows up to this many
+constexpr int kLongCap = 4096;   // one workgroup per row, LDS: longer rows up to this many;
+                                 // longer still: a device radix sort of those rows' triplets
 constexpr int kLongThreads = 256;
 constexpr int kRowsPerBlock = 4;  // one wavefront per row
 
@@ -61,7 +67,8 @@ __global__ void __launch_bounds__(64 * kRowsPerBlock) k_row_sort(int nbr, const 
                                                                  int* __restrict__ sorted,
                                                                  int* __restrict__ ucount,
                                                                  int* __restrict__ long_rows,
-                                                                 int* __restrict__ n_long) {
+                                                                 int* __restrict__ n_long,
+                                                                 int* __restrict__ vlong_rows) {
     __shared__ unsigned long long sk[kRowsPerBlock][kSortCap];
     __shared__ unsigned long long so[kRowsPerBlock][kSortCap];
     int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -95,24 +102,69 @@ __global__ void __launch_bounds__(64 * kRowsPerBlock) k_row_sort(int nbr, const 
         // long row: sorted by one workgroup in k_row_sort_long
         if (lane == 0) long_rows[atomicAdd(n_long, 1)] = r;
     } else if (len > 0) {
-        // very long rows (rare): rank counting straight from global memory
-        for (int i = lane; i < len; i += 64) {
-            int ti = bucket[s + i];
-            unsigned long long k = mkkey(cols[ti], ti);
-            int rank = 0;
-            bool dup_before = false;  // same col with a smaller triplet id
-            for (int j = 0; j < len; ++j) {
-                int tj = bucket[s + j];
-                unsigned long long kj = mkkey(cols[tj], tj);
-                rank += (kj < k) ? 1 : 0;
-                dup_before |= ((kj >> 32) == (k >> 32)) && (kj < k);
-            }
-            sorted[s + rank] = ti;
-            heads += dup_before ? 0 : 1;
-        }
+        // very long rows (rows of the affine bodies collect thousands):
+        // listed for the radix-sort path (sort_very_long_rows)
+        if (lane == 0) vlong_rows[atomicAdd(n_long + 1, 1)] = r;
     }
     heads = wave_sum(heads);
-    if (r < nbr && lane == 0 && !(len > kSortCap && len <= kLongCap)) ucount[r] = heads;
+    if (r < nbr && lane == 0 && len <= kSortCap) ucount[r] = heads;
+}
+
+// triplet t belongs to a very long row
+struct VeryLongRow {
+    const int* rows;
+    const int* flag;
+    int nbr;
+    __device__ bool operator()(int t) const {
+        const int r = rows[t];
+        return (unsigned)r < (unsigned)nbr && flag[r] != 0;
+    }
+};
+
+__global__ void k_mark_rows(int n, const int* __restrict__ list, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[list[i]] = 1;
+}
+
+__global__ void k_row_lengths(int n, const int* __restrict__ list, const int* __restrict__ start, int* __restrict__ len) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) len[i] = start[list[i] + 1] - start[list[i]];
+}
+
+__global__ void k_row_col_keys(int m, const int* __restrict__ sel, const int* __restrict__ rows,
+                               const int* __restrict__ cols, unsigned long long* __restrict__ key) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) key[i] = ((unsigned long long)(unsigned)rows[sel[i]] << 32) | (unsigned)cols[sel[i]];
+}
+
+// one workgroup per very long row: its (col, id)-sorted triplet ids into
+// place and the count of distinct columns
+__global__ void __launch_bounds__(kLongThreads) k_place_rows(int n, const int* __restrict__ list,
+                                                             const long long* __restrict__ off,
+                                                             const int* __restrict__ start,
+                                                             const unsigned long long* __restrict__ key,
+                                                             const int* __restrict__ val, int* __restrict__ sorted,
+                                                             int* __restrict__ ucount) {
+    __shared__ int hsum[kLongThreads / 64];
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const int r = list[q];
+        const int s = start[r], len = start[r + 1] - s;
+        const long long o = off[q];
+        int heads = 0;
+        for (int i = threadIdx.x; i < len; i += kLongThreads) {
+            sorted[s + i] = val[o + i];
+            heads += (i == 0 || (unsigned)key[o + i - 1] != (unsigned)key[o + i]) ? 1 : 0;
+        }
+        heads = wave_sum(heads);
+        if ((threadIdx.x & 63) == 0) hsum[threadIdx.x >> 6] = heads;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int h = 0;
+            for (int w = 0; w < kLongThreads / 64; ++w) h += hsum[w];
+            ucount[r] = h;
+        }
+        __syncthreads();
+    }
 }
 
 // Rows of kSortCap < len <= kLongCap triplets, one workgroup per row
@@ -343,6 +395,58 @@ __global__ void k_copy_back(int K, const int* __restrict__ r, const int* __restr
 // Sort the triplets [t0, t1) by (row, col, triplet id): start (nbr+1 row
 // starts into bucket/sorted), sorted (triplet ids), ucnt / ustart (unique
 // blocks per row; total -> *total_dev), bad indices -> *err_dev.
+// Rows longer than kLongCap (listed by k_row_sort): their triplets picked
+// from [t0, t1) in id order, radix-sorted (stably) by (row, col), and placed
+// row by row -- the same (col, id) order as the other rows.
+static void sort_very_long_rows(Ctx& c, long long t0, long long t1, int nbr, const int* start, int* sorted,
+                                int* ucnt, const int* vlong_dev, int nvl) {
+    hipStream_t s = c.stream;
+    std::vector<int> rows_h(nvl);
+    SGIPC_HIP(hipMemcpyAsync(rows_h.data(), vlong_dev, sizeof(int) * nvl, hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    std::sort(rows_h.begin(), rows_h.end());
+    int* list = c.vl_rows.reserve((size_t)nvl + 1);
+    int* lens = c.vl_lens.reserve((size_t)nvl + 1);
+    SGIPC_HIP(hipMemcpyAsync(list, rows_h.data(), sizeof(int) * nvl, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_row_lengths, dim3(div_up(nvl, 256)), dim3(256), 0, s, nvl, (const int*)list, start, lens);
+    SGIPC_LAUNCH_CHECK();
+    std::vector<int> lens_h(nvl);
+    SGIPC_HIP(hipMemcpyAsync(lens_h.data(), lens, sizeof(int) * nvl, hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    std::vector<long long> off_h(nvl);
+    long long m = 0;
+    for (int i = 0; i < nvl; ++i) { off_h[i] = m; m += lens_h[i]; }
+    long long* off = c.vl_off.reserve((size_t)nvl + 1);
+    SGIPC_HIP(hipMemcpyAsync(off, off_h.data(), sizeof(long long) * nvl, hipMemcpyHostToDevice, s));
+    int* flag = c.vl_flag.reserve((size_t)nbr + 1);
+    fill<int>(flag, 0, nbr, s);
+    hipLaunchKernelGGL(k_mark_rows, dim3(div_up(nvl, 256)), dim3(256), 0, s, nvl, (const int*)list, flag);
+    SGIPC_LAUNCH_CHECK();
+    int* sel = c.vl_sel.reserve((size_t)m + 1);
+    int* val2 = c.vl_val.reserve((size_t)m + 1);
+    unsigned long long* key = c.vl_key.reserve((size_t)m + 1);
+    unsigned long long* key2 = c.vl_key2.reserve((size_t)m + 1);
+    int* nsel = c.vl_lens.get() + nvl;  // one spare int after the lengths
+    rocprim::counting_iterator<int> ids((int)t0);
+    VeryLongRow pred{c.t_rows, flag, nbr};
+    size_t b1 = 0, b2 = 0;
+    SGIPC_HIP(rocprim::select(nullptr, b1, ids, sel, nsel, (size_t)(t1 - t0), pred, s));
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, b2, key, key2, sel, val2, (size_t)m, 0, 64, s));
+    char* tmp = c.vl_tmp.reserve(std::max(b1, b2) + 1);
+    b1 = std::max(b1, b2);
+    SGIPC_HIP(rocprim::select((void*)tmp, b1, ids, sel, nsel, (size_t)(t1 - t0), pred, s));
+    hipLaunchKernelGGL(k_row_col_keys, dim3(div_up(m, 256)), dim3(256), 0, s, (int)m, (const int*)sel, c.t_rows,
+                       c.t_cols, key);
+    SGIPC_LAUNCH_CHECK();
+    int end_bit = 33;
+    while (end_bit < 64 && (1ll << (end_bit - 32)) <= (long long)nbr) ++end_bit;
+    b2 = std::max(b1, b2);
+    SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, b2, key, key2, sel, val2, (size_t)m, 0, end_bit, s));
+    hipLaunchKernelGGL(k_place_rows, dim3(std::min(nvl, 1024)), dim3(kLongThreads), 0, s, nvl, (const int*)list,
+                       (const long long*)off, start, (const unsigned long long*)key2, (const int*)val2, sorted, ucnt);
+    SGIPC_LAUNCH_CHECK();
+}
+
 static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* cnt, int* start, int* bucket,
                          int* sorted, int* ucnt, int* ustart, int* err_dev, int* total_dev) {
     hipStream_t s = c.stream;
@@ -361,14 +465,19 @@ static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* cnt, 
         SGIPC_LAUNCH_CHECK();
     }
     int* long_rows = c.long_rows.reserve((size_t)nbr + 1);
-    int* n_long = c.dev_int2.reserve(2);
-    fill<int>(n_long, 0, 1, s);
+    int* vlong_rows = c.vlong_rows.reserve((size_t)nbr + 1);
+    int* n_long = c.dev_int2.reserve(2);   // [0] long rows, [1] very long rows
+    fill<int>(n_long, 0, 2, s);
     hipLaunchKernelGGL(k_row_sort, dim3(div_up(nbr, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, nbr, start,
-                       bucket, c.t_cols, sorted, ucnt, long_rows, n_long);
+                       bucket, c.t_cols, sorted, ucnt, long_rows, n_long, vlong_rows);
     SGIPC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_row_sort_long, dim3(1024), dim3(kLongThreads), 0, s, (const int*)long_rows,
                        (const int*)n_long, start, bucket, c.t_cols, sorted, ucnt);
     SGIPC_LAUNCH_CHECK();
+    int nvl = 0;
+    SGIPC_HIP(hipMemcpyAsync(&nvl, n_long + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    if (nvl > 0) sort_very_long_rows(c, t0, t1, nbr, start, sorted, ucnt, vlong_rows, nvl);
     exclusive_scan(c, ucnt, ustart, nbr, total_dev, s);
 }
 

@@ -66,7 +66,6 @@ SIGNATURES = {
     "sgipc_ground_step_bound": ([P, I, P, P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_set_fem_projection": ([P, I], I),
     "sgipc_abd_setup": ([P, I, I, P, P, P, P, P], I),
-    "sgipc_abd_set_exact_mapping": ([P, I], I),
     "sgipc_abd_positions": ([P, P, P], I),
     "sgipc_abd_assemble": ([P, P, P, D, D, LL, P], I),
     "sgipc_abd_contact_gradient": ([P, P, P], I),

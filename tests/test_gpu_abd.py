@@ -1,8 +1,8 @@
 """GPU parity of the affine-body (ABD) slice against the oracle
 (oracle/abd_ref.py), through the C ABI: body positions, body gradient and
-Hessian triplets, the DOF-space mapping of the contact blocks (reference and
-exact forms), the coupled system and its MAS + 12x12 block-Jacobi solve, the
-FP64 energy, fixed bodies, and a Newton solve of the hybrid scene."""
+Hessian triplets, the DOF-space mapping of the contact blocks (J^T H J),
+the coupled system and its MAS + 12x12 block-Jacobi solve, the FP64
+energy, fixed bodies, and a Newton solve of the hybrid scene."""
 import numpy as np
 import pytest
 
@@ -36,10 +36,9 @@ def _mat(v):
     return np.asarray(v).reshape(-1, 3, 3).transpose(0, 2, 1)
 
 
-def _system(hs, exact=False):
+def _system(hs):
     from sgipc.hybrid import HybridSystem
     s = HybridSystem(hs, "cuda:0")
-    s.ctx.call("sgipc_abd_set_exact_mapping", 1 if exact else 0)
     n = s.detect()
     assert n > 0
     s.assemble()
@@ -72,14 +71,13 @@ def test_positions_and_body_terms(torch_cuda):
     np.testing.assert_allclose(_mat(v[-nb10:]), bv, rtol=0, atol=1e-12 * np.abs(H).max())
 
 
-@pytest.mark.parametrize("exact", [False, True])
-def test_contact_mapping(torch_cuda, exact):
+def test_contact_mapping(torch_cuda):
     import abd_ref as A
-    print("case: abd contact mapping exact=%d" % exact, flush=True)
+    print("case: abd contact mapping", flush=True)
     hs = _scene()
-    s = _system(hs, exact)
+    s = _system(hs)
     sr, sc, sv = s.source_triplets()
-    R, Cc, V = A.map_contacts(sr, sc, _mat(sv), s.nav, s.nb, _body_of(hs), hs.xbar, hs.fixed, exact=exact)
+    R, Cc, V = A.map_contacts(sr, sc, _mat(sv), s.nav, s.nb, _body_of(hs), hs.xbar, hs.fixed)
     r, c, v = s.triplets()
     m = s.n_mapped
     assert m == len(R)

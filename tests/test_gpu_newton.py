@@ -224,3 +224,37 @@ def test_reduced_precision_solves(torch_cuda, oracle, cube10, mode):
     assert st.iterations <= 1.5 * st64.iterations + 10
     if prec == capi.PRECISION_MIXED:
         assert st.refinements >= 1
+
+
+def test_convert_long_rows_bitexact(torch_cuda, oracle, cube4):
+    """Rows of every length class (one wavefront, one workgroup, and the
+    device radix-sort path above 4096 triplets, which the rows of affine
+    bodies reach) convert bit for bit as the oracle does."""
+    import torch
+    print("case: convert long rows", flush=True)
+    rng = np.random.default_rng(21)
+    n = cube4.n
+    lens = {0: 9000, 1: 5000, 2: 3000, 3: 200}
+    R, Cc = [], []
+    for r, L in lens.items():
+        R.append(np.full(L, r)); Cc.append(rng.integers(r, n, L))
+    for r in range(4, n):
+        L = int(rng.integers(0, 12))
+        R.append(np.full(L, r)); Cc.append(rng.integers(r, n, L))
+    R = np.concatenate(R).astype(np.int32); Cc = np.concatenate(Cc).astype(np.int32)
+    perm = rng.permutation(len(R))                     # rows interleaved in the store
+    R, Cc = R[perm], Cc[perm]
+    V = rng.normal(size=(len(R), 9))
+    sys_ = make_system(cube4)
+    nt = len(R)
+    sys_._set_capacity(nt)
+    sys_.tr[:nt] = torch.tensor(R, dtype=torch.int32, device=sys_.dev)
+    sys_.tc[:nt] = torch.tensor(Cc, dtype=torch.int32, device=sys_.dev)
+    sys_.tv[:9 * nt] = torch.tensor(V.reshape(-1), dtype=torch.float64, device=sys_.dev)
+    sys_.nt = nt
+    K = sys_.convert()
+    kr, kc, kv = sys_.bcoo()
+    orow, ocol, oval = oracle.convert(R, Cc, V)
+    assert K == len(orow)
+    assert np.array_equal(kr, orow) and np.array_equal(kc, ocol)
+    assert np.array_equal(kv, oval)

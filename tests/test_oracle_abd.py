@@ -135,35 +135,42 @@ def test_mapping_exact_equals_dense_jthj(fixed):
                 Hv[:, 3 * i:3 * i + 3] = 0.0
     Jg = A.global_jacobian(nav, nf, nb, body_of, xbar)
     want = Jg.T @ Hv @ Jg
-    R, Cc, V = A.map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fx, exact=True)
+    R, Cc, V = A.map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fx)
     got = A.dense_upper(R, Cc, V, 4 * nb + nf)
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
     assert np.all(R <= Cc)
 
 
-def test_mapping_reference_form_drops_lower_same_body_blocks():
+def test_upper_storage_mapping_equals_reference_full_storage_procedure():
+    """The reference maps full vertex-space storage ((i, j) and (j, i) for
+    every pair, SymGH off); the library maps upper storage.  Same matrix."""
     rng = np.random.default_rng(7)
-    nb, per, nf = 2, 8, 6
-    xbar, vstart, mass, M, q, qt, vol, fx = _bodies(rng, nb, per)
+    nb, per, nf = 3, 8, 6
+    xbar, vstart, mass, M, q, qt, vol, fx = _bodies(rng, nb, per, fixed=[0, 0, 1])
     nav = nb * per
     body_of = np.repeat(np.arange(nb), per)
     rows, cols, vals = _vertex_hessian(rng, nav + nf, 60)
-    Re, Ce, Ve = A.map_contacts(rows, cols, vals, nav, nb, body_of, xbar, exact=True)
-    Rr, Cr, Vr = A.map_contacts(rows, cols, vals, nav, nb, body_of, xbar, exact=False)
-    diff = A.dense_upper(Re, Ce, Ve, 4 * nb + nf) - A.dense_upper(Rr, Cr, Vr, 4 * nb + nf)
-    # what the reference form leaves out: for each same-body pair i < j, the
-    # upper blocks of X^T (X = J_i^T H J_j)
-    miss = np.zeros_like(diff)
+    # full storage: each upper entry also as its transpose at (c, r)
+    fr, fc, fv = [], [], []
     for r, c, H in zip(rows, cols, vals):
-        if r < c < nav and body_of[r] == body_of[c] and np.any(H):
-            X = A.jacobian(xbar[r]).T @ H @ A.jacobian(xbar[c])
-            b = body_of[r]
-            U = np.triu(np.ones((4, 4)))
-            Xt = X.T * np.kron(U, np.ones((3, 3)))
-            blk = Xt + Xt.T - np.kron(np.eye(4), np.ones((3, 3))) * X   # diagonal blocks once
-            miss[12 * b:12 * b + 12, 12 * b:12 * b + 12] += blk
-    assert np.abs(miss).max() > 0
-    np.testing.assert_allclose(diff, miss, atol=1e-12 * np.abs(miss).max())
+        if not np.any(H):
+            continue
+        fr.append(r); fc.append(c); fv.append(H)
+        if r != c:
+            fr.append(c); fc.append(r); fv.append(H.T)
+    Rr, Cr, Vr = A.map_contacts_full_storage(np.array(fr), np.array(fc), np.array(fv), nav, nb, body_of, xbar, fx)
+    Ru, Cu, Vu = A.map_contacts(rows, cols, vals, nav, nb, body_of, xbar, fx)
+    nbr = 4 * nb + nf
+    want = A.dense_upper(Rr, Cr, Vr, nbr)
+    np.testing.assert_allclose(A.dense_upper(Ru, Cu, Vu, nbr), want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # and both are the dense J^T H J
+    Hv = A.dense_upper(rows, cols, vals, nav + nf)
+    for i in range(nav):
+        if fx[body_of[i]]:
+            Hv[3 * i:3 * i + 3, :] = 0.0
+            Hv[:, 3 * i:3 * i + 3] = 0.0
+    Jg = A.global_jacobian(nav, nf, nb, body_of, xbar)
+    np.testing.assert_allclose(want, Jg.T @ Hv @ Jg, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
 def test_contact_gradient_is_dense_jt_g():
