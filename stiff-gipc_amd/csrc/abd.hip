@@ -21,6 +21,9 @@
 #include "context.hpp"
 #include "pcg_dev.hpp"
 
+#include <algorithm>
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace sgipc {
 namespace {
 
@@ -281,6 +284,61 @@ __global__ void k_abd_map(long long m, const double* __restrict__ sv, const int*
         }
 }
 
+// Duplicate vertex pairs of the contact blocks are merged before the mapping
+// (the reference converts its body-body contact blocks first,
+// setup_abd_system_gradient_and_hessian.cu:430-440): one key per source
+// block, (min, max) vertex pair, or the sentinel for blocks the mapping
+// drops (all zero, or touching a fixed body).
+constexpr unsigned long long kNoPair = ~0ull;
+
+__global__ void k_abd_pair_keys(long long m, const double* __restrict__ sv, const int* __restrict__ sr,
+                                const int* __restrict__ sc, int nav, const int* __restrict__ body_of,
+                                const int* __restrict__ fixed, unsigned long long* __restrict__ key,
+                                int* __restrict__ idx) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int r = sr[t], c = sc[t];
+    double H[9];
+    for (int e = 0; e < 9; ++e) H[e] = sv[9 * t + e];
+    const bool keep = map_count(r, c, H, nav, body_of, fixed) > 0;
+    const unsigned lo = (unsigned)min(r, c), hi = (unsigned)max(r, c);
+    key[t] = keep ? (((unsigned long long)lo << 32) | hi) : kNoPair;
+    idx[t] = (int)t;
+}
+
+__global__ void k_abd_run_heads(long long m, const unsigned long long* __restrict__ key, int* __restrict__ head) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    head[i] = (key[i] != kNoPair && (i == 0 || key[i - 1] != key[i])) ? 1 : 0;
+}
+
+// each run head sums its run (ascending source index; blocks stored below
+// the diagonal transposed) into one upper vertex-space block
+__global__ void k_abd_run_sum(long long m, const unsigned long long* __restrict__ key, const int* __restrict__ idx,
+                              const int* __restrict__ head, const int* __restrict__ pos,
+                              const double* __restrict__ sv, const int* __restrict__ sr, double* __restrict__ uv,
+                              int* __restrict__ ur, int* __restrict__ uc) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || !head[i]) return;
+    const unsigned long long k = key[i];
+    const int r = (int)(k >> 32), c = (int)(k & 0xffffffffu);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long j = i; j < m && key[j] == k; ++j) {
+        const int t = idx[j];
+        const double* H = sv + 9 * (size_t)t;
+        if (sr[t] == r) {
+            for (int e = 0; e < 9; ++e) acc[e] += H[e];
+        } else {  // stored at (c, r): add its transpose
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) acc[p + 3 * q] += H[q + 3 * p];
+        }
+    }
+    const int o = pos[i];
+    for (int e = 0; e < 9; ++e) uv[9 * (size_t)o + e] = acc[e];
+    ur[o] = r;
+    uc[o] = c;
+}
+
 // grad_q[b] += sum over the body's vertices, in vertex order, of J_i^T g_i
 // (fixed bodies unchanged)
 __global__ void k_abd_grad(int nb, const int* __restrict__ vstart, const int* __restrict__ fixed,
@@ -465,25 +523,57 @@ long long abd_map_contacts(Ctx& c, long long src, long long m, long long dst) {
     if (m == 0) return 0;
     if (m > 0x7fffffffLL / 16) throw arg_error("abd_map_contacts: too many triplets");
     hipStream_t s = c.stream;
-    int* cnt = a.cnt.reserve((size_t)m + 1);
-    int* off = a.off.reserve((size_t)m + 1);
+    const double* sv = c.t_vals + 9 * src;
+    const int* sr = c.t_rows + src;
+    const int* sc = c.t_cols + src;
+    // 1. merge duplicate vertex pairs: stable radix sort of the pair keys,
+    //    run heads, fixed-order run sums
+    unsigned long long* key = a.key.reserve(2 * (size_t)m + 2);
+    int* idx = a.idx.reserve(2 * (size_t)m + 2);
+    int* head = a.cnt.reserve((size_t)m + 1);
+    int* pos = a.off.reserve((size_t)m + 1);
     int* total_dev = a.total.reserve(2);
-    hipLaunchKernelGGL(k_abd_map<0>, dim3(div_up(m, 256)), dim3(256), 0, s, m, (const double*)(c.t_vals + 9 * src),
-                       (const int*)(c.t_rows + src), (const int*)(c.t_cols + src), a.nav, a.nb, (const int*)a.body_of.p,
-                       (const int*)a.fixed.p, (const double*)a.xbar.p, cnt, (const int*)nullptr,
-                       (double*)nullptr, (int*)nullptr, (int*)nullptr);
+    hipLaunchKernelGGL(k_abd_pair_keys, dim3(div_up(m, 256)), dim3(256), 0, s, m, sv, sr, sc, a.nav,
+                       (const int*)a.body_of.p, (const int*)a.fixed.p, key, idx);
     SGIPC_LAUNCH_CHECK();
-    exclusive_scan(c, cnt, off, (int)m, total_dev, s);
+    size_t bytes = 0;
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
+    char* tmp = a.sort_tmp.reserve(bytes + 1);
+    SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
+    const unsigned long long* skey = key + m;
+    const int* sidx = idx + m;
+    hipLaunchKernelGGL(k_abd_run_heads, dim3(div_up(m, 256)), dim3(256), 0, s, m, skey, head);
+    SGIPC_LAUNCH_CHECK();
+    exclusive_scan(c, head, pos, (int)m, total_dev, s);
+    int nu = 0;
+    SGIPC_HIP(hipMemcpyAsync(&nu, total_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    if (nu == 0) return 0;
+    double* uv = a.uval.reserve(9 * (size_t)nu);
+    int* ur = a.urow.reserve((size_t)nu);
+    int* uc = a.ucol.reserve((size_t)nu);
+    hipLaunchKernelGGL(k_abd_run_sum, dim3(div_up(m, 256)), dim3(256), 0, s, m, skey, sidx, (const int*)head,
+                       (const int*)pos, sv, sr, uv, ur, uc);
+    SGIPC_LAUNCH_CHECK();
+    // 2. map the merged blocks: counts, scan, blocks
+    int* cnt = a.cnt.reserve((size_t)m + 1);   // head flags no longer needed
+    int* off = a.off.reserve((size_t)m + 1);
+    hipLaunchKernelGGL(k_abd_map<0>, dim3(div_up(nu, 256)), dim3(256), 0, s, (long long)nu, (const double*)uv,
+                       (const int*)ur, (const int*)uc, a.nav, a.nb, (const int*)a.body_of.p, (const int*)a.fixed.p,
+                       (const double*)a.xbar.p, cnt, (const int*)nullptr, (double*)nullptr, (int*)nullptr,
+                       (int*)nullptr);
+    SGIPC_LAUNCH_CHECK();
+    exclusive_scan(c, cnt, off, nu, total_dev, s);
     int total = 0;
     SGIPC_HIP(hipMemcpyAsync(&total, total_dev, sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
     if (dst + total > c.t_cap) throw capacity_error("abd_map_contacts: triplet capacity exceeded");
     if (dst < src + m && src < dst + total) throw arg_error("abd_map_contacts: output range overlaps the input range");
     if (total == 0) return 0;
-    hipLaunchKernelGGL(k_abd_map<1>, dim3(div_up(m, 256)), dim3(256), 0, s, m, (const double*)(c.t_vals + 9 * src),
-                       (const int*)(c.t_rows + src), (const int*)(c.t_cols + src), a.nav, a.nb, (const int*)a.body_of.p,
-                       (const int*)a.fixed.p, (const double*)a.xbar.p, cnt, (const int*)off,
-                       c.t_vals + 9 * dst, c.t_rows + dst, c.t_cols + dst);
+    hipLaunchKernelGGL(k_abd_map<1>, dim3(div_up(nu, 256)), dim3(256), 0, s, (long long)nu, (const double*)uv,
+                       (const int*)ur, (const int*)uc, a.nav, a.nb, (const int*)a.body_of.p, (const int*)a.fixed.p,
+                       (const double*)a.xbar.p, cnt, (const int*)off, c.t_vals + 9 * dst, c.t_rows + dst,
+                       c.t_cols + dst);
     SGIPC_LAUNCH_CHECK();
     return total;
 }

@@ -118,6 +118,11 @@ struct AbdState {
     DBuf<double> xbar, mass, vol;      // 3 nav rest coordinates; 144 nb mass matrices (row-major); nb volumes
     DBuf<double> inv;                  // 144 nb block-Jacobi inverses (row-major)
     DBuf<int> cnt, off, total;         // contact mapping counts / offsets / total
+    DBuf<unsigned long long> key;      // vertex-pair keys (double-buffered for the sort)
+    DBuf<int> idx;                     // source block ids (double-buffered)
+    DBuf<char> sort_tmp;
+    DBuf<double> uval;                 // merged vertex-space blocks
+    DBuf<int> urow, ucol;
 };
 
 struct PcgWork {

