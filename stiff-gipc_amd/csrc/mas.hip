@@ -394,7 +394,7 @@ __device__ inline int project_block(int B, int li, int lj, bool same_vertex, con
 constexpr int kProjWin = 2;
 constexpr int kProjWinLevels = 2;
 constexpr int kProjBank16 = 136 * 9;  // doubles per packed 16-vertex bank
-constexpr int kProjTopBanks = 6;
+constexpr int kProjTopBanks = 8;
 __global__ void __launch_bounds__(512) k_project_banked(
     const int* __restrict__ ustart, const int* __restrict__ rows, const int* __restrict__ cols,
     const double* __restrict__ vals, int off, int n, int L, const int* __restrict__ node_of,
