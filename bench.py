@@ -346,14 +346,10 @@ def hybrid(a):
     g = os.environ.get("SGIPC_SPMV_G", "8")
     g = g if g in ("4", "8", "16") else "8"
     kname = ("k_spmv_pap<%s, float, float>" % g) if fp32 else ("k_spmv_pap<%s, double, double>" % g)
-    cls = {"body_body": 0, "body_fem": 0, "fem_fem": 0}
     s.assemble()            # the mapped blocks, before a convert overwrites the store with the BCOO
-    r_, c_, _ = s.triplets()
-    m = s.n_mapped
-    import numpy as np
-    nb4 = 4 * s.nb
-    rb, cb = r_[:m] < nb4, c_[:m] < nb4
-    cls["body_body"] = int(np.sum(rb & cb)); cls["body_fem"] = int(np.sum(rb ^ cb)); cls["fem_fem"] = int(np.sum(~rb & ~cb))
+    m, nb4 = s.n_mapped, 4 * s.nb
+    rb, cb = s.tr[:m] < nb4, s.tc[:m] < nb4
+    cls = {"body_body": int((rb & cb).sum()), "body_fem": int((rb ^ cb).sum()), "fem_fem": int((~rb & ~cb).sum())}
     out = {
         "metric": "Newton solves/sec (hybrid: assembly + J^T H J mapping + BCOO convert + MAS/ABD build + PCG to "
                   "1e-6 true rel. residual)",

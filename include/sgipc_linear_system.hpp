@@ -99,9 +99,20 @@ public:
         int M = 0;
         check(sgipc_build_bank_map(n, h_part_id, bank, part_map.data(), cap, &M, real_map.data()), ctx_,
               "sgipc_build_bank_map");
-        check(sgipc_set_mas_topology(ctx_, n, 0, start.data(), num.data(), list.data(), len, part_map.data(), M,
-                                     real_map.data(), bank),
+        // contact pairs carry vertex ids: FEM vertices follow the body vertices
+        check(sgipc_set_mas_topology(ctx_, n, n_body_vertices_, start.data(), num.data(), list.data(), len,
+                                     part_map.data(), M, real_map.data(), bank),
               ctx_, "sgipc_set_mas_topology");
+    }
+    // Affine bodies (ABDSystem / ABDSimData, abd_system/abd_sim_data.h:40-100):
+    // call before set_mas_partition.  Body vertices come first in vertex
+    // space; FEM block rows then start at 4 * n_bodies (abd_fem_count_info).
+    void set_affine_bodies(int n_bodies, int n_body_vertices, const int* h_vstart, const double* h_xbar,
+                           const double* h_mass12, const double* h_volume, const int* h_fixed = nullptr) {
+        check(sgipc_abd_setup(ctx_, n_bodies, n_body_vertices, h_vstart, h_xbar, h_mass12, h_volume, h_fixed), ctx_,
+              "sgipc_abd_setup");
+        n_bodies_ = n_bodies;
+        n_body_vertices_ = n_body_vertices;
     }
     void set_triplet_matrix(double* d_vals, int* d_rows, int* d_cols, long long capacity) {
         check(sgipc_triplets_view(ctx_, d_vals, d_rows, d_cols, capacity), ctx_, "sgipc_triplets_view");
@@ -129,13 +140,14 @@ public:
     }
     void assemble_fem(int T, const double* d_x, const uint32_t* d_tets, const double* d_DmInv, const double* d_vol,
                       const double* d_mu, const double* d_lambda, double dt, const int* d_btype, double* d_shape) {
-        check(sgipc_assemble_fem(ctx_, T, d_x, d_tets, d_DmInv, d_vol, d_mu, d_lambda, dt, d_btype, 0,
+        check(sgipc_assemble_fem(ctx_, T, d_x, d_tets, d_DmInv, d_vol, d_mu, d_lambda, dt, d_btype, 4 * n_bodies_,
                                  trip_.global_triplet_offset, d_shape),
               ctx_, "sgipc_assemble_fem");
         trip_.global_triplet_offset += 10LL * T;
     }
     void mass_diagonal(int n, const double* d_mass) {
-        check(sgipc_mass_triplets(ctx_, n, d_mass, 0, trip_.global_triplet_offset), ctx_, "sgipc_mass_triplets");
+        check(sgipc_mass_triplets(ctx_, n, d_mass, 4 * n_bodies_, trip_.global_triplet_offset), ctx_,
+              "sgipc_mass_triplets");
         trip_.global_triplet_offset += n;
     }
     // collision detection (GIPC::buildCP, mlbvh.cu): LBVH candidates, then
@@ -186,18 +198,55 @@ public:
         trip_.global_triplet_offset += n_close;
     }
     // FEMLinearSubsystem::assemble: b = fb + shape, zero on pinned vertices
+    // (with bodies: the FEM part, fb and b from the FEM vertex / row onwards)
     void assemble_rhs(int n, const double* d_fb, const double* d_shape, const int* d_btype, double* d_b) {
         check(sgipc_fem_rhs(ctx_, n, d_fb, d_shape, d_btype, d_b), ctx_, "sgipc_fem_rhs");
+    }
+    // ---- affine bodies: ABDSystem::setup_abd_system_gradient_hessian -----
+    // the contact barrier over all vertices, written past the converted range
+    // (end of the store) and mapped into DOF space at the current offset
+    void assemble_barrier_with_bodies(int n_pairs, const int* d_pairs, const int* d_mat_index, const int counts[3],
+                                      const double* d_x, const double* d_xrest, double dhat, double kappa,
+                                      const int* d_btype, double* d_fb) {
+        const long long m = 16LL * counts[2] + 9LL * counts[1] + 4LL * counts[0];
+        const long long src = trip_.capacity - m;
+        check(sgipc_assemble_barrier(ctx_, n_pairs, d_pairs, d_mat_index, counts, d_x, d_xrest, dhat, kappa, d_btype,
+                                     0, src, d_fb),
+              ctx_, "sgipc_assemble_barrier");
+        long long n = 0;
+        check(sgipc_abd_map_contacts(ctx_, src, m, trip_.global_triplet_offset, &n), ctx_, "sgipc_abd_map_contacts");
+        trip_.global_triplet_offset += n;
+    }
+    // body gradient (the first 12 * n_bodies entries of b) and body blocks;
+    // then the contact forces of the body vertices, J^T fb
+    void assemble_bodies(const double* d_q, const double* d_q_tilde, double kappa_abd, double dt, const double* d_fb,
+                         double* d_b) {
+        check(sgipc_abd_assemble(ctx_, d_q, d_q_tilde, kappa_abd, dt, trip_.global_triplet_offset, d_b), ctx_,
+              "sgipc_abd_assemble");
+        trip_.global_triplet_offset += 10LL * n_bodies_;
+        if (d_fb) check(sgipc_abd_contact_gradient(ctx_, d_fb, d_b), ctx_, "sgipc_abd_contact_gradient");
+    }
+    // body vertex positions from q, or their step from dq (cal_x_from_q.cu)
+    void body_positions(const double* d_q, double* d_x) {
+        check(sgipc_abd_positions(ctx_, d_q, d_x), ctx_, "sgipc_abd_positions");
+    }
+    double body_energy(const double* d_q, const double* d_q_tilde, double kappa_abd, double dt) {
+        double e = 0.0;
+        check(sgipc_abd_energy(ctx_, d_q, d_q_tilde, kappa_abd, dt, &e), ctx_, "sgipc_abd_energy");
+        return e;
     }
 
     // ---- solve: GlobalLinearSystem::solve_linear_system -----------------
     // convert the assembled triplets in place, build the preconditioner, run
     // PCG; returns the iteration count (0 for an empty system).
     std::size_t solve_linear_system(const double* d_b, double* d_x) {
-        if (n_fem_ <= 0 || trip_.global_triplet_offset == 0) return 0;
+        const int rows = 4 * n_bodies_ + n_fem_;
+        if (rows <= 0 || trip_.global_triplet_offset == 0) return 0;
         int K = 0;
-        check(sgipc_convert(ctx_, trip_.global_triplet_offset, n_fem_, &K), ctx_, "sgipc_convert");
-        check(sgipc_precond_build(ctx_, precond_, 0, n_fem_, d_pairs_, n_pairs_), ctx_, "sgipc_precond_build");
+        check(sgipc_convert(ctx_, trip_.global_triplet_offset, rows, &K), ctx_, "sgipc_convert");
+        // MAS on the FEM rows; with bodies, their 12x12 block-Jacobi below
+        check(sgipc_precond_build(ctx_, precond_, 4 * n_bodies_, n_fem_, d_pairs_, n_pairs_), ctx_,
+              "sgipc_precond_build");
         sgipc_solve_opts o{cfg_.precision, cfg_.stop_rule, cfg_.global_tol_rate, cfg_.max_iter_ratio, cfg_.max_iter,
                            cfg_.check_every};
         check(sgipc_solve(ctx_, d_b, d_x, &o, &stats_), ctx_, "sgipc_solve");
@@ -241,6 +290,7 @@ private:
     sgipc_solve_stats stats_{};
     TripletMatrix trip_;
     int n_fem_ = 0;
+    int n_bodies_ = 0, n_body_vertices_ = 0;
     int precond_ = SGIPC_PRECOND_MAS;
     const int* d_pairs_ = nullptr;
     int n_pairs_ = 0;
