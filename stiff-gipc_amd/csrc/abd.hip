@@ -68,54 +68,6 @@ __device__ double shape_eval(const double* a, double* g, double* H) {
     return e;
 }
 
-// PSD projection of a symmetric 9x9 (row-major, in place): cyclic Jacobi
-// rotations until the off-diagonal mass is below 1e-30 of the total (at most
-// 30 sweeps), negative eigenvalues clamped to zero, A = V max(w, 0) V^T
-__device__ void psd_project9(double* A) {
-    double V[81];
-    for (int i = 0; i < 81; ++i) V[i] = (i % 10 == 0) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 30; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (int p = 0; p < 9; ++p)
-            for (int q = 0; q < 9; ++q) {
-                tot += A[9 * p + q] * A[9 * p + q];
-                if (p != q) off += A[9 * p + q] * A[9 * p + q];
-            }
-        if (off <= 1e-30 * tot) break;
-        for (int p = 0; p < 8; ++p)
-            for (int q = p + 1; q < 9; ++q) {
-                const double apq = A[9 * p + q];
-                if (apq == 0.0) continue;
-                const double th = (A[9 * q + q] - A[9 * p + p]) / (2.0 * apq);
-                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-                for (int k = 0; k < 9; ++k) {  // A <- A R (columns p, q)
-                    const double akp = A[9 * k + p], akq = A[9 * k + q];
-                    A[9 * k + p] = cs * akp - sn * akq;
-                    A[9 * k + q] = sn * akp + cs * akq;
-                }
-                for (int k = 0; k < 9; ++k) {  // A <- R^T A (rows p, q)
-                    const double apk = A[9 * p + k], aqk = A[9 * q + k];
-                    A[9 * p + k] = cs * apk - sn * aqk;
-                    A[9 * q + k] = sn * apk + cs * aqk;
-                }
-                for (int k = 0; k < 9; ++k) {  // V <- V R
-                    const double vkp = V[9 * k + p], vkq = V[9 * k + q];
-                    V[9 * k + p] = cs * vkp - sn * vkq;
-                    V[9 * k + q] = sn * vkp + cs * vkq;
-                }
-            }
-    }
-    double w[9];
-    for (int k = 0; k < 9; ++k) w[k] = fmax(A[10 * k], 0.0);
-    for (int i = 0; i < 9; ++i)
-        for (int j = 0; j < 9; ++j) {
-            double s = 0.0;
-            for (int k = 0; k < 9; ++k) s += V[9 * i + k] * w[k] * V[9 * j + k];
-            A[9 * i + j] = s;
-        }
-}
-
 // x_i = J_i q (also the vertex displacement J_i dq, the map being linear)
 __global__ void k_abd_positions(int nav, const int* __restrict__ body_of, const double* __restrict__ xbar,
                                 const double* __restrict__ q, double* __restrict__ x) {
@@ -127,51 +79,139 @@ __global__ void k_abd_positions(int nav, const int* __restrict__ body_of, const 
     for (int k = 0; k < 3; ++k) x[3 * (size_t)i + k] = Q[k] + Q[3 + 3 * k] * b0 + Q[4 + 3 * k] * b1 + Q[5 + 3 * k] * b2;
 }
 
-// body gradient (overwrites grad_q) and the 10 upper 3x3 blocks (column-major)
-// of the body Hessian at block rows/cols 4b .. 4b+3
-__global__ void k_abd_body(int nb, const double* __restrict__ q, const double* __restrict__ qt,
-                           const double* __restrict__ M, const double* __restrict__ vol,
-                           const int* __restrict__ fixed, double kdt2, double* __restrict__ tv, int* __restrict__ tr,
-                           int* __restrict__ tc, double* __restrict__ grad_q) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Body gradient (overwrites grad_q) and the 10 upper 3x3 blocks
+// (column-major) of the body Hessian at block rows/cols 4b .. 4b+3; one
+// 64-lane workgroup per body.  The 9x9 shape block is made PSD by cyclic
+// Jacobi rotations (fixed (p, q) order; at most 30 sweeps, stopping when the
+// off-diagonal mass is below 1e-30 of the total), negative eigenvalues
+// clamped to zero, H = V max(w, 0) V^T; each rotation updates the two
+// columns, then the two rows and V's two columns, one lane per entry.
+constexpr int kBodyThreads = 64;
+__global__ void __launch_bounds__(kBodyThreads) k_abd_body(int nb, const double* __restrict__ q,
+                                                           const double* __restrict__ qt,
+                                                           const double* __restrict__ M,
+                                                           const double* __restrict__ vol,
+                                                           const int* __restrict__ fixed, double kdt2,
+                                                           double* __restrict__ tv, int* __restrict__ tr,
+                                                           int* __restrict__ tc, double* __restrict__ grad_q) {
+    __shared__ double A[9][9], V[9][9], H[9][9];
+    __shared__ double dq[12], red[2][kBodyThreads];
+    __shared__ int flag;
+    const int b = blockIdx.x, t = threadIdx.x;
     if (b >= nb) return;
     const double* Q = q + 12 * (size_t)b;
-    const double* Qt = qt + 12 * (size_t)b;
     const double* Mb = M + 144 * (size_t)b;
     const bool fix = fixed[b] != 0;
-    double G[12], Hs[81];
-    if (fix) {
-        for (int k = 0; k < 12; ++k) G[k] = 0.0;
-        for (int k = 0; k < 81; ++k) Hs[k] = 0.0;
-    } else {
-        double dq[12], gs[9];
-        for (int k = 0; k < 12; ++k) dq[k] = Q[k] - Qt[k];
-        for (int r = 0; r < 12; ++r) {
-            double s = 0.0;
-            for (int k = 0; k < 12; ++k) s += Mb[12 * r + k] * dq[k];
-            G[r] = s;
+    const double w = kdt2 * vol[b];
+    if (t < 12) dq[t] = Q[t] - qt[12 * (size_t)b + t];
+    // shape Hessian entries, one per lane (i, j: affine rows; k, l: components)
+    double a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] = Q[3 + e];
+    for (int e = t; e < 81; e += kBodyThreads) {
+        const int r = e / 9, c = e % 9;
+        const int i = r / 3, k = r % 3, j = c / 3, l = c % 3;
+        double dii = a[3 * i] * a[3 * i] + a[3 * i + 1] * a[3 * i + 1] + a[3 * i + 2] * a[3 * i + 2];
+        double dij = a[3 * i] * a[3 * j] + a[3 * i + 1] * a[3 * j + 1] + a[3 * i + 2] * a[3 * j + 2];
+        double h;
+        if (i == j) {
+            h = (k == l ? 4.0 * (dii - 1.0) : 0.0) + 8.0 * a[3 * i + k] * a[3 * i + l];
+            for (int m = 0; m < 3; ++m)
+                if (m != i) h += 4.0 * a[3 * m + k] * a[3 * m + l];
+        } else {
+            h = (k == l ? 4.0 * dij : 0.0) + 4.0 * a[3 * j + k] * a[3 * i + l];
         }
-        shape_eval(Q + 3, gs, Hs);
-        const double w = kdt2 * vol[b];
-        for (int k = 0; k < 9; ++k) G[3 + k] += w * gs[k];
-        for (int k = 0; k < 81; ++k) Hs[k] *= w;
-        psd_project9(Hs);
+        A[r][c] = fix ? 0.0 : w * h;
+        V[r][c] = (r == c) ? 1.0 : 0.0;
     }
-    for (int k = 0; k < 12; ++k) grad_q[12 * (size_t)b + k] = G[k];
-    int t = 0;
-    for (int bi = 0; bi < 4; ++bi)
-        for (int bj = bi; bj < 4; ++bj, ++t) {
-            double* dst = tv + 9 * ((size_t)10 * b + t);
-            for (int cc = 0; cc < 3; ++cc)
-                for (int p = 0; p < 3; ++p) {
-                    const int r = 3 * bi + p, c = 3 * bj + cc;
-                    double h = Mb[12 * r + c];
-                    if (r >= 3 && c >= 3) h += Hs[9 * (r - 3) + (c - 3)];
-                    dst[p + 3 * cc] = h;
+    __syncthreads();
+    if (!fix) {
+        for (int sweep = 0; sweep < 30; ++sweep) {
+            double o = 0.0, tt = 0.0;
+            for (int e = t; e < 81; e += kBodyThreads) {
+                const double v2 = A[e / 9][e % 9] * A[e / 9][e % 9];
+                tt += v2;
+                if (e / 9 != e % 9) o += v2;
+            }
+            red[0][t] = o; red[1][t] = tt;
+            __syncthreads();
+            if (t == 0) {
+                double so = 0.0, st = 0.0;
+                for (int u = 0; u < kBodyThreads; ++u) { so += red[0][u]; st += red[1][u]; }
+                flag = so <= 1e-30 * st;
+            }
+            __syncthreads();
+            if (flag) break;
+            for (int pp = 0; pp < 8; ++pp)
+                for (int qq = pp + 1; qq < 9; ++qq) {
+                    const double apq = A[pp][qq];  // every lane reads the same value
+                    if (apq == 0.0) continue;      // uniform branch
+                    const double th = (A[qq][qq] - A[pp][pp]) / (2.0 * apq);
+                    const double tn = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                    const double cs = 1.0 / sqrt(tn * tn + 1.0), sn = tn * cs;
+                    __syncthreads();               // all lanes read A before it changes
+                    if (t < 9) {                   // A <- A R (columns pp, qq)
+                        const double akp = A[t][pp], akq = A[t][qq];
+                        A[t][pp] = cs * akp - sn * akq;
+                        A[t][qq] = sn * akp + cs * akq;
+                    }
+                    __syncthreads();
+                    if (t < 9) {                   // A <- R^T A (rows pp, qq)
+                        const double apk = A[pp][t], aqk = A[qq][t];
+                        A[pp][t] = cs * apk - sn * aqk;
+                        A[qq][t] = sn * apk + cs * aqk;
+                    } else if (t < 18) {           // V <- V R
+                        const int k = t - 9;
+                        const double vkp = V[k][pp], vkq = V[k][qq];
+                        V[k][pp] = cs * vkp - sn * vkq;
+                        V[k][qq] = sn * vkp + cs * vkq;
+                    }
+                    __syncthreads();
                 }
-            tr[10 * (size_t)b + t] = 4 * b + bi;
-            tc[10 * (size_t)b + t] = 4 * b + bj;
         }
+        for (int e = t; e < 81; e += kBodyThreads) {
+            const int i = e / 9, j = e % 9;
+            double acc = 0.0;
+            for (int k = 0; k < 9; ++k) acc += V[i][k] * fmax(A[k][k], 0.0) * V[j][k];
+            H[i][j] = acc;
+        }
+    } else {
+        for (int e = t; e < 81; e += kBodyThreads) H[e / 9][e % 9] = 0.0;
+    }
+    __syncthreads();
+    // gradient: M dq plus the shape term (free bodies), 0 for fixed ones
+    if (t < 12) {
+        double g = 0.0;
+        if (!fix) {
+            for (int k = 0; k < 12; ++k) g += Mb[12 * t + k] * dq[k];
+            if (t >= 3) {
+                const int i = (t - 3) / 3, k = (t - 3) % 3;
+                const double dii = a[3 * i] * a[3 * i] + a[3 * i + 1] * a[3 * i + 1] + a[3 * i + 2] * a[3 * i + 2];
+                double sg = 4.0 * (dii - 1.0) * a[3 * i + k];
+                for (int j = 0; j < 3; ++j)
+                    if (j != i)
+                        sg += 4.0 * (a[3 * i] * a[3 * j] + a[3 * i + 1] * a[3 * j + 1] + a[3 * i + 2] * a[3 * j + 2]) *
+                              a[3 * j + k];
+                g += w * sg;
+            }
+        }
+        grad_q[12 * (size_t)b + t] = g;
+    }
+    // 10 upper blocks x 9 entries
+    for (int e = t; e < 90; e += kBodyThreads) {
+        const int blk = e / 9, f = e % 9, p = f % 3, cc = f / 3;
+        int bi = 0, rem = blk;
+        while (rem >= 4 - bi) { rem -= 4 - bi; ++bi; }
+        const int bj = bi + rem;
+        const int r = 3 * bi + p, c = 3 * bj + cc;
+        double h = Mb[12 * r + c];
+        if (r >= 3 && c >= 3) h += H[r - 3][c - 3];
+        tv[9 * ((size_t)10 * b + blk) + f] = h;
+        if (f == 0) {
+            tr[10 * (size_t)b + blk] = 4 * b + bi;
+            tc[10 * (size_t)b + blk] = 4 * b + bj;
+        }
+    }
 }
 
 // Block (k, l) of J_i^T H J_j for a 3x3 column-major H: entry (a, c) =
@@ -390,49 +430,65 @@ __global__ void __launch_bounds__(kAbdEThreads) k_abd_energy(int nb, const doubl
 
 // 12x12 block-Jacobi inverses from the converted BCOO (block rows 4b .. 4b+3;
 // upper blocks, diagonal blocks stored whole; row r holds entries
-// ustart[r] .. ustart[r + 1], the last row ending at K)
-__global__ void k_abd_jacobi(int nb, int nbr, int K, const int* __restrict__ ustart, const int* __restrict__ cols,
-                             const double* __restrict__ vals, double* __restrict__ inv) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+// ustart[r] .. ustart[r + 1], the last row ending at K); one 64-lane
+// workgroup per body: Gauss-Jordan with partial pivoting (first largest
+// pivot; a zero pivot leaves its row scaled by 0, i.e. z = 0 along it)
+constexpr int kJacThreads = 64;
+__global__ void __launch_bounds__(kJacThreads) k_abd_jacobi(int nb, int nbr, int K, const int* __restrict__ ustart,
+                                                            const int* __restrict__ cols,
+                                                            const double* __restrict__ vals,
+                                                            double* __restrict__ inv) {
+    __shared__ double A[12][25];  // [A | I], one pad column
+    __shared__ double fac[12];
+    __shared__ int piv_s;
+    const int b = blockIdx.x, t = threadIdx.x;
     if (b >= nb) return;
-    double A[144], I[144];
-    for (int k = 0; k < 144; ++k) { A[k] = 0.0; I[k] = (k % 13 == 0) ? 1.0 : 0.0; }
+    for (int e = t; e < 12 * 24; e += kJacThreads) A[e / 24][e % 24] = (e % 24 >= 12) ? ((e / 24 == e % 24 - 12) ? 1.0 : 0.0) : 0.0;
+    __syncthreads();
     for (int k = 0; k < 4; ++k) {
         const int row = 4 * b + k;
         const int end = row + 1 < nbr ? ustart[row + 1] : K;
-        for (int e = ustart[row]; e < end; ++e) {
+        for (int e = ustart[row] + t; e < end; e += kJacThreads) {  // distinct blocks: no write conflicts
             const int l = cols[e] - 4 * b;
             if (l < k || l > 3) continue;
-            const double* B = vals + 9 * (size_t)e;  // column-major block (k, l)
+            const double* B = vals + 9 * (size_t)e;
             for (int p = 0; p < 3; ++p)
-                for (int q = 0; q < 3; ++q) {
-                    A[12 * (3 * k + p) + 3 * l + q] = B[p + 3 * q];
-                    if (l != k) A[12 * (3 * l + q) + 3 * k + p] = B[p + 3 * q];  // diagonal blocks as stored
+                for (int qq = 0; qq < 3; ++qq) {
+                    A[3 * k + p][3 * l + qq] = B[p + 3 * qq];
+                    if (l != k) A[3 * l + qq][3 * k + p] = B[p + 3 * qq];
                 }
         }
     }
-    // Gauss-Jordan with partial pivoting (a zero pivot leaves its row scaled
-    // by 0: z = 0 along that direction)
+    __syncthreads();
     for (int c = 0; c < 12; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < 12; ++r)
-            if (fabs(A[12 * r + c]) > fabs(A[12 * piv + c])) piv = r;
-        if (piv != c)
-            for (int k = 0; k < 12; ++k) {
-                double t = A[12 * c + k]; A[12 * c + k] = A[12 * piv + k]; A[12 * piv + k] = t;
-                t = I[12 * c + k]; I[12 * c + k] = I[12 * piv + k]; I[12 * piv + k] = t;
-            }
-        const double d = A[12 * c + c];
-        const double id = d != 0.0 ? 1.0 / d : 0.0;
-        for (int k = 0; k < 12; ++k) { A[12 * c + k] *= id; I[12 * c + k] *= id; }
-        for (int r = 0; r < 12; ++r) {
-            if (r == c) continue;
-            const double f = A[12 * r + c];
-            if (f == 0.0) continue;
-            for (int k = 0; k < 12; ++k) { A[12 * r + k] -= f * A[12 * c + k]; I[12 * r + k] -= f * I[12 * c + k]; }
+        if (t == 0) {
+            int piv = c;
+            for (int r = c + 1; r < 12; ++r)
+                if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+            piv_s = piv;
         }
+        __syncthreads();
+        const int piv = piv_s;
+        if (piv != c && t < 24) {
+            const double tmp = A[c][t];
+            A[c][t] = A[piv][t];
+            A[piv][t] = tmp;
+        }
+        __syncthreads();
+        const double d = A[c][c];
+        const double id = d != 0.0 ? 1.0 / d : 0.0;
+        __syncthreads();
+        if (t < 24) A[c][t] *= id;
+        __syncthreads();
+        if (t < 12) fac[t] = A[t][c];
+        __syncthreads();
+        for (int e = t; e < 12 * 24; e += kJacThreads) {
+            const int r = e / 24, k = e % 24;
+            if (r != c) A[r][k] -= fac[r] * A[c][k];
+        }
+        __syncthreads();
     }
-    for (int k = 0; k < 144; ++k) inv[144 * (size_t)b + k] = I[k];
+    for (int e = t; e < 144; e += kJacThreads) inv[144 * (size_t)b + e] = A[e / 12][12 + e % 12];
 }
 
 // z = P_b^-1 r on the body rows (standalone apply)
@@ -502,7 +558,7 @@ void abd_assemble(Ctx& c, const double* q, const double* qt, double kappa, doubl
     const AbdState& a = abd_ready(c, "abd_assemble");
     if (!c.t_vals) throw state_error("abd_assemble: no triplet view set");
     if (toff < 0 || toff + 10LL * a.nb > c.t_cap) throw capacity_error("abd_assemble: triplet capacity exceeded");
-    hipLaunchKernelGGL(k_abd_body, dim3(div_up(a.nb, 64)), dim3(64), 0, c.stream, a.nb, q, qt,
+    hipLaunchKernelGGL(k_abd_body, dim3(a.nb), dim3(kBodyThreads), 0, c.stream, a.nb, q, qt,
                        (const double*)a.mass.p, (const double*)a.vol.p, (const int*)a.fixed.p, kappa * dt * dt,
                        c.t_vals + 9 * toff, c.t_rows + toff, c.t_cols + toff, grad_q);
     SGIPC_LAUNCH_CHECK();
@@ -594,7 +650,7 @@ void abd_jacobi_build(Ctx& c) {
     if (!a.ready) return;
     if (!c.have_bcoo) throw state_error("abd preconditioner: convert first");
     if (c.nb < 4 * a.nb) throw state_error("abd preconditioner: the system has fewer block rows than the bodies");
-    hipLaunchKernelGGL(k_abd_jacobi, dim3(div_up(a.nb, 64)), dim3(64), 0, c.stream, a.nb, c.nb, c.K, (const int*)c.uniq_start.p,
+    hipLaunchKernelGGL(k_abd_jacobi, dim3(a.nb), dim3(kJacThreads), 0, c.stream, a.nb, c.nb, c.K, (const int*)c.uniq_start.p,
                        (const int*)c.t_cols, (const double*)c.t_vals, a.inv.p);
     SGIPC_LAUNCH_CHECK();
     a.inv_ready = true;
