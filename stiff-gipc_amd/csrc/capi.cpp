@@ -82,6 +82,7 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
             int v = std::atoi(g);
             if (v == 4 || v == 8 || v == 16) c->spmv_group = v;
         }
+        if (const char* f = std::getenv("SGIPC_SPMV_FULL32")) c->spmv_full32 = std::atoi(f) != 0;
     });
     if (rc != SGIPC_OK) { delete c; return rc; }
     *out = c;

@@ -23,6 +23,12 @@ struct SymBcoo {
     DBuf<int2> ent;     // nlow + K
     DBuf<float> valf;   // 12 * K FP32 copy of the BCOO values, padded columns (mixed precision)
     bool valf_ok = false;
+    // row-contiguous FP32 copy of both triangles in entry order (mixed
+    // precision): entry e of ent as its block, transposed where it is a
+    // transposed reference, 10 floats (9 column-major + pad); its column
+    DBuf<float> valfull;
+    DBuf<int> colfull;
+    bool full_ok = false;
 };
 
 // Sort of the static (FEM + mass) triplet segment, built where it started at
@@ -125,8 +131,23 @@ struct AbdState {
     DBuf<int> urow, ucol;
 };
 
+// Pinned, device-mapped host words the PCG publishes its progress to, so the
+// host follows the solve without copies or stream synchronisation.
+struct PinnedPoll {
+    int* h = nullptr;  // host view: [0] stop iteration, [1] iteration, [2] publication count; double at [8]
+    int* d = nullptr;  // device view of the same memory
+    bool failed = false;
+    PinnedPoll() = default;
+    PinnedPoll(const PinnedPoll&) = delete;
+    PinnedPoll& operator=(const PinnedPoll&) = delete;
+    ~PinnedPoll() {
+        if (h) (void)hipHostFree(h);
+    }
+};
+
 struct PcgWork {
     DBuf<double> r, z, p, Ap, xw;   // 3N each
+    PinnedPoll poll;
     DBuf<double> r64b, d64;         // mixed precision: FP64 residual and correction
     PcgGraph graph64, graph32, graph_mixed;
     DBuf<double> partials;          // per-workgroup partial dot products
@@ -189,6 +210,7 @@ struct Ctx {
     SymBcoo bsr;
     bool have_bsr = false;
     int spmv_group = 8;  // lanes per block row in the PCG SpMV (env SGIPC_SPMV_G: 4, 8, 16)
+    int spmv_full32 = 1; // mixed precision: row-contiguous FP32 copy of both triangles (env SGIPC_SPMV_FULL32=0: upper BCOO)
     int deterministic = 0;  // 1: MAS projection and barrier gradient summed in a fixed order (bitwise repeatable)
     DBuf<double> grad_scr;               // ordered barrier gradient: per-pair rows (12 doubles per pair)
     DBuf<unsigned> grad_key, grad_rec;   // (vertex, record) pairs, double-buffered for the sort
@@ -243,6 +265,7 @@ double ground_step_bound(Ctx& c, int m, const unsigned* ids, const double* x, co
 int bcoo_convert(Ctx& c, long long nt, int nblockrows);
 void bsr_build(Ctx& c);
 const float* bsr_values_f32(Ctx& c);
+const float* bsr_full_f32(Ctx& c, const int** cols);
 void spmv64(Ctx& c, double a, const double* x, double b, double* y);
 
 double kinetic_energy(Ctx& c, int n, const double* x, const double* xt, const double* m);

@@ -15,6 +15,8 @@
 // rz after the x/r update).
 #include "spmv_sym.hpp"
 #include "pcg_dev.hpp"
+#include <atomic>
+#include <chrono>
 #include <cmath>
 
 namespace sgipc {
@@ -227,6 +229,89 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
     write_partial(v, part_rz, sh);
 }
 
+// The same with the row-contiguous FP32 copy of both triangles: the G lanes
+// of a row read consecutive entries (five 8-byte loads each), no block
+// index, no transposition in registers; same entry order, so the same sums.
+template <int G, typename V>
+__global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* __restrict__ eptr,
+                                                            const int* __restrict__ colf,
+                                                            const float* __restrict__ vf, const V* __restrict__ z,
+                                                            V* __restrict__ p, V* __restrict__ Ap,
+                                                            const double* __restrict__ rz_last,
+                                                            const double* __restrict__ rz_prev, int np_rz,
+                                                            double* __restrict__ part, int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
+    const int iter = ctrl[C_ITER];
+    if (iter >= ctrl[C_MAXIT]) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
+        return;
+    }
+    __shared__ double sh[kThreads / 64];
+    const bool first = iter == 1;
+    double be = 0.0;
+    if (!first) {
+        const double rn = sum_partials(rz_last, np_rz, sh);
+        const double ro = sum_partials(rz_prev, np_rz, sh);
+        be = rn / ro;
+    }
+    constexpr int RPB = kThreads / G;
+    const int nchunks = (nb + RPB - 1) / RPB;
+    ChunkWalk w = xcd_chunks(nchunks);
+    double contrib = 0.0;
+    for (int ch = w.first; ch < w.end; ch += w.step) {
+        int row = ch * RPB + threadIdx.x / G, sub = threadIdx.x % G;
+        float s0 = 0, s1 = 0, s2 = 0;
+        if (row < nb) {
+            const int end = eptr[row + 1];
+            int e = eptr[row] + sub;
+            for (; e + G < end; e += 2 * G) {  // two entries in flight
+                const float2* A = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)e);
+                const float2* B = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)(e + G));
+                const float2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
+                const float2 b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4];
+                const int ca = colf[e], cb = colf[e + G];
+                const float x0 = (float)z[3 * (size_t)ca], x1 = (float)z[3 * (size_t)ca + 1],
+                            x2 = (float)z[3 * (size_t)ca + 2];
+                const float y0 = (float)z[3 * (size_t)cb], y1 = (float)z[3 * (size_t)cb + 1],
+                            y2 = (float)z[3 * (size_t)cb + 2];
+                s0 += a0.x * x0 + a1.y * x1 + a3.x * x2;
+                s1 += a0.y * x0 + a2.x * x1 + a3.y * x2;
+                s2 += a1.x * x0 + a2.y * x1 + a4.x * x2;
+                s0 += b0.x * y0 + b1.y * y1 + b3.x * y2;
+                s1 += b0.y * y0 + b2.x * y1 + b3.y * y2;
+                s2 += b1.x * y0 + b2.y * y1 + b4.x * y2;
+            }
+            if (e < end) {
+                const float2* A = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)e);
+                const float2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
+                const int ca = colf[e];
+                const float x0 = (float)z[3 * (size_t)ca], x1 = (float)z[3 * (size_t)ca + 1],
+                            x2 = (float)z[3 * (size_t)ca + 2];
+                s0 += a0.x * x0 + a1.y * x1 + a3.x * x2;
+                s1 += a0.y * x0 + a2.x * x1 + a3.y * x2;
+                s2 += a1.x * x0 + a2.y * x1 + a4.x * x2;
+            }
+        }
+        group_sum3<G, float>(s0, s1, s2);
+        if (sub < 3 && row < nb) {
+            const size_t o = 3 * (size_t)row + sub;
+            const V aw = (V)(sub == 0 ? s0 : (sub == 1 ? s1 : s2));
+            V pq, aq;
+            if (first) {
+                pq = z[o];
+                aq = aw;
+            } else {
+                pq = (V)((double)z[o] + be * (double)p[o]);
+                aq = (V)((double)aw + be * (double)Ap[o]);
+            }
+            p[o] = pq;
+            Ap[o] = aq;
+            contrib += (double)pq * (double)aq;
+        }
+    }
+    write_partial(contrib, part, sh);
+}
+
 template <typename TD, typename TS>
 __global__ void k_copy_cast(long long N, TD* __restrict__ dst, const TS* __restrict__ src) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
@@ -238,11 +323,28 @@ __global__ void k_axpy(long long N, double* __restrict__ y, double a, const doub
         y[i] += a * x[i];
 }
 
+constexpr int C_PUB = C_NUM + 2;  // publications of the progress words (pinned polling)
+
 __global__ void k_ctrl_init(int* ctrl, int maxit) {
     ctrl[C_STOP] = 0;
     ctrl[C_ITER] = 1;
     ctrl[C_MAXIT] = maxit;
     ctrl[C_NOCONV] = 0;
+    ctrl[C_PUB] = 0;
+}
+
+// Last kernel of a captured group of iterations: copies the stop and
+// iteration words and the progress ratio to pinned host memory, then (after
+// a system-scope fence) the publication count the host waits on.
+__global__ void k_publish(int* __restrict__ ctrl, const double* __restrict__ sc, volatile int* hp,
+                          volatile double* hq) {
+    const int pub = ctrl[C_PUB] + 1;
+    ctrl[C_PUB] = pub;
+    hq[0] = sc[S_PROG];
+    hp[0] = ctrl[C_STOP];
+    hp[1] = ctrl[C_ITER];
+    __threadfence_system();
+    hp[2] = pub;
 }
 
 __global__ void k_set_scalar(double* sc, int idx, double v) { sc[idx] = v; }
@@ -294,6 +396,9 @@ struct Engine {
     int gU = 1;  // workgroups (and r.r partials) of the x/r update
 
     int G = 8;  // lanes per block row in the SpMV (4, 8 or 16)
+    // mixed precision: the row-contiguous FP32 copy (null: the upper BCOO)
+    const float* vfull = nullptr;
+    const int* cfull = nullptr;
     static constexpr bool kFp32Mas = sizeof(T) == 4;
 
     // z = M^-1 r fused with the r.z partials (gN workgroups) and the stop test
@@ -323,7 +428,15 @@ struct Engine {
             hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val, (const V*)z, p, Ap,
                                (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
         };
-        if (G == 4) launch(k_spmv_pap<4, MV, V>);
+        if (vfull) {
+            auto launchf = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, cfull, vfull, (const V*)z, p,
+                                   Ap, (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
+            };
+            if (G == 4) launchf(k_spmv_pap_full<4, V>);
+            else if (G == 16) launchf(k_spmv_pap_full<16, V>);
+            else launchf(k_spmv_pap_full<8, V>);
+        } else if (G == 4) launch(k_spmv_pap<4, MV, V>);
         else if (G == 16) launch(k_spmv_pap<16, MV, V>);
         else launch(k_spmv_pap<8, MV, V>);
         SGIPC_LAUNCH_CHECK();
@@ -379,7 +492,7 @@ struct Engine {
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
-        mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c));
+        mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c)); mix((size_t)vfull); mix((size_t)cfull);
         if (mas) {
             const MasState& m = c.mas;
             mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
@@ -402,6 +515,24 @@ struct Engine {
         // profiling records events per kernel, which a captured graph cannot
         const bool use_graph = !c.prof.on;
         int pairs_per_check = std::max(1, check_every / 2);
+        PinnedPoll& pp = c.pcg.poll;
+        if (use_graph && !pp.h && !pp.failed) {
+            void* hptr = nullptr;
+            if (hipHostMalloc(&hptr, 64, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+                void* dptr = nullptr;
+                if (hipHostGetDevicePointer(&dptr, hptr, 0) == hipSuccess) {
+                    pp.h = static_cast<int*>(hptr);
+                    pp.d = static_cast<int*>(dptr);
+                } else {
+                    (void)hipHostFree(hptr);
+                    pp.failed = true;
+                }
+            } else {
+                pp.failed = true;
+            }
+            (void)hipGetLastError();
+        }
+        const bool pinned = use_graph && pp.h;
         if (use_graph) {
             size_t key = graph_key();
             if (!gcache.exec || gcache.key != key) {
@@ -411,6 +542,11 @@ struct Engine {
                 try {
                     enqueue_iteration(1);
                     enqueue_iteration(0);
+                    if (pinned) {
+                        hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, ctrl, (const double*)sc, pp.d,
+                                           reinterpret_cast<double*>(pp.d + 8));
+                        SGIPC_LAUNCH_CHECK();
+                    }
                 } catch (...) {
                     hipGraph_t dead = nullptr;
                     (void)hipStreamEndCapture(s, &dead);
@@ -425,7 +561,43 @@ struct Engine {
             }
         }
         int hc[C_NUM] = {0};
-        if (use_graph) {
+        if (pinned) {
+            // Keep two groups of iterations queued ahead of the device and
+            // follow its published progress in pinned memory: the device never
+            // waits for the host, and at most two groups run past the stop
+            // (their kernels return at once).
+            volatile int* hp = pp.h;
+            hp[0] = 0; hp[1] = 0; hp[2] = 0;
+            std::atomic_thread_fence(std::memory_order_seq_cst);
+            const int max_groups = maxit / 2 + 2;
+            constexpr int kAhead = 2;
+            int launched = 0, last_done = -1;
+            auto last_change = std::chrono::steady_clock::now();
+            for (;;) {
+                const int done = hp[2];
+                std::atomic_thread_fence(std::memory_order_acquire);
+                if (done > 0 && hp[0] != 0) break;  // stopped (converged or budget)
+                if (launched < max_groups && launched - done < kAhead) {
+                    SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
+                    ++launched;
+                    continue;
+                }
+                if (done >= launched) break;        // all launched groups ran, none stopped: budget used up
+                if (done != last_done) {
+                    last_done = done;
+                    last_change = std::chrono::steady_clock::now();
+                } else if (std::chrono::steady_clock::now() - last_change > std::chrono::seconds(2)) {
+                    // no progress for 2 s: let the stream report its state
+                    const hipError_t q = hipStreamQuery(s);
+                    if (q != hipSuccess && q != hipErrorNotReady) SGIPC_HIP(q);
+                    if (q == hipSuccess) break;     // idle: read the words below
+                    last_change = std::chrono::steady_clock::now();
+                }
+                __builtin_ia32_pause();
+            }
+            SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+            SGIPC_HIP(hipStreamSynchronize(s));
+        } else if (use_graph) {
             // Poll the stop word; after two polls the next one is placed where
             // the measured convergence rate predicts the stop (at most twice
             // the nominal interval ahead), so few polls are needed and few
@@ -583,6 +755,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
                                       c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
         E.G = G;
+        if (c.spmv_full32) E.vfull = bsr_full_f32(c, &E.cfull);
         fill<double>(x, 0.0, N, s);
         hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();

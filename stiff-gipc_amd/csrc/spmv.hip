@@ -94,6 +94,25 @@ __global__ void k_to_f32(long long K, const double* __restrict__ s, float* __res
     }
 }
 
+// entry-ordered FP32 copy: one thread per block row
+__global__ void k_full_f32(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
+                           const double* __restrict__ vals, int* __restrict__ colf, float* __restrict__ vf) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nb) return;
+    for (int e = eptr[r]; e < eptr[r + 1]; ++e) {
+        const int2 q = ent[e];
+        const double* A = vals + 9 * (size_t)q.x;
+        float* d = vf + kF32FullStride * (size_t)e;
+        const bool tr = q.y < r;
+#pragma unroll
+        for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) d[pp + 3 * qq] = (float)(tr ? A[qq + 3 * pp] : A[pp + 3 * qq]);
+        d[9] = 0.0f;
+        colf[e] = q.y;
+    }
+}
+
 }  // namespace
 
 // Build the transposed-reference structure for the BCOO now resident in the
@@ -133,7 +152,25 @@ void bsr_build(Ctx& c) {
                        (const int2*)lref, (const int*)c.uniq_start.p, (const int*)c.t_cols, eptr, ent);
     SGIPC_LAUNCH_CHECK();
     S.valf_ok = false;
+    S.full_ok = false;
     c.have_bsr = true;
+}
+
+// Row-contiguous FP32 copy of both triangles (mixed-precision SpMV)
+const float* bsr_full_f32(Ctx& c, const int** cols) {
+    if (!c.have_bsr) bsr_build(c);
+    SymBcoo& S = c.bsr;
+    const size_t ne = (size_t)S.nlow + (size_t)S.K;
+    float* vf = S.valfull.reserve(kF32FullStride * ne + 4);
+    int* cf = S.colfull.reserve(ne + 1);
+    if (!S.full_ok && S.nb > 0) {
+        hipLaunchKernelGGL(k_full_f32, dim3(div_up(S.nb, 256)), dim3(256), 0, c.stream, S.nb, (const int*)S.eptr.p,
+                           (const int2*)S.ent.p, (const double*)c.t_vals, cf, vf);
+        SGIPC_LAUNCH_CHECK();
+    }
+    S.full_ok = true;
+    *cols = cf;
+    return vf;
 }
 
 // FP32 copy of the BCOO values for the mixed-precision solve

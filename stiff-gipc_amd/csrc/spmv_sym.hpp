@@ -40,6 +40,7 @@ __device__ inline ChunkWalk xcd_chunks(int nchunks) {
 // (48 B per block, columns in float4 lanes 0-2), three aligned 16-byte loads.
 typedef double double2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 constexpr int kF32BlockStride = 12;  // floats per block in the FP32 copy
+constexpr int kF32FullStride = 10;   // floats per entry in the row-contiguous FP32 copy (five aligned float2)
 
 template <typename TV>
 __device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, TV a[9]);
