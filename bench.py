@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=40)
     ap.add_argument("--bank", type=int, choices=[16, 32], default=16, help="MAS subdomain size (vertices)")
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "fp32apply", "both"], default="both",
+                    help="both: every form of the workload (contact: fp64 + mixed; hybrid: fp64, fp32apply, mixed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["contact", "cantilever_adjoint", "batched", "hybrid"], default="contact",
                     help="contact: the Newton-step benchmark (default); cantilever_adjoint: the differentiable "
@@ -245,8 +246,10 @@ def main():
     modes = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     res = {}
     for m in modes:
-        res[m] = run_mode(sysm, capi.PRECISION_FP64 if m == "fp64" else capi.PRECISION_MIXED, a.steps, a.warmup)
-    main_mode = "mixed" if "mixed" in res else "fp64"
+        sysm.precond = capi.PRECOND_MAS_FP32 if m == "fp32apply" else capi.PRECOND_MAS
+        res[m] = run_mode(sysm, capi.PRECISION_MIXED if m == "mixed" else capi.PRECISION_FP64, a.steps, a.warmup)
+    sysm.precond = capi.PRECOND_MAS
+    main_mode = "mixed" if "mixed" in res else modes[0]
     r = res[main_mode]
 
     # roofline of the dominant kernel (PCG SpMV) of the reported mode
@@ -286,7 +289,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64" if fp32 else "f64",
+        "dtype": "f64" if main_mode == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {
             "workload": "two stacked soft boxes, 2 x %d^3 cells, 5 tets/cell, %d vertices, %d tets, gap 0.5 dhat"
@@ -335,10 +338,18 @@ def hybrid(a):
     hs = scene.hybrid_scene()
     s = HybridSystem(hs, "cuda:0")
     s.detect()   # LBVH broad phase + narrow phase once per iterate, outside the timed hot path
-    modes = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
-    res = {m: run_mode(s, capi.PRECISION_FP64 if m == "fp64" else capi.PRECISION_MIXED, a.steps, a.warmup)
-           for m in modes}
-    main_mode = "mixed" if "mixed" in res else "fp64"
+    # fp64: FP64 PCG + FP64 MAS; fp32apply: FP64 PCG with the MAS stored and
+    # applied in FP32; mixed: FP32 inner PCG with FP64 refinement
+    forms = {"fp64": (capi.PRECISION_FP64, capi.PRECOND_MAS),
+             "fp32apply": (capi.PRECISION_FP64, capi.PRECOND_MAS_FP32),
+             "mixed": (capi.PRECISION_MIXED, capi.PRECOND_MAS)}
+    modes = list(forms) if a.precision == "both" else [a.precision]
+    res = {}
+    for m in modes:
+        s.precond = forms[m][1]
+        res[m] = run_mode(s, forms[m][0], a.steps, a.warmup)
+    s.precond = capi.PRECOND_MAS
+    main_mode = "fp32apply" if "fp32apply" in res else modes[0]
     r = res[main_mode]
     fp32 = main_mode == "mixed"
     alg = spmv_alg_bytes(s.K, s.nbr, fp32)
@@ -355,11 +366,14 @@ def hybrid(a):
                   "1e-6 true rel. residual)",
         "value": round(r["value"], 4), "unit": "Newton solves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64" if fp32 else "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if main_mode == "fp64" else "f32/f64",
+        "data": "synthetic",
         "config": {"workload": "hybrid: %d affine cubes on a 100x20x100-cell slab, %d slab vertices, %d tets, "
                                "%d block rows" % (s.nb, s.nf, s.T, s.nbr),
                    "contact_pairs": int(s.n_pairs), "mapped_blocks": cls, "upper_blocks": int(s.K),
-                   "precision": main_mode, "preconditioner": "MAS (16-vertex banks) on slab rows + 12x12 "
+                   "precision": main_mode + (" (FP64 PCG, MAS stored and applied in FP32)"
+                                             if main_mode == "fp32apply" else ""),
+                   "preconditioner": "MAS (16-vertex banks) on slab rows + 12x12 "
                    "block-Jacobi on body rows", "stop": "true relative residual 1e-6",
                    "parallelism": "single GPU"},
         "pcg_iters_per_step": r["iters"], "pcg_iters_per_s": round(r["pcg_iters_per_s"], 2),
@@ -370,13 +384,17 @@ def hybrid(a):
         "cpu_baseline": None,
         "cpu_baseline_note": "not measured for this workload: the oracle's C PCG has no body block-Jacobi",
     }
-    if main_mode == "mixed" and "fp64" in res:
-        f = res["fp64"]
-        out["fp64"] = {"value": round(f["value"], 4), "ms_per_step": round(f["ms_per_step"], 4),
-                       "pcg_iters_per_step": f["iters"], "pcg_iters_per_s": round(f["pcg_iters_per_s"], 2),
-                       "rel_residual_max": f["rel_max"], "phase_ms": f["phase_ms"],
-                       "spmv_avg_ms": round(f["spmv_ms"], 5)}
-        out["mixed_over_fp64"] = round(r["value"] / f["value"], 4)
+    for m in res:
+        if m == main_mode:
+            continue
+        f = res[m]
+        out[m] = {"value": round(f["value"], 4), "ms_per_step": round(f["ms_per_step"], 4),
+                  "pcg_iters_per_step": f["iters"], "pcg_iters_per_s": round(f["pcg_iters_per_s"], 2),
+                  "rel_residual_max": f["rel_max"], "phase_ms": f["phase_ms"], "spmv_avg_ms": round(f["spmv_ms"], 5)}
+    if "fp64" in res and main_mode != "fp64":
+        out["%s_over_fp64" % main_mode] = round(r["value"] / res["fp64"]["value"], 4)
+    if "mixed" in res and "fp64" in res:
+        out["mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 4)
     print(json.dumps(out))
 
 
