@@ -276,8 +276,13 @@ def main():
 
     g = os.environ.get("SGIPC_SPMV_G", "8")
     g = g if g in ("4", "8", "16") else "8"   # lanes per block row, as the library reads it
-    kname = ("k_spmv_pap<%s, float, float> (PCG SpMV + direction update, FP32 values)" % g) if fp32 else \
-        ("k_spmv_pap<%s, double, double> (PCG SpMV + direction update, FP64)" % g)
+    full32 = os.environ.get("SGIPC_SPMV_FULL32", "1") != "0"   # the library's default for mixed solves
+    if fp32 and full32:
+        kname = "k_spmv_pap_full<%s, float> (PCG SpMV + direction update, row-contiguous FP32 copy)" % g
+    elif fp32:
+        kname = "k_spmv_pap<%s, float, float> (PCG SpMV + direction update, FP32 upper BCOO)" % g
+    else:
+        kname = "k_spmv_pap<%s, double, double> (PCG SpMV + direction update, FP64)" % g
     out = {
         "metric": "Newton solves/sec (assembly + BCOO convert + MAS build + MAS-PCG to 1e-6 true rel. residual)",
         "value": round(r["value"], 4),
@@ -356,7 +361,9 @@ def hybrid(a):
     achieved = alg / (r["spmv_ms"] * 1e-3) / 1e9
     g = os.environ.get("SGIPC_SPMV_G", "8")
     g = g if g in ("4", "8", "16") else "8"
-    kname = ("k_spmv_pap<%s, float, float>" % g) if fp32 else ("k_spmv_pap<%s, double, double>" % g)
+    full32 = os.environ.get("SGIPC_SPMV_FULL32", "1") != "0"
+    kname = (("k_spmv_pap_full<%s, float>" if full32 else "k_spmv_pap<%s, float, float>") % g) if fp32 else \
+        ("k_spmv_pap<%s, double, double>" % g)
     s.assemble()            # the mapped blocks, before a convert overwrites the store with the BCOO
     m, nb4 = s.n_mapped, 4 * s.nb
     rb, cb = s.tr[:m] < nb4, s.tc[:m] < nb4
