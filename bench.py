@@ -167,6 +167,36 @@ def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5):
     return total, steps, it
 
 
+def oracle_ground_step(O, sc, topo, tol):
+    """One Newton step of a scene over the ground plane with the oracle's FP64
+    C restatement (ground detection + barrier, kinetic + FEM, mass, convert,
+    MAS build, MAS-PCG to `tol` true rel. residual); returns PCG iterations."""
+    import numpy as np
+    m = sc.meta
+    nrm, off = m["ground_normal"], float(m["ground_offset"])
+    gp = O.ground_detect(m["surf"], sc.x, nrm, off, sc.dhat)
+    fb, gv, gr, gc = O.ground_assemble(gp, sc.x, nrm, off, sc.dhat, sc.kappa, sc.btype)
+    shape = O.kinetic_gradient(sc.x, sc.x_tilde, sc.mass)
+    grad, vals, rows, cols = O.fem_assemble(sc.x, sc.tets, sc.DmInv, sc.vol, sc.mu, sc.lam, sc.dt, sc.btype, 0,
+                                            shape.copy())
+    mv, mr, mc = O.mass_triplets(sc.mass)
+    kr, kc, kv = O.convert(np.concatenate([gr, rows, mr]), np.concatenate([gc, cols, mc]),
+                           np.concatenate([gv, vals, mv]))
+    b = O.fem_rhs(fb.reshape(-1, 3), grad, sc.btype).reshape(-1)
+    ns, nn, nl, pm, rm = topo
+    mas = O.Mas(sc.n, pm, rm, ns, nn, nl, sc.bank)
+    mas.build(kr, kc, kv, fp32_store=False)
+    x, it, rr = O.pcg(kr, kc, kv, sc.n, b, precond=2, mas=mas, tol=tol, rule=1, max_iter=100000)
+    del mas
+    return it
+
+
+def oracle_topology(O, sc):
+    ns, nn, nl = O.vertex_neighbors(sc.n, sc.tets)
+    pm, rm = O.bank_map(sc.part, sc.bank)
+    return ns, nn, nl, pm, rm
+
+
 def run_mode(sysm, precision, steps, warmup):
     """Warm-up, then exactly `steps` timed Newton steps; then two steps with
     the library's HIP-event timing for the kernel figures."""
@@ -450,6 +480,26 @@ def cantilever_adjoint(a):
     t_fd = time.perf_counter() - t0
     rels = [r[2] for r in res]
     gnorm = float(torch.sqrt((gE * gE).sum() + (gv * gv).sum()))
+    cpu = None
+    if not a.no_cpu_baseline:
+        # the oracle (FP64 C restatement, one thread): the first Newton
+        # iteration of step 1 (ground barrier, SNK1 with the per-tet E, MAS-PCG
+        # to the same tolerance), repeated for about 10 s
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        print("  CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        topo = oracle_topology(O, sc)
+        n_it, secs, pits = 0, 0.0, 0
+        while n_it < 5 and (n_it == 0 or secs < 10.0):
+            t0 = time.perf_counter()
+            pits = oracle_ground_step(O, sc, topo, a.pcg_tol)
+            secs += time.perf_counter() - t0
+            n_it += 1
+        cpu = {"value": round(n_it / secs, 4), "unit": "Newton iterations/s", "cores": 1, "kind": "port",
+               "sample": "%d x the first Newton iteration of step 1 (ground barrier + SNK1 assembly, convert, MAS "
+                         "build, MAS-PCG to %g true rel. residual: %d iterations), oracle C restatement, 1 thread, "
+                         "%.1f s" % (n_it, a.pcg_tol, pits, secs),
+               "gpu_forward_newton_iterations_per_s": round(fwd_stats["newton"] / t_fwd, 3)}
     out = {
         "metric": "differentiable rollout: adjoint gradient vs central FD (max rel. error) and wall time",
         "value": max(rels) if rels else None,
@@ -481,6 +531,7 @@ def cantilever_adjoint(a):
         "forward_plus_adjoint_stats": grad_stats,
         "ground_contacts_at_T": contacts,
         "fd": [dict(zip(("fd", "adjoint", "rel", "fd_richardson", "rel_richardson"), r)) for r in res],
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -525,6 +576,27 @@ def batched(a):
     torch.cuda.synchronize()
     t_s = (time.perf_counter() - t0) / reps
     sits = np.array([r[0] for r in sres])
+    cpu = None
+    if not a.no_cpu_baseline:
+        # the oracle (FP64 C restatement, one thread) on the first scenes in
+        # order, until about 10 s of CPU work; topology setup untimed
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        print("  CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        done, secs, nverts = 0, 0.0, 0
+        for sc_i in scs:
+            topo = oracle_topology(O, sc_i)
+            t0 = time.perf_counter()
+            oracle_ground_step(O, sc_i, topo, 1e-6)
+            secs += time.perf_counter() - t0
+            done += 1
+            nverts += sc_i.n
+            if secs >= 10.0:
+                break
+        cpu = {"value": round(done / secs, 4), "unit": "scene Newton solves/s", "cores": 1, "kind": "port",
+               "sample": "the first %d of the %d scenes (%d vertices), one FP64 Newton step each (ground barrier + "
+                         "FEM assembly, convert, MAS build, MAS-PCG to 1e-6 true rel. residual), oracle C "
+                         "restatement, 1 thread, %.1f s" % (done, len(scs), nverts, secs)}
     out = {
         "metric": "batched Newton steps: scenes per second (each scene to 1e-6 true rel. residual)",
         "value": round(bn.S / t_b, 3),
@@ -549,6 +621,7 @@ def batched(a):
                                                     "max": int(sits.max())},
                             "rel_residual_max": max(r[1] for r in sres)},
         "batched_over_sequential": round(t_s / t_b, 3),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
