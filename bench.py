@@ -29,8 +29,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=40)
     ap.add_argument("--bank", type=int, choices=[16, 32], default=16, help="MAS subdomain size (vertices)")
     ap.add_argument("--precision", choices=["fp64", "mixed", "fp32apply", "both"], default="both",
@@ -275,9 +275,10 @@ def run_mode(sysm, precision, steps, warmup):
     for _ in range(warmup):
         sysm.newton_step(**solve_kw)
     torch.cuda.synchronize()
-    iters, pcg_s, rel = 0, 0.0, []
+    iters, pcg_s, rel, per = 0, 0.0, [], []
     t0 = time.perf_counter()
     for _ in range(steps):
+        ta = time.perf_counter()
         sysm.assemble()
         sysm.convert()
         sysm.build_precond()
@@ -286,8 +287,11 @@ def run_mode(sysm, precision, steps, warmup):
         pcg_s += time.perf_counter() - ts
         iters += st.iterations
         rel.append(st.rel_residual)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        per.append(time.perf_counter() - ta)
     secs = time.perf_counter() - t0
+    per.sort()
+    dx = sysm.dx.detach().clone()
 
     ctx = sysm.ctx
     ctx.call("sgipc_profile_enable", 1)
@@ -307,7 +311,76 @@ def run_mode(sysm, precision, steps, warmup):
     ctx.call("sgipc_profile_enable", 0)
     return dict(value=steps / secs, ms_per_step=1e3 * secs / steps, iters=iters / steps,
                 pcg_iters_per_s=iters / pcg_s, rel_max=max(rel), phase_ms=phase,
-                spmv_ms=ms.value / max(calls.value, 1), spmv_calls=calls.value)
+                spmv_ms=ms.value / max(calls.value, 1), spmv_calls=calls.value,
+                ms_median=1e3 * per[len(per) // 2], ms_p90=1e3 * per[max(0, -(-9 * len(per) // 10) - 1)], dx=dx)
+
+
+def reference_rule(sysm, steps):
+    """FP64 Newton steps with the reference's stop rule (|r.z| <= 1e-4 r0.z0)."""
+    import torch
+    from sgipc import capi
+    kw = dict(tol=1e-4, stop_rule=capi.STOP_REF_RZ, precision=capi.PRECISION_FP64)
+    sysm.newton_step(**kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, rel = 0, 0.0
+    for _ in range(steps):
+        st = sysm.newton_step(**kw)
+        its += st.iterations
+        rel = max(rel, st.rel_residual)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    return {"value": round(steps / secs, 4), "unit": "Newton solves/s", "ms_per_step": round(1e3 * secs / steps, 4),
+            "pcg_iters_per_step": its / steps, "true_rel_residual_max": rel,
+            "stop": "reference rule |r.z| <= 1e-4 r0.z0, FP64"}
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def mas_banks(sysm, bank):
+    import ctypes as C
+    L = C.c_int(0); tot = C.c_int(0)
+    sysm.ctx.call("sgipc_mas_info", C.byref(L), C.byref(tot), None)
+    return tot.value // bank
+
+
+def kernel_rooflines(sysm, r, fp32_mas):
+    """Achieved GB/s of the assembly and MAS kernels from the library's
+    HIP-event timing and their algorithmic bytes per unit."""
+    out = {}
+    sc = sysm.sc
+    fem_b = 1104.0 * sc.T
+    if r["phase_ms"]["fem_kernel"] > 0:
+        out["k_fem_snk1"] = {"alg_bytes": fem_b, "avg_ms": r["phase_ms"]["fem_kernel"],
+                             "achieved_GBs": round(fem_b / (r["phase_ms"]["fem_kernel"] * 1e-3) / 1e9, 1),
+                             "unit_bytes": "1104 B per tet"}
+    c = [int(v) for v in sysm.counts]
+    bar_b = sum(n * (116.0 + nblk * 80.0 + 96.0) for n, nblk in ((c[0], 4), (c[1], 9), (c[2], 16)))
+    if r["phase_ms"]["barrier_kernel"] > 0 and bar_b > 0:
+        out["k_barrier"] = {"alg_bytes": bar_b, "avg_ms": r["phase_ms"]["barrier_kernel"],
+                            "achieved_GBs": round(bar_b / (r["phase_ms"]["barrier_kernel"] * 1e-3) / 1e9, 1),
+                            "unit_bytes": "116 B read + 80 B per block + 96 B gradient per pair"}
+    nbk = mas_banks(sysm, sc.bank)
+    D, w = 3 * sc.bank, (4.0 if fp32_mas else 8.0)
+    mas_b = nbk * (D * (D + 1) / 2 * w + 2 * D * w)
+    if r["phase_ms"]["mas_bank_solve_kernel"] > 0:
+        out["k_bank_solve"] = {"alg_bytes": mas_b, "avg_ms": r["phase_ms"]["mas_bank_solve_kernel"],
+                               "achieved_GBs": round(mas_b / (r["phase_ms"]["mas_bank_solve_kernel"] * 1e-3) / 1e9, 1),
+                               "unit_bytes": "packed %dx%d inverse + R and Z per bank, %s"
+                                             % (D, D, "FP32" if fp32_mas else "FP64"),
+                               "banks": nbk}
+    return out
 
 
 def main():
@@ -349,6 +422,7 @@ def main():
     sysm.precond = capi.PRECOND_MAS
     main_mode = "mixed" if "mixed" in res else modes[0]
     r = res[main_mode]
+    ref_rule = reference_rule(sysm, 5)
 
     # roofline of the dominant kernel (PCG SpMV) of the reported mode
     fp32 = main_mode == "mixed"
@@ -370,7 +444,8 @@ def main():
         cpu = {"value": round(csteps / csecs, 5), "unit": "Newton solves/s", "cores": 1, "kind": "port",
                "sample": "%d full Newton step(s) of the same workload (barrier + FEM assembly, convert, MAS build, "
                          "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, 1 thread, "
-                         "%.1f s" % (csteps, cit, csecs)}
+                         "%.1f s" % (csteps, cit, csecs),
+               "label": "CPU restatement of the reference algorithm (baseline only)", "host": host_cpu()}
 
     g = os.environ.get("SGIPC_SPMV_G", "8")
     g = g if g in ("4", "8", "16") else "8"   # lanes per block row, as the library reads it
@@ -404,13 +479,19 @@ def main():
             "stop": "true relative residual 1e-6",
             "parallelism": "single GPU",
         },
+        "ms_per_step_median": round(r["ms_median"], 4),
+        "ms_per_step_p90": round(r["ms_p90"], 4),
         "pcg_iters_per_step": r["iters"],
         "pcg_iters_per_s": round(r["pcg_iters_per_s"], 2),
         "rel_residual_max": r["rel_max"],
         "phase_ms": r["phase_ms"],
+        "kernel_rooflines": kernel_rooflines(sysm, r, main_mode == "mixed"),
+        "reference_rule": ref_rule,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_ms": round(r["spmv_ms"], 5), "launches": r["spmv_calls"], "alg_bytes": alg},
+                     "avg_ms": round(r["spmv_ms"], 5), "launches": r["spmv_calls"], "alg_bytes": alg,
+                     "cache_note": "the SpMV's matrix copy (about 76 MB FP32 / 68 MB FP64) fits MI355X's 256 MB "
+                                   "Infinity Cache; FETCH_SIZE counts L2 misses, which it may serve instead of HBM"},
         "cpu_baseline": cpu,
         "broad_phase": bp,
     }
@@ -423,6 +504,11 @@ def main():
                        "spmv_achieved_GBs": round(spmv_alg_bytes(sysm.K, sysm.n, False) / (f["spmv_ms"] * 1e-3) / 1e9,
                                                   2)}
         out["mixed_over_fp64"] = round(r["value"] / f["value"], 4)
+        out["fp64"]["ms_per_step_median"] = round(f["ms_median"], 4)
+        out["fp64"]["ms_per_step_p90"] = round(f["ms_p90"], 4)
+        out["fp64"]["kernel_rooflines"] = kernel_rooflines(sysm, f, False)
+        d64 = f["dx"]
+        out["mixed_dx_rel_diff_vs_fp64"] = float((r["dx"] - d64).norm() / d64.norm())
     print(json.dumps(out))
 
 
@@ -515,6 +601,11 @@ def hybrid(a):
         out["%s_over_fp64" % main_mode] = round(r["value"] / res["fp64"]["value"], 4)
     if "mixed" in res and "fp64" in res:
         out["mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 4)
+    if "fp64" in res:
+        d64 = res["fp64"]["dx"]
+        out["dx_rel_diff_vs_fp64"] = {m: float((res[m]["dx"] - d64).norm() / d64.norm()) for m in res if m != "fp64"}
+    out["ms_per_step_median"] = round(r["ms_median"], 4)
+    out["ms_per_step_p90"] = round(r["ms_p90"], 4)
     print(json.dumps(out))
 
 
