@@ -670,6 +670,42 @@ __device__ __forceinline__ void gj_all(double (&S)[D], double (*colm)[D], int i,
     (gj_step<J, D>(S, colm, i), ...);
 }
 
+// value v of lane `lane` (a compile-time constant) in every lane, through
+// scalar registers
+template <int LANE>
+__device__ __forceinline__ double lane_value(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), LANE);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), LANE);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The same step with the pivot column broadcast lane to lane (no LDS, no
+// barrier).  Each entry k of the owner's column is read before the owner
+// overwrites it in the same iteration; the owner starts from the unit
+// vector e_J, as in gj_step, so the arithmetic is identical.
+template <int J, int D, int K>
+__device__ __forceinline__ void gj_rl_entry(double (&S)[D], double pv, bool own) {
+    if constexpr (K != J) {
+        const double c = lane_value<J>(S[K]);
+        const double base = own ? 0.0 : S[K];
+        S[K] = base + -c * pv;
+    }
+}
+template <int J, int D, int... K>
+__device__ __forceinline__ void gj_rl_step(double (&S)[D], int i, std::integer_sequence<int, K...>) {
+    const bool own = i == J;
+    const double rt = lane_value<J>(S[J]);
+    const double pv = (own ? 1.0 : S[J]) / rt;
+    (gj_rl_entry<J, D, K>(S, pv, own), ...);
+    S[J] = pv;
+}
+template <int D, int... J>
+__device__ __forceinline__ void gj_rl_all(double (&S)[D], int i, std::integer_sequence<int, J...>) {
+    (gj_rl_step<J, D>(S, i, std::make_integer_sequence<int, D>{}), ...);
+}
+
+constexpr bool kBankInverseReadlane = true;  // pivot column by lane broadcast (false: through LDS)
 __global__ void __launch_bounds__(64) k_bank_inverse_reg48(int nbanks, const double* __restrict__ bank_upper,
                                                            double* __restrict__ inv64, float* __restrict__ inv32) {
     constexpr int B = 16, D = 48, npack = B * (B + 1) / 2, NP = D * (D + 1) / 2;
@@ -690,7 +726,12 @@ __global__ void __launch_bounds__(64) k_bank_inverse_reg48(int nbanks, const dou
         }
         S[j] = v;
     }
-    gj_all<D>(S, colm, i, std::make_integer_sequence<int, D>{});
+    if (kBankInverseReadlane) {
+        (void)colm;
+        gj_rl_all<D>(S, i, std::make_integer_sequence<int, D>{});
+    } else {
+        gj_all<D>(S, colm, i, std::make_integer_sequence<int, D>{});
+    }
     if (act) {
         double* o64 = inv64 + (size_t)b * NP;
         float* o32 = inv32 + (size_t)b * NP;
