@@ -153,20 +153,23 @@ def test_pcg_matches_oracle(torch_cuda, oracle, cube10, precond):
     assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(xs)
 
 
-def test_pcg_iteration_budget_matches_oracle(torch_cuda, oracle, cube10):
+@pytest.mark.parametrize("budget", [6, 7, 1, 2])
+def test_pcg_iteration_budget_matches_oracle(torch_cuda, oracle, cube10, budget):
     # an unreachable tolerance: both stop at the budget with the same iterate
-    print("case: pcg budget max_iter=6", flush=True)
+    # (graphs hold two iterations: odd budgets stop inside one; a budget of
+    # one runs no iteration)
+    print("case: pcg budget max_iter=%d" % budget, flush=True)
     ref = oracle_system(oracle, cube10)
     sys_ = make_system(cube10)
     sys_.assemble(); sys_.convert(); sys_.build_precond()
     kr, kc, kv = ref["bcoo"]
     mas = oracle.Mas(cube10.n, sys_.part_map, sys_.real_map, sys_.ns, sys_.nn, sys_.nl, cube10.bank)
     mas.build(kr, kc, kv)
-    st = sys_.solve(tol=1e-30, stop_rule=0, max_iter=6)
-    xo, ito, _ = oracle.pcg(kr, kc, kv, cube10.n, ref["b"], precond=2, mas=mas, tol=1e-30, rule=0, max_iter=6)
-    assert st.iterations == ito == 6 and not st.converged
+    st = sys_.solve(tol=1e-30, stop_rule=0, max_iter=budget)
+    xo, ito, _ = oracle.pcg(kr, kc, kv, cube10.n, ref["b"], precond=2, mas=mas, tol=1e-30, rule=0, max_iter=budget)
+    assert st.iterations == ito and not st.converged
     x = sys_.dx.cpu().numpy()
-    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    assert np.linalg.norm(x - xo) <= 1e-10 * max(np.linalg.norm(xo), 1e-300)
 
 
 def test_pcg_graph_replays_match_eager(torch_cuda, cube10):
