@@ -294,7 +294,11 @@ int sgipc_spmv(sgipc_ctx* ctx, double a, const double* d_x, double b, double* d_
  * 18-30) -> MASPreconditioner::setPreconditioner_bcoo
  * (MASPreconditioner.cu:2180-2218); DiagPreconditioner::assemble
  * (preconditioner/diag_preconditioner.cu:53-59).  d_pairs: int4 contact
- * pairs (may be NULL). */
+ * pairs (may be NULL).  With affine bodies set and a MAS kind whose range
+ * starts at or after their 4 n_bodies rows, the bodies' 12x12 block-Jacobi
+ * inverses are built too (ABDPreconditioner::assemble,
+ * preconditioner/abd_preconditioner.cu:15-20) and applied on those rows;
+ * other rows outside the range are left unpreconditioned (identity). */
 int sgipc_precond_build(sgipc_ctx* ctx, int kind, int sub_offset, int sub_nb,
                         const int* d_pairs, int n_pairs);
 /* z = M^-1 r; MASPreconditioner::preconditioning (MASPreconditioner.cu:2221-2262) */

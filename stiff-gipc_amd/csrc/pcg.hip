@@ -17,6 +17,7 @@
 #include "pcg_dev.hpp"
 #include <atomic>
 #include <chrono>
+#include <thread>
 #include <cmath>
 
 namespace sgipc {
@@ -593,7 +594,11 @@ struct Engine {
                     if (q == hipSuccess) break;     // idle: read the words below
                     last_change = std::chrono::steady_clock::now();
                 }
+#if defined(__x86_64__)
                 __builtin_ia32_pause();
+#else
+                std::this_thread::yield();
+#endif
             }
             SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
             SGIPC_HIP(hipStreamSynchronize(s));
