@@ -361,18 +361,37 @@ __global__ void k_merge_values(int K, const int4* __restrict__ desc, const int* 
     int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= K) return;
     int4 d = desc[u];
+    const int n = d.y + d.w;
     double acc[9];
-    bool first = true;
-    for (int q = 0; q < d.y + d.w; ++q) {
-        int t = (q < d.y) ? dsorted[d.x + q] : ssorted[d.z + (q - d.y)] + sshift;
-        const double* v = vals + 9 * (size_t)t;
-        if (first) {
+    // groups of four contributions: their ids, then their values, are loaded
+    // before any is added, so the loads overlap; the sum keeps the order
+    // q = 0, 1, ... (bit-identical to a sequential loop)
+    constexpr int kG = 4;
+    for (int q0 = 0; q0 < n; q0 += kG) {
+        int t[kG];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) acc[e] = v[e];
-            first = false;
-        } else {
+        for (int j = 0; j < kG; ++j) {
+            const int q = q0 + j;
+            t[j] = q >= n ? -1 : ((q < d.y) ? dsorted[d.x + q] : ssorted[d.z + (q - d.y)] + sshift);
+        }
+        double v[kG][9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) acc[e] += v[e];
+        for (int j = 0; j < kG; ++j)
+            if (t[j] >= 0) {
+                const double* src = vals + 9 * (size_t)t[j];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) v[j][e] = src[e];
+            }
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            if (t[j] < 0) break;
+            if (q0 + j == 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) acc[e] = v[j][e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) acc[e] += v[j][e];
+            }
         }
     }
 #pragma unroll
