@@ -39,9 +39,9 @@ struct ConvertPlan {
     long long s0 = 0, ns = 0;
     int nbr = 0, Ks = 0;
     DBuf<int> crow, ccol;       // ns: cached indices
-    DBuf<int> start, bucket;    // nbr + 1 row starts; ns bucketed ids
+    DBuf<int> start;            // nbr + 1 row starts into sorted
     DBuf<int> sorted;           // ns: triplet ids sorted by (row, col, id)
-    DBuf<int> ucnt, ustart;     // unique blocks per row; row starts (nbr + 1)
+    DBuf<int> ustart;           // unique-block row starts (nbr + 1)
     DBuf<int> ucol, urun;       // per unique block: column, run start in sorted
 };
 
@@ -195,14 +195,12 @@ struct Ctx {
     bool have_bcoo = false;
     DBuf<double> bcoo_val; // 9K scratch before copy-back
     DBuf<int> bcoo_row, bcoo_col;
-    DBuf<int> row_count, row_start, bucket, uniq_count, uniq_start;
+    DBuf<int> row_count, row_start, uniq_start;
     DBuf<int> dyn_ustart, dyn_ucol, dyn_urun, merge_row;  // contact segment runs, merged rows
     DBuf<int4> merge_desc;                                 // per merged block: contact / static runs
-    DBuf<int> long_rows, dev_int2;                         // rows sorted by one workgroup; their count
-    DBuf<int> vlong_rows, vl_rows, vl_lens, vl_flag, vl_sel, vl_val;  // rows sorted by a device radix sort
-    DBuf<long long> vl_off;
-    DBuf<unsigned long long> vl_key, vl_key2;
-    DBuf<char> vl_tmp;
+    DBuf<unsigned long long> seg_key, seg_key2;            // (row, col) sort keys of a segment, sorted
+    DBuf<int> seg_head, seg_hscan, seg_beg, seg_fin;       // unique-key heads and their scan; row runs
+    DBuf<char> seg_tmp;                                    // radix sort / scan scratch
     DBuf<double> energy_part;                              // energy partials (+ total)
     long long static_begin = -1;  // first triplet of the FEM + mass segment (set by FEM assembly)
     ConvertPlan cplan;

@@ -2,11 +2,12 @@
 //
 // Reference: S/linear_system/utils/converter.cu:23-196 (radix sort of
 // row<<32|col keys, run heads, exclusive scan, segmented block reduction with
-// FP64 atomics).  Here: counting sort by block row (one pass of row
-// histogram + scan + scatter), then a per-row rank sort of (col, triplet
-// index) inside one wavefront, and a per-unique-block sequential sum in
-// ascending triplet order -- deterministic, no floating-point atomics, and
-// each 72-byte triplet value is read exactly once.
+// FP64 atomics).  Here: one stable device radix sort of (row << bc | col)
+// keys over the triplet ids (bc = bits of the block row count), row bounds
+// and unique-key heads read off the sorted keys, and a per-unique-block
+// sequential sum in ascending triplet order -- deterministic, no
+// floating-point atomics, and each 72-byte triplet value is read exactly
+// once.
 //
 // Across Newton steps the FEM + mass segment of the triplet store keeps its
 // indices (the mesh does not change); only the contact triplets in front of
@@ -19,197 +20,82 @@
 
 #include <algorithm>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_select.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
 namespace sgipc {
 namespace {
 
-constexpr int kSortCap = 256;    // one wavefront per row, LDS: rows up to this many triplets
-constexpr int kLongCap = 4096;   // one workgroup per row, LDS: longer rows up to this many;
-                                 // longer still: a device radix sort of those rows' triplets
-constexpr int kLongThreads = 256;
 constexpr int kRowsPerBlock = 4;  // one wavefront per row
 
-__global__ void k_count_rows(long long t0, long long t1, const int* __restrict__ rows, const int* __restrict__ cols,
-                             int nbr, int* __restrict__ cnt, int* __restrict__ err) {
-    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= t1) return;
-    int r = rows[t], c = cols[t];
+// Sort keys of the triplets [t0, t0 + n): row << bc | col.  A row or column
+// outside [0, nbr) sets *err and takes the key nbr << bc, which sorts after
+// every valid key (nbr < 2^bc).
+__global__ void k_seg_keys(long long t0, long long n, const int* __restrict__ rows, const int* __restrict__ cols,
+                           int nbr, int bc, unsigned long long* __restrict__ key, int* __restrict__ err) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = rows[t0 + i], c = cols[t0 + i];
+    unsigned long long k;
     if ((unsigned)r >= (unsigned)nbr || (unsigned)c >= (unsigned)nbr) {
         atomicOr(err, 1);
+        k = (unsigned long long)nbr << bc;
+    } else {
+        k = ((unsigned long long)r << bc) | (unsigned)c;
+    }
+    key[i] = k;
+}
+
+// First and one-past-last position of each valid row's run of sorted keys;
+// rows without triplets keep beg = -1.
+__global__ void k_row_bounds(long long n, const unsigned long long* __restrict__ key, int bc, int nbr,
+                             int* __restrict__ beg, int* __restrict__ fin) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long r = key[i] >> bc;
+    if (r >= (unsigned long long)nbr) return;
+    if (i == 0 || (key[i - 1] >> bc) != r) beg[r] = (int)i;
+    if (i == n - 1 || (key[i + 1] >> bc) != r) fin[r] = (int)(i + 1);
+}
+
+__global__ void k_row_counts(int nbr, const int* __restrict__ beg, const int* __restrict__ fin, int* __restrict__ cnt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nbr) cnt[r] = beg[r] < 0 ? 0 : fin[r] - beg[r];
+}
+
+// head[i] = 1 where sorted key i is valid and differs from key i - 1;
+// head[n] = 0 (so the exclusive scan's entry n is the unique count)
+__global__ void k_key_heads(long long n, const unsigned long long* __restrict__ key, int bc, int nbr,
+                            int* __restrict__ head) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        head[n] = 0;
         return;
     }
-    atomicAdd(cnt + r, 1);
+    const unsigned long long k = key[i];
+    head[i] = ((k >> bc) < (unsigned long long)nbr && (i == 0 || key[i - 1] != k)) ? 1 : 0;
 }
 
-__global__ void k_scatter_rows(long long t0, long long t1, const int* __restrict__ rows, int nbr,
-                               const int* __restrict__ start, int* __restrict__ cursor,
-                               int* __restrict__ bucket) {
-    long long t = t0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= t1) return;
-    int r = rows[t];
-    if ((unsigned)r >= (unsigned)nbr) return;
-    int pos = start[r] + atomicAdd(cursor + r, 1);
-    bucket[pos] = (int)t;
+// ustart[r] = unique blocks before row r (r in [0, nbr]); *total = all of them
+__global__ void k_unique_starts(int nbr, const int* __restrict__ start, const int* __restrict__ hscan,
+                                int* __restrict__ ustart, int* __restrict__ total) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nbr) return;
+    const int u = hscan[start[r]];
+    ustart[r] = u;
+    if (r == nbr && total) *total = u;
 }
 
-__device__ inline unsigned long long mkkey(int col, int t) {
-    return ((unsigned long long)(unsigned)col << 32) | (unsigned)t;
-}
-
-// One wavefront per row: sort the row's triplets by (col, t) and count the
-// distinct columns.  sorted[] receives triplet ids in sorted order.  All
-// waves of the block reach the same barriers (no early exit).
-__global__ void __launch_bounds__(64 * kRowsPerBlock) k_row_sort(int nbr, const int* __restrict__ start,
-                                                                 const int* __restrict__ bucket,
-                                                                 const int* __restrict__ cols,
-                                                                 int* __restrict__ sorted,
-                                                                 int* __restrict__ ucount,
-                                                                 int* __restrict__ long_rows,
-                                                                 int* __restrict__ n_long,
-                                                                 int* __restrict__ vlong_rows) {
-    __shared__ unsigned long long sk[kRowsPerBlock][kSortCap];
-    __shared__ unsigned long long so[kRowsPerBlock][kSortCap];
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int r = blockIdx.x * kRowsPerBlock + w;
-    int s = 0, len = 0;
-    if (r < nbr) { s = start[r]; len = start[r + 1] - s; }
-    const bool fast = len > 0 && len <= kSortCap;
-    if (fast)
-        for (int i = lane; i < len; i += 64) {
-            int t = bucket[s + i];
-            sk[w][i] = mkkey(cols[t], t);
-        }
-    __syncthreads();
-    if (fast)
-        for (int i = lane; i < len; i += 64) {
-            unsigned long long k = sk[w][i];
-            int rank = 0;
-            for (int j = 0; j < len; ++j) rank += (sk[w][j] < k) ? 1 : 0;
-            so[w][rank] = k;
-        }
-    __syncthreads();
-    int heads = 0;
-    if (fast) {
-        for (int i = lane; i < len; i += 64) {
-            unsigned long long k = so[w][i];
-            sorted[s + i] = (int)(unsigned)(k & 0xffffffffu);
-            bool head = (i == 0) || ((so[w][i - 1] >> 32) != (k >> 32));
-            heads += head ? 1 : 0;
-        }
-    } else if (len > 0 && len <= kLongCap) {
-        // long row: sorted by one workgroup in k_row_sort_long
-        if (lane == 0) long_rows[atomicAdd(n_long, 1)] = r;
-    } else if (len > 0) {
-        // very long rows (rows of the affine bodies collect thousands):
-        // listed for the radix-sort path (sort_very_long_rows)
-        if (lane == 0) vlong_rows[atomicAdd(n_long + 1, 1)] = r;
-    }
-    heads = wave_sum(heads);
-    if (r < nbr && lane == 0 && len <= kSortCap) ucount[r] = heads;
-}
-
-// triplet t belongs to a very long row
-struct VeryLongRow {
-    const int* rows;
-    const int* flag;
-    int nbr;
-    __device__ bool operator()(int t) const {
-        const int r = rows[t];
-        return (unsigned)r < (unsigned)nbr && flag[r] != 0;
-    }
-};
-
-__global__ void k_mark_rows(int n, const int* __restrict__ list, int* __restrict__ flag) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[list[i]] = 1;
-}
-
-__global__ void k_row_lengths(int n, const int* __restrict__ list, const int* __restrict__ start, int* __restrict__ len) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) len[i] = start[list[i] + 1] - start[list[i]];
-}
-
-__global__ void k_row_col_keys(int m, const int* __restrict__ sel, const int* __restrict__ rows,
-                               const int* __restrict__ cols, unsigned long long* __restrict__ key) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) key[i] = ((unsigned long long)(unsigned)rows[sel[i]] << 32) | (unsigned)cols[sel[i]];
-}
-
-// one workgroup per very long row: its (col, id)-sorted triplet ids into
-// place and the count of distinct columns
-__global__ void __launch_bounds__(kLongThreads) k_place_rows(int n, const int* __restrict__ list,
-                                                             const long long* __restrict__ off,
-                                                             const int* __restrict__ start,
-                                                             const unsigned long long* __restrict__ key,
-                                                             const int* __restrict__ val, int* __restrict__ sorted,
-                                                             int* __restrict__ ucount) {
-    __shared__ int hsum[kLongThreads / 64];
-    for (int q = blockIdx.x; q < n; q += gridDim.x) {
-        const int r = list[q];
-        const int s = start[r], len = start[r + 1] - s;
-        const long long o = off[q];
-        int heads = 0;
-        for (int i = threadIdx.x; i < len; i += kLongThreads) {
-            sorted[s + i] = val[o + i];
-            heads += (i == 0 || (unsigned)key[o + i - 1] != (unsigned)key[o + i]) ? 1 : 0;
-        }
-        heads = wave_sum(heads);
-        if ((threadIdx.x & 63) == 0) hsum[threadIdx.x >> 6] = heads;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int h = 0;
-            for (int w = 0; w < kLongThreads / 64; ++w) h += hsum[w];
-            ucount[r] = h;
-        }
-        __syncthreads();
-    }
-}
-
-// Rows of kSortCap < len <= kLongCap triplets, one workgroup per row
-// (grid-stride over the list): rank sort of (col, triplet id) keys in LDS.
-__global__ void __launch_bounds__(kLongThreads) k_row_sort_long(const int* __restrict__ long_rows,
-                                                                const int* __restrict__ n_long,
-                                                                const int* __restrict__ start,
-                                                                const int* __restrict__ bucket,
-                                                                const int* __restrict__ cols,
-                                                                int* __restrict__ sorted, int* __restrict__ ucount) {
-    __shared__ unsigned long long sk[kLongCap];
-    __shared__ unsigned long long so[kLongCap];
-    __shared__ int hsum[kLongThreads / 64];
-    const int nl = *n_long;
-    for (int q = blockIdx.x; q < nl; q += gridDim.x) {
-        const int r = long_rows[q];
-        const int s = start[r], len = start[r + 1] - s;
-        for (int i = threadIdx.x; i < len; i += kLongThreads) {
-            int t = bucket[s + i];
-            sk[i] = mkkey(cols[t], t);
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < len; i += kLongThreads) {
-            unsigned long long k = sk[i];
-            int rank = 0;
-            for (int j = 0; j < len; ++j) rank += (sk[j] < k) ? 1 : 0;
-            so[rank] = k;
-        }
-        __syncthreads();
-        int heads = 0;
-        for (int i = threadIdx.x; i < len; i += kLongThreads) {
-            unsigned long long k = so[i];
-            sorted[s + i] = (int)(unsigned)(k & 0xffffffffu);
-            heads += ((i == 0) || ((so[i - 1] >> 32) != (k >> 32))) ? 1 : 0;
-        }
-        heads = wave_sum(heads);
-        if ((threadIdx.x & 63) == 0) hsum[threadIdx.x >> 6] = heads;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int h = 0;
-            for (int w = 0; w < kLongThreads / 64; ++w) h += hsum[w];
-            ucount[r] = h;
-        }
-        __syncthreads();
-    }
+// column and first sorted position of each unique block
+__global__ void k_key_runs(long long n, const unsigned long long* __restrict__ key, int bc,
+                           const int* __restrict__ head, const int* __restrict__ hscan, int* __restrict__ ucol,
+                           int* __restrict__ urun) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !head[i]) return;
+    const int u = hscan[i];
+    ucol[u] = (int)(key[i] & ((1ull << bc) - 1ull));
+    urun[u] = (int)i;
 }
 
 // One wavefront per row: each head lane sums its run of duplicates in
@@ -253,35 +139,6 @@ __global__ void __launch_bounds__(64 * kRowsPerBlock) k_row_reduce(int nbr, cons
             ocol[u] = col;
 #pragma unroll
             for (int e = 0; e < 9; ++e) oval[9 * (size_t)u + e] = acc[e];
-        }
-        seen += __popcll(hm);
-    }
-}
-
-// unique blocks of a sorted segment: column and first position of each run
-__global__ void __launch_bounds__(64 * kRowsPerBlock) k_row_runs(int nbr, const int* __restrict__ start,
-                                                                 const int* __restrict__ sorted,
-                                                                 const int* __restrict__ cols,
-                                                                 const int* __restrict__ ustart,
-                                                                 int* __restrict__ ucol, int* __restrict__ urun) {
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int r = blockIdx.x * kRowsPerBlock + w;
-    if (r >= nbr) return;
-    int s = start[r], len = start[r + 1] - s;
-    int ubase = ustart[r], seen = 0;
-    for (int c0 = 0; c0 < len; c0 += 64) {
-        int i = c0 + lane;
-        int col = -1;
-        bool head = false;
-        if (i < len) {
-            col = cols[sorted[s + i]];
-            head = (i == 0) || (cols[sorted[s + i - 1]] != col);
-        }
-        unsigned long long hm = __ballot(head);
-        if (head) {
-            int u = ubase + seen + __popcll(hm & ((1ull << lane) - 1ull));
-            ucol[u] = col;
-            urun[u] = s + i;
         }
         seen += __popcll(hm);
     }
@@ -411,93 +268,63 @@ __global__ void k_copy_back(int K, const int* __restrict__ r, const int* __restr
 // ---- full symmetric block CSR --------------------------------------------
 }  // namespace
 
-// Sort the triplets [t0, t1) by (row, col, triplet id): start (nbr+1 row
-// starts into bucket/sorted), sorted (triplet ids), ucnt / ustart (unique
-// blocks per row; total -> *total_dev), bad indices -> *err_dev.
-// Rows longer than kLongCap (listed by k_row_sort): their triplets picked
-// from [t0, t1) in id order, radix-sorted (stably) by (row, col), and placed
-// row by row -- the same (col, id) order as the other rows.
-static void sort_very_long_rows(Ctx& c, long long t0, long long t1, int nbr, const int* start, int* sorted,
-                                int* ucnt, const int* vlong_dev, int nvl) {
-    hipStream_t s = c.stream;
-    std::vector<int> rows_h(nvl);
-    SGIPC_HIP(hipMemcpyAsync(rows_h.data(), vlong_dev, sizeof(int) * nvl, hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    std::sort(rows_h.begin(), rows_h.end());
-    int* list = c.vl_rows.reserve((size_t)nvl + 1);
-    int* lens = c.vl_lens.reserve((size_t)nvl + 1);
-    SGIPC_HIP(hipMemcpyAsync(list, rows_h.data(), sizeof(int) * nvl, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_row_lengths, dim3(div_up(nvl, 256)), dim3(256), 0, s, nvl, (const int*)list, start, lens);
-    SGIPC_LAUNCH_CHECK();
-    std::vector<int> lens_h(nvl);
-    SGIPC_HIP(hipMemcpyAsync(lens_h.data(), lens, sizeof(int) * nvl, hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    std::vector<long long> off_h(nvl);
-    long long m = 0;
-    for (int i = 0; i < nvl; ++i) { off_h[i] = m; m += lens_h[i]; }
-    long long* off = c.vl_off.reserve((size_t)nvl + 1);
-    SGIPC_HIP(hipMemcpyAsync(off, off_h.data(), sizeof(long long) * nvl, hipMemcpyHostToDevice, s));
-    int* flag = c.vl_flag.reserve((size_t)nbr + 1);
-    fill<int>(flag, 0, nbr, s);
-    hipLaunchKernelGGL(k_mark_rows, dim3(div_up(nvl, 256)), dim3(256), 0, s, nvl, (const int*)list, flag);
-    SGIPC_LAUNCH_CHECK();
-    int* sel = c.vl_sel.reserve((size_t)m + 1);
-    int* val2 = c.vl_val.reserve((size_t)m + 1);
-    unsigned long long* key = c.vl_key.reserve((size_t)m + 1);
-    unsigned long long* key2 = c.vl_key2.reserve((size_t)m + 1);
-    int* nsel = c.vl_lens.get() + nvl;  // one spare int after the lengths
-    rocprim::counting_iterator<int> ids((int)t0);
-    VeryLongRow pred{c.t_rows, flag, nbr};
-    size_t b1 = 0, b2 = 0;
-    SGIPC_HIP(rocprim::select(nullptr, b1, ids, sel, nsel, (size_t)(t1 - t0), pred, s));
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, b2, key, key2, sel, val2, (size_t)m, 0, 64, s));
-    char* tmp = c.vl_tmp.reserve(std::max(b1, b2) + 1);
-    b1 = std::max(b1, b2);
-    SGIPC_HIP(rocprim::select((void*)tmp, b1, ids, sel, nsel, (size_t)(t1 - t0), pred, s));
-    hipLaunchKernelGGL(k_row_col_keys, dim3(div_up(m, 256)), dim3(256), 0, s, (int)m, (const int*)sel, c.t_rows,
-                       c.t_cols, key);
-    SGIPC_LAUNCH_CHECK();
-    int end_bit = 33;
-    while (end_bit < 64 && (1ll << (end_bit - 32)) <= (long long)nbr) ++end_bit;
-    b2 = std::max(b1, b2);
-    SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, b2, key, key2, sel, val2, (size_t)m, 0, end_bit, s));
-    hipLaunchKernelGGL(k_place_rows, dim3(std::min(nvl, 1024)), dim3(kLongThreads), 0, s, nvl, (const int*)list,
-                       (const long long*)off, start, (const unsigned long long*)key2, (const int*)val2, sorted, ucnt);
-    SGIPC_LAUNCH_CHECK();
-}
-
-static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* cnt, int* start, int* bucket,
-                         int* sorted, int* ucnt, int* ustart, int* err_dev, int* total_dev) {
+// Sort the triplets [t0, t1) by (row, col, triplet id) with one stable
+// device radix sort: start (nbr + 1 row starts into sorted, positions
+// relative to t0), sorted (triplet ids), ustart (nbr + 1 unique-block row
+// starts; the total -> *total_dev), and, where ucol / urun are given, the
+// column and first sorted position of each unique block.  Bad indices ->
+// *err_dev (their triplets sort after every row and are left out).
+static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* start, int* sorted, int* ustart,
+                         int* ucol, int* urun, int* err_dev, int* total_dev) {
     hipStream_t s = c.stream;
     const long long n = t1 - t0;
-    fill<int>(cnt, 0, nbr, s);
+    int bc = 1;
+    while ((1ll << bc) <= (long long)nbr) ++bc;  // nbr < 2^bc: the bad-index key fits
+    unsigned long long* key = c.seg_key.reserve((size_t)n + 1);
+    unsigned long long* key2 = c.seg_key2.reserve((size_t)n + 1);
+    int* head = c.seg_head.reserve((size_t)n + 2);
+    int* hscan = c.seg_hscan.reserve((size_t)n + 2);
+    int* beg = c.seg_beg.reserve((size_t)nbr + 1);
+    int* fin = c.seg_fin.reserve((size_t)nbr + 1);
+    int* cnt = c.row_count.reserve((size_t)nbr + 1);
+    rocprim::counting_iterator<int> ids((int)t0);
+    size_t b1 = 0, b2 = 0;
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, b1, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
+    SGIPC_HIP(rocprim::exclusive_scan(nullptr, b2, head, hscan, 0, (size_t)n + 1, rocprim::plus<int>(), s));
+    size_t bytes = std::max(b1, b2);
+    char* tmp = c.seg_tmp.reserve(bytes + 1);
     if (n > 0) {
-        hipLaunchKernelGGL(k_count_rows, dim3(div_up(n, 256)), dim3(256), 0, s, t0, t1, c.t_rows, c.t_cols, nbr, cnt,
+        hipLaunchKernelGGL(k_seg_keys, dim3(div_up(n, 256)), dim3(256), 0, s, t0, n, c.t_rows, c.t_cols, nbr, bc, key,
                            err_dev);
         SGIPC_LAUNCH_CHECK();
+        size_t b = bytes;
+        SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, b, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
     }
-    exclusive_scan(c, cnt, start, nbr, nullptr, s);
-    fill<int>(cnt, 0, nbr, s);
+    fill<int>(beg, -1, nbr, s);
     if (n > 0) {
-        hipLaunchKernelGGL(k_scatter_rows, dim3(div_up(n, 256)), dim3(256), 0, s, t0, t1, c.t_rows, nbr, start, cnt,
-                           bucket);
+        hipLaunchKernelGGL(k_row_bounds, dim3(div_up(n, 256)), dim3(256), 0, s, n, (const unsigned long long*)key2, bc,
+                           nbr, beg, fin);
         SGIPC_LAUNCH_CHECK();
     }
-    int* long_rows = c.long_rows.reserve((size_t)nbr + 1);
-    int* vlong_rows = c.vlong_rows.reserve((size_t)nbr + 1);
-    int* n_long = c.dev_int2.reserve(2);   // [0] long rows, [1] very long rows
-    fill<int>(n_long, 0, 2, s);
-    hipLaunchKernelGGL(k_row_sort, dim3(div_up(nbr, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, nbr, start,
-                       bucket, c.t_cols, sorted, ucnt, long_rows, n_long, vlong_rows);
+    hipLaunchKernelGGL(k_row_counts, dim3(div_up(nbr, 256)), dim3(256), 0, s, nbr, (const int*)beg, (const int*)fin,
+                       cnt);
     SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_row_sort_long, dim3(1024), dim3(kLongThreads), 0, s, (const int*)long_rows,
-                       (const int*)n_long, start, bucket, c.t_cols, sorted, ucnt);
+    exclusive_scan(c, cnt, start, nbr, nullptr, s);
+    hipLaunchKernelGGL(k_key_heads, dim3(div_up(n + 1, 256)), dim3(256), 0, s, n, (const unsigned long long*)key2, bc,
+                       nbr, head);
     SGIPC_LAUNCH_CHECK();
-    int nvl = 0;
-    SGIPC_HIP(hipMemcpyAsync(&nvl, n_long + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    if (nvl > 0) sort_very_long_rows(c, t0, t1, nbr, start, sorted, ucnt, vlong_rows, nvl);
-    exclusive_scan(c, ucnt, ustart, nbr, total_dev, s);
+    {
+        size_t b = bytes;
+        SGIPC_HIP(rocprim::exclusive_scan((void*)tmp, b, head, hscan, 0, (size_t)n + 1, rocprim::plus<int>(), s));
+    }
+    hipLaunchKernelGGL(k_unique_starts, dim3(div_up(nbr + 1, 256)), dim3(256), 0, s, nbr, (const int*)start,
+                       (const int*)hscan, ustart, total_dev);
+    SGIPC_LAUNCH_CHECK();
+    if (ucol && n > 0) {
+        hipLaunchKernelGGL(k_key_runs, dim3(div_up(n, 256)), dim3(256), 0, s, n, (const unsigned long long*)key2, bc,
+                           (const int*)head, (const int*)hscan, ucol, urun);
+        SGIPC_LAUNCH_CHECK();
+    }
 }
 
 static void copy_back(Ctx& c, int K, const int* orow, const int* ocol, const double* oval) {
@@ -525,18 +352,15 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
     const long long s0 = c.static_begin;
     if (s0 < 0 || s0 >= nt) {
         // ---- one sort of all triplets ----
-        int* cnt = c.row_count.reserve((size_t)nbr + 1);
         int* start = c.row_start.reserve((size_t)nbr + 2);
-        int* bucket = c.bucket.reserve((size_t)nt + 1);
         int* sorted = c.bcoo_row.reserve((size_t)nt + 1);
-        int* ucnt = c.uniq_count.reserve((size_t)nbr + 1);
-        sort_segment(c, 0, nt, nbr, cnt, start, bucket, sorted, ucnt, ustart, dint + 2, dint);
+        sort_segment(c, 0, nt, nbr, start, sorted, ustart, nullptr, nullptr, dint + 2, dint);
         int hk[4] = {0, 0, 0, 0};
         SGIPC_HIP(hipMemcpyAsync(hk, dint, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
         SGIPC_HIP(hipStreamSynchronize(s));
         if (hk[2]) throw arg_error("convert: triplet row/col index out of range [0, n_block_rows)");
         const int K = hk[0];
-        int* orow = c.bucket.get();  // bucket no longer needed after the sort
+        int* orow = c.merge_row.reserve((size_t)K + 1);
         int* ocol = c.bcoo_col.reserve((size_t)K + 1);
         double* oval = c.bcoo_val.reserve(9 * (size_t)K + 1);
         hipLaunchKernelGGL(k_row_reduce, dim3(div_up(nbr, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, nbr,
@@ -560,18 +384,12 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
                            (const int*)(c.t_cols + s0), (const int*)P.crow.p, (const int*)P.ccol.p, dint + 5);
         SGIPC_LAUNCH_CHECK();
     }
-    int* cnt = c.row_count.reserve((size_t)nbr + 1);
     int* dstart = c.row_start.reserve((size_t)nbr + 2);
-    int* dbucket = c.bucket.reserve((size_t)s0 + 1);
     int* dsorted = c.bcoo_row.reserve((size_t)s0 + 1);
-    int* ducnt = c.uniq_count.reserve((size_t)nbr + 1);
     int* dus = c.dyn_ustart.reserve((size_t)nbr + 2);
-    sort_segment(c, 0, s0, nbr, cnt, dstart, dbucket, dsorted, ducnt, dus, dint + 2, dint);
     int* ducol = c.dyn_ucol.reserve((size_t)s0 + 1);
     int* durun = c.dyn_urun.reserve((size_t)s0 + 1);
-    hipLaunchKernelGGL(k_row_runs, dim3(div_up(nbr, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, nbr, dstart,
-                       dsorted, c.t_cols, dus, ducol, durun);
-    SGIPC_LAUNCH_CHECK();
+    sort_segment(c, 0, s0, nbr, dstart, dsorted, dus, ducol, durun, dint + 2, dint);
     int hk[8] = {0};
     SGIPC_HIP(hipMemcpyAsync(hk, dint, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
@@ -580,17 +398,12 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
         // (re)build the plan of the static segment
         P.valid = false;
         int* pstart = P.start.reserve((size_t)nbr + 2);
-        int* pbucket = P.bucket.reserve((size_t)ns + 1);
         int* psorted = P.sorted.reserve((size_t)ns + 1);
-        int* pucnt = P.ucnt.reserve((size_t)nbr + 1);
         int* pus = P.ustart.reserve((size_t)nbr + 2);
-        fill<int>(dint + 6, 0, 2, s);
-        sort_segment(c, s0, nt, nbr, cnt, pstart, pbucket, psorted, pucnt, pus, dint + 6, dint + 7);
         int* pucol = P.ucol.reserve((size_t)ns + 1);
         int* purun = P.urun.reserve((size_t)ns + 1);
-        hipLaunchKernelGGL(k_row_runs, dim3(div_up(nbr, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, nbr, pstart,
-                           psorted, c.t_cols, pus, pucol, purun);
-        SGIPC_LAUNCH_CHECK();
+        fill<int>(dint + 6, 0, 2, s);
+        sort_segment(c, s0, nt, nbr, pstart, psorted, pus, pucol, purun, dint + 6, dint + 7);
         SGIPC_HIP(hipMemcpyAsync(P.crow.reserve((size_t)ns), c.t_rows + s0, sizeof(int) * ns, hipMemcpyDeviceToDevice,
                                  s));
         SGIPC_HIP(hipMemcpyAsync(P.ccol.reserve((size_t)ns), c.t_cols + s0, sizeof(int) * ns, hipMemcpyDeviceToDevice,

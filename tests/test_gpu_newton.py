@@ -230,18 +230,19 @@ def test_reduced_precision_solves(torch_cuda, oracle, cube10, mode):
 
 
 def test_convert_long_rows_bitexact(torch_cuda, oracle, cube4):
-    """Rows of every length class (one wavefront, one workgroup, and the
-    device radix-sort path above 4096 triplets, which the rows of affine
-    bodies reach) convert bit for bit as the oracle does."""
+    """Rows of every length class (one wavefront up to 256 triplets, one
+    workgroup's bitonic sort up to 4096, and the device radix-sort path above
+    that, which the rows of affine bodies reach), at and either side of each
+    class boundary, convert bit for bit as the oracle does."""
     import torch
     print("case: convert long rows", flush=True)
     rng = np.random.default_rng(21)
     n = cube4.n
-    lens = {0: 9000, 1: 5000, 2: 3000, 3: 200}
+    lens = {0: 9000, 1: 5000, 2: 3000, 3: 200, 4: 257, 5: 4096, 6: 256}
     R, Cc = [], []
     for r, L in lens.items():
         R.append(np.full(L, r)); Cc.append(rng.integers(r, n, L))
-    for r in range(4, n):
+    for r in range(len(lens), n):
         L = int(rng.integers(0, 12))
         R.append(np.full(L, r)); Cc.append(rng.integers(r, n, L))
     R = np.concatenate(R).astype(np.int32); Cc = np.concatenate(Cc).astype(np.int32)
