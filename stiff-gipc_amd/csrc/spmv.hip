@@ -94,22 +94,38 @@ __global__ void k_to_f32(long long K, const double* __restrict__ s, float* __res
     }
 }
 
-// entry-ordered FP32 copy: one thread per block row
-__global__ void k_full_f32(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
-                           const double* __restrict__ vals, int* __restrict__ colf, float* __restrict__ vf) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nb) return;
-    for (int e = eptr[r]; e < eptr[r + 1]; ++e) {
-        const int2 q = ent[e];
-        const double* A = vals + 9 * (size_t)q.x;
-        float* d = vf + kF32FullStride * (size_t)e;
-        const bool tr = q.y < r;
+// entry-ordered FP32 copy: eight lanes per block row, lane j taking entries
+// j, j + 8, ... of the row, so a row's lanes write adjacent 40-byte entries
+// (five aligned float2 stores each); rows are walked in per-XCD ranges, as
+// in the SpMV, so the transposed blocks a row gathers are shared in L2.
+constexpr int kFullG = 8;
+__global__ void __launch_bounds__(256) k_full_f32(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
+                                                  const double* __restrict__ vals, int* __restrict__ colf,
+                                                  float* __restrict__ vf) {
+    constexpr int RPB = 256 / kFullG;
+    const ChunkWalk w = xcd_chunks((nb + RPB - 1) / RPB);
+    for (int ch = w.first; ch < w.end; ch += w.step) {
+        const int r = ch * RPB + threadIdx.x / kFullG, sub = threadIdx.x % kFullG;
+        if (r >= nb) continue;
+        const int end = eptr[r + 1];
+        for (int e = eptr[r] + sub; e < end; e += kFullG) {
+            const int2 q = ent[e];
+            const double* A = vals + 9 * (size_t)q.x;
+            double a[9];
 #pragma unroll
-        for (int pp = 0; pp < 3; ++pp)
+            for (int k = 0; k < 9; ++k) a[k] = A[k];
+            float d[10];
+            const bool tr = q.y < r;
 #pragma unroll
-            for (int qq = 0; qq < 3; ++qq) d[pp + 3 * qq] = (float)(tr ? A[qq + 3 * pp] : A[pp + 3 * qq]);
-        d[9] = 0.0f;
-        colf[e] = q.y;
+            for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+                for (int qq = 0; qq < 3; ++qq) d[pp + 3 * qq] = (float)(tr ? a[qq + 3 * pp] : a[pp + 3 * qq]);
+            d[9] = 0.0f;
+            float2* o = reinterpret_cast<float2*>(vf + kF32FullStride * (size_t)e);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) o[k] = make_float2(d[2 * k], d[2 * k + 1]);
+            colf[e] = q.y;
+        }
     }
 }
 
@@ -164,7 +180,8 @@ const float* bsr_full_f32(Ctx& c, const int** cols) {
     float* vf = S.valfull.reserve(kF32FullStride * ne + 4);
     int* cf = S.colfull.reserve(ne + 1);
     if (!S.full_ok && S.nb > 0) {
-        hipLaunchKernelGGL(k_full_f32, dim3(div_up(S.nb, 256)), dim3(256), 0, c.stream, S.nb, (const int*)S.eptr.p,
+        const int blocks = 8 * std::min(div_up(div_up(S.nb, 256 / kFullG), 8), 2048);
+        hipLaunchKernelGGL(k_full_f32, dim3(blocks), dim3(256), 0, c.stream, S.nb, (const int*)S.eptr.p,
                            (const int2*)S.ent.p, (const double*)c.t_vals, cf, vf);
         SGIPC_LAUNCH_CHECK();
     }
