@@ -211,26 +211,18 @@ __global__ void k_merge_write(int nbr, const int* __restrict__ sstart, const int
 }
 
 // value of each merged block: its contact triplets, then its static ones,
-// each run in ascending triplet order.  Nine lanes per block (seven blocks
-// per wavefront): lane e of a group sums element e of every contribution in
-// that order, so each element's sum is the sequential one, and the group
-// reads each 72-byte contribution as one contiguous piece.
-constexpr int kMergeGroups = 7;
-__global__ void __launch_bounds__(256) k_merge_values(int K, const int4* __restrict__ desc,
-                                                      const int* __restrict__ dsorted,
-                                                      const int* __restrict__ ssorted, int sshift,
-                                                      const double* __restrict__ vals, double* __restrict__ oval) {
-    const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int g = lane / 9, e = lane - 9 * g;
-    const long long u = wave * kMergeGroups + g;
-    if (g >= kMergeGroups || u >= K) return;
-    const int4 d = desc[u];
+// each run in ascending triplet order
+__global__ void k_merge_values(int K, const int4* __restrict__ desc, const int* __restrict__ dsorted,
+                               const int* __restrict__ ssorted, int sshift, const double* __restrict__ vals,
+                               double* __restrict__ oval) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= K) return;
+    int4 d = desc[u];
     const int n = d.y + d.w;
-    double acc = 0.0;
+    double acc[9];
     // groups of four contributions: their ids, then their values, are loaded
     // before any is added, so the loads overlap; the sum keeps the order
-    // q = 0, 1, ...
+    // q = 0, 1, ... (bit-identical to a sequential loop)
     constexpr int kG = 4;
     for (int q0 = 0; q0 < n; q0 += kG) {
         int t[kG];
@@ -239,16 +231,28 @@ __global__ void __launch_bounds__(256) k_merge_values(int K, const int4* __restr
             const int q = q0 + j;
             t[j] = q >= n ? -1 : ((q < d.y) ? dsorted[d.x + q] : ssorted[d.z + (q - d.y)] + sshift);
         }
-        double v[kG];
+        double v[kG][9];
 #pragma unroll
-        for (int j = 0; j < kG; ++j) v[j] = t[j] >= 0 ? vals[9 * (size_t)t[j] + e] : 0.0;
+        for (int j = 0; j < kG; ++j)
+            if (t[j] >= 0) {
+                const double* src = vals + 9 * (size_t)t[j];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) v[j][e] = src[e];
+            }
 #pragma unroll
         for (int j = 0; j < kG; ++j) {
             if (t[j] < 0) break;
-            acc = (q0 + j == 0) ? v[j] : acc + v[j];
+            if (q0 + j == 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) acc[e] = v[j][e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) acc[e] += v[j][e];
+            }
         }
     }
-    oval[9 * u + e] = acc;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) oval[9 * (size_t)u + e] = acc[e];
 }
 
 __global__ void k_copy_back(int K, const int* __restrict__ r, const int* __restrict__ c,
@@ -431,8 +435,7 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
                        (const int*)dus, (const int*)ducol, (const int*)durun, (const int*)ustart, orow, ocol, desc);
     SGIPC_LAUNCH_CHECK();
     if (K > 0) {
-        hipLaunchKernelGGL(k_merge_values, dim3(div_up(div_up(K, kMergeGroups), 4)), dim3(256), 0, s, K,
-                           (const int4*)desc,
+        hipLaunchKernelGGL(k_merge_values, dim3(div_up(K, 256)), dim3(256), 0, s, K, (const int4*)desc,
                            (const int*)dsorted, (const int*)P.sorted.p, (int)(s0 - P.s0), (const double*)c.t_vals,
                            oval);
         SGIPC_LAUNCH_CHECK();

@@ -231,8 +231,43 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
 }
 
 // The same with the row-contiguous FP32 copy of both triangles: the G lanes
-// of a row read consecutive entries (five 8-byte loads each), no block
-// index, no transposition in registers; same entry order, so the same sums.
+// of a row read consecutive entries (40 bytes each: two 16-byte loads at
+// 8-byte alignment and one 8-byte load), no block index, no transposition in
+// registers; the column's three z values come in one 12-byte load; same
+// entry order and the same products, so the same sums.
+typedef float f4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef float f3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+
+struct FullEntry {
+    f4_a8 lo, hi;  // column-major elements 0-3, 4-7
+    float a8;
+};
+__device__ inline FullEntry load_full_entry(const float* __restrict__ vf, int e) {
+    const float* A = vf + kF32FullStride * (size_t)e;
+    FullEntry f;
+    f.lo = *reinterpret_cast<const f4_a8*>(A);
+    f.hi = *reinterpret_cast<const f4_a8*>(A + 4);
+    f.a8 = reinterpret_cast<const float2*>(A + 8)->x;
+    return f;
+}
+template <typename V>
+__device__ inline f3_a4 load_z3(const V* __restrict__ z, int c) {
+    if constexpr (sizeof(V) == 4) {
+        return *reinterpret_cast<const f3_a4*>(z + 3 * (size_t)c);
+    } else {
+        f3_a4 r;
+        r.x = (float)z[3 * (size_t)c];
+        r.y = (float)z[3 * (size_t)c + 1];
+        r.z = (float)z[3 * (size_t)c + 2];
+        return r;
+    }
+}
+__device__ inline void full_madd(const FullEntry& f, const f3_a4& x, float& s0, float& s1, float& s2) {
+    s0 += f.lo.x * x.x + f.lo.w * x.y + f.hi.z * x.z;
+    s1 += f.lo.y * x.x + f.hi.x * x.y + f.hi.w * x.z;
+    s2 += f.lo.z * x.x + f.hi.y * x.y + f.a8 * x.z;
+}
+
 template <int G, typename V>
 __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* __restrict__ eptr,
                                                             const int* __restrict__ colf,
@@ -266,32 +301,12 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* _
             const int end = eptr[row + 1];
             int e = eptr[row] + sub;
             for (; e + G < end; e += 2 * G) {  // two entries in flight
-                const float2* A = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)e);
-                const float2* B = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)(e + G));
-                const float2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
-                const float2 b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4];
-                const int ca = colf[e], cb = colf[e + G];
-                const float x0 = (float)z[3 * (size_t)ca], x1 = (float)z[3 * (size_t)ca + 1],
-                            x2 = (float)z[3 * (size_t)ca + 2];
-                const float y0 = (float)z[3 * (size_t)cb], y1 = (float)z[3 * (size_t)cb + 1],
-                            y2 = (float)z[3 * (size_t)cb + 2];
-                s0 += a0.x * x0 + a1.y * x1 + a3.x * x2;
-                s1 += a0.y * x0 + a2.x * x1 + a3.y * x2;
-                s2 += a1.x * x0 + a2.y * x1 + a4.x * x2;
-                s0 += b0.x * y0 + b1.y * y1 + b3.x * y2;
-                s1 += b0.y * y0 + b2.x * y1 + b3.y * y2;
-                s2 += b1.x * y0 + b2.y * y1 + b4.x * y2;
+                const FullEntry fa = load_full_entry(vf, e), fb = load_full_entry(vf, e + G);
+                const f3_a4 xa = load_z3(z, colf[e]), xb = load_z3(z, colf[e + G]);
+                full_madd(fa, xa, s0, s1, s2);
+                full_madd(fb, xb, s0, s1, s2);
             }
-            if (e < end) {
-                const float2* A = reinterpret_cast<const float2*>(vf + kF32FullStride * (size_t)e);
-                const float2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
-                const int ca = colf[e];
-                const float x0 = (float)z[3 * (size_t)ca], x1 = (float)z[3 * (size_t)ca + 1],
-                            x2 = (float)z[3 * (size_t)ca + 2];
-                s0 += a0.x * x0 + a1.y * x1 + a3.x * x2;
-                s1 += a0.y * x0 + a2.x * x1 + a3.y * x2;
-                s2 += a1.x * x0 + a2.y * x1 + a4.x * x2;
-            }
+            if (e < end) full_madd(load_full_entry(vf, e), load_z3(z, colf[e]), s0, s1, s2);
         }
         group_sum3<G, float>(s0, s1, s2);
         if (sub < 3 && row < nb) {
@@ -770,6 +785,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
         int total = 0;
         bool conv = false;
         double rn = bn;
+        double rr = bb;  // ||b - A x||^2 of the current x (x = 0)
         st.rz0 = 0.0;
         for (int outer = 0; outer < 50 && total < max_iter; ++outer) {
             if (rn <= o.tol * bn || bn == 0.0) { conv = true; break; }
@@ -788,12 +804,16 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
             hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
             SGIPC_LAUNCH_CHECK();
             spmv64(c, -1.0, x, 1.0, r64);
-            rn = std::sqrt(device_norm2(c, r64, N, part_o, sc, S_NUM));
+            rr = device_norm2(c, r64, N, part_o, sc, S_NUM);
+            rn = std::sqrt(rr);
             st.refinements = outer + 1;
         }
         if (rn <= o.tol * bn) conv = true;
         st.iterations = total;
         st.converged = conv ? 1 : 0;
+        // the refinement loop's last FP64 true residual is that of the final x
+        st.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+        return;
     }
     // true residual ||b - A x|| / ||b||, FP64
     double* tmp = w.xw.reserve(N + 1);
