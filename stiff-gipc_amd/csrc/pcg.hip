@@ -154,6 +154,22 @@ __device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_pr
     return stop;
 }
 
+constexpr int C_PUB = C_NUM + 2;  // publications of the progress words (pinned polling)
+
+// Progress words for the host's pinned-memory polling: the stop and
+// iteration words and the progress ratio, then (after a system-scope fence)
+// the publication count the host waits on.  One thread.
+__device__ inline void publish_progress(int* __restrict__ ctrl, const double* __restrict__ sc, volatile int* hp,
+                                        volatile double* hq) {
+    const int pub = ctrl[C_PUB] + 1;
+    ctrl[C_PUB] = pub;
+    hq[0] = sc[S_PROG];
+    hp[0] = ctrl[C_STOP];
+    hp[1] = ctrl[C_ITER];
+    __threadfence_system();
+    hp[2] = pub;
+}
+
 // Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range, the 12x12
 // block-Jacobi of the affine bodies on block rows < abd_rows (when abd_inv is
 // set), identity elsewhere, and partials of r.z; one thread per block row.
@@ -164,8 +180,12 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
                                                           int abd_rows, double* __restrict__ part_rz,
                                                           const double* __restrict__ part_rz_prev, int np_rz,
                                                           const double* __restrict__ part_rr, int np_rr,
-                                                          double* __restrict__ sc, int* __restrict__ ctrl, int rule) {
-    if (ctrl[C_STOP]) return;
+                                                          double* __restrict__ sc, int* __restrict__ ctrl, int rule,
+                                                          volatile int* hp, volatile double* hq) {
+    if (ctrl[C_STOP]) {
+        if (hp && blockIdx.x == 0 && threadIdx.x == 0) publish_progress(ctrl, sc, hp, hq);
+        return;
+    }
     __shared__ double sh[kThreads / 64];
     double v = 0.0;
     // z first (its loads do not depend on the stop test), then the test
@@ -198,7 +218,10 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
         z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
         v += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
     }
-    if (stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh)) return;
+    const bool stop = stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh);
+    // the thread that ran the test's writes publishes its outcome
+    if (hp && blockIdx.x == 0 && threadIdx.x == 0) publish_progress(ctrl, sc, hp, hq);
+    if (stop) return;
     write_partial(v, part_rz, sh);
 }
 
@@ -339,7 +362,6 @@ __global__ void k_axpy(long long N, double* __restrict__ y, double a, const doub
         y[i] += a * x[i];
 }
 
-constexpr int C_PUB = C_NUM + 2;  // publications of the progress words (pinned polling)
 
 __global__ void k_ctrl_init(int* ctrl, int maxit) {
     ctrl[C_STOP] = 0;
@@ -354,13 +376,7 @@ __global__ void k_ctrl_init(int* ctrl, int maxit) {
 // a system-scope fence) the publication count the host waits on.
 __global__ void k_publish(int* __restrict__ ctrl, const double* __restrict__ sc, volatile int* hp,
                           volatile double* hq) {
-    const int pub = ctrl[C_PUB] + 1;
-    ctrl[C_PUB] = pub;
-    hq[0] = sc[S_PROG];
-    hp[0] = ctrl[C_STOP];
-    hp[1] = ctrl[C_ITER];
-    __threadfence_system();
-    hp[2] = pub;
+    publish_progress(ctrl, sc, hp, hq);
 }
 
 __global__ void k_set_scalar(double* sc, int idx, double v) { sc[idx] = v; }
@@ -415,6 +431,10 @@ struct Engine {
     // mixed precision: the row-contiguous FP32 copy (null: the upper BCOO)
     const float* vfull = nullptr;
     const int* cfull = nullptr;
+    // pinned progress words published by the MAS collect kernel (set while
+    // the last iteration of a captured group is enqueued)
+    volatile int* pub_h = nullptr;
+    volatile double* pub_q = nullptr;
     static constexpr bool kFp32Mas = sizeof(T) == 4;
 
     // z = M^-1 r fused with the r.z partials (gN workgroups) and the stop test
@@ -424,7 +444,7 @@ struct Engine {
             mas_bank_solve(c, kFp32Mas, ctrl);
             hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
                                c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, abd_inv(c), abd_rows(c), rz_out,
-                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now);
+                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now, pub_h, pub_q);
             SGIPC_LAUNCH_CHECK();
         } else {
             hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_out,
@@ -557,13 +577,21 @@ struct Engine {
                 SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 try {
                     enqueue_iteration(1);
+                    if (pinned && mas) {  // the group's last collect kernel publishes
+                        pub_h = pp.d;
+                        pub_q = reinterpret_cast<volatile double*>(pp.d + 8);
+                    }
                     enqueue_iteration(0);
-                    if (pinned) {
+                    pub_h = nullptr;
+                    pub_q = nullptr;
+                    if (pinned && !mas) {
                         hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, ctrl, (const double*)sc, pp.d,
                                            reinterpret_cast<double*>(pp.d + 8));
                         SGIPC_LAUNCH_CHECK();
                     }
                 } catch (...) {
+                    pub_h = nullptr;
+                    pub_q = nullptr;
                     hipGraph_t dead = nullptr;
                     (void)hipStreamEndCapture(s, &dead);
                     if (dead) (void)hipGraphDestroy(dead);
