@@ -19,6 +19,9 @@ struct SymBcoo {
     int nlow = 0;       // strictly-lower references
     DBuf<int> lptr;     // nb + 1 (build scratch: lower counts)
     DBuf<int2> lref;    // nlow   (build scratch: lower references)
+    DBuf<unsigned> lkey, lkey2;  // K (build scratch: column sort keys)
+    DBuf<int> lidx;              // K (build scratch: block ids in column order)
+    DBuf<char> ltmp;             // radix sort scratch
     DBuf<int> eptr;     // nb + 1
     DBuf<int2> ent;     // nlow + K
     DBuf<float> valf;   // 12 * K FP32 copy of the BCOO values, padded columns (mixed precision)

@@ -42,6 +42,8 @@ __device__ inline unsigned closure(unsigned own, int lane, const unsigned* bank_
 }
 
 // ---- level 0 ---------------------------------------------------------------
+constexpr int kNbrBatch = 8;  // neighbour-list gathers in flight per thread
+
 __global__ void k_l0_masks(int M, int B, const int* __restrict__ part_map, const int* __restrict__ real_map,
                            const unsigned* __restrict__ nstart, const unsigned* __restrict__ nnum,
                            const unsigned* __restrict__ nlist, unsigned* __restrict__ fine_mask) {
@@ -51,9 +53,17 @@ __global__ void k_l0_masks(int M, int B, const int* __restrict__ part_map, const
     if (v < 0) return;
     int bank = s / B, lane = s % B;
     unsigned msk = 1u << lane, st = nstart[v], nn = nnum[v];
-    for (unsigned i = 0; i < nn; ++i) {
-        int su = real_map[nlist[st + i]];
-        if (su / B == bank) msk |= 1u << (su % B);
+    // neighbours eight at a time: their ids, then their slots, are loaded
+    // before any is used, so the gathers overlap (the OR is order-free)
+    for (unsigned i0 = 0; i0 < nn; i0 += kNbrBatch) {
+        int ids[kNbrBatch], su[kNbrBatch];
+#pragma unroll
+        for (int k = 0; k < kNbrBatch; ++k) ids[k] = (i0 + k < nn) ? (int)nlist[st + i0 + k] : -1;
+#pragma unroll
+        for (int k = 0; k < kNbrBatch; ++k) su[k] = ids[k] >= 0 ? real_map[ids[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < kNbrBatch; ++k)
+            if (ids[k] >= 0 && su[k] / B == bank) msk |= 1u << (su[k] % B);
     }
     fine_mask[v] = msk;
 }
@@ -198,9 +208,15 @@ __global__ void k_lx_masks(int n, int B, int level, const unsigned* __restrict__
     unsigned msk = 0u;
     if (valid) {
         unsigned st = nstart[v], nn = nnum[v];
-        for (unsigned i = 0; i < nn; ++i) {
-            int cu = cs[nlist[st + i]];
-            if (cu / B == ci / B) msk |= 1u << (cu % B);
+        for (unsigned i0 = 0; i0 < nn; i0 += kNbrBatch) {  // as in k_l0_masks
+            int ids[kNbrBatch], cu[kNbrBatch];
+#pragma unroll
+            for (int k = 0; k < kNbrBatch; ++k) ids[k] = (i0 + k < nn) ? (int)nlist[st + i0 + k] : -1;
+#pragma unroll
+            for (int k = 0; k < kNbrBatch; ++k) cu[k] = ids[k] >= 0 ? cs[ids[k]] : -1;
+#pragma unroll
+            for (int k = 0; k < kNbrBatch; ++k)
+                if (ids[k] >= 0 && cu[k] / B == ci / B) msk |= 1u << (cu[k] % B);
         }
     }
     int first = __shfl(ci, __ffsll((long long)__ballot(valid)) - 1);

@@ -7,6 +7,9 @@
 // fixed.
 #include "spmv_sym.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 namespace sgipc {
 namespace {
 
@@ -43,30 +46,21 @@ __global__ void k_low_count(int K, const int* __restrict__ r, const int* __restr
     if (r[k] != c[k]) atomicAdd(cnt + c[k], 1);
 }
 
-__global__ void k_low_place(int K, const int* __restrict__ r, const int* __restrict__ c, const int* __restrict__ lptr,
-                            int* __restrict__ cursor, int2* __restrict__ lref) {
+// sort key of upper block k for the lower references: its column, or nb
+// (after every column) for a diagonal block, which has none
+__global__ void k_low_keys(int K, const int* __restrict__ r, const int* __restrict__ c, int nb,
+                           unsigned* __restrict__ key) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    int i = r[k], j = c[k];
-    if (i == j) return;
-    int pos = lptr[j] + atomicAdd(cursor + j, 1);
-    lref[pos] = make_int2(k, i);
+    key[k] = r[k] == c[k] ? (unsigned)nb : (unsigned)c[k];
 }
 
-// order each row's references by block index (= by row i): fixed sum order
-__global__ void k_low_sort(int nb, const int* __restrict__ lptr, int2* __restrict__ lref) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nb) return;
-    int b = lptr[j], e = lptr[j + 1];
-    for (int a = b + 1; a < e; ++a) {
-        int2 v = lref[a];
-        int q = a - 1;
-        while (q >= b && lref[q].x > v.x) {
-            lref[q + 1] = lref[q];
-            --q;
-        }
-        lref[q + 1] = v;
-    }
+// references in (column, block index) order: block ids from the stable sort
+__global__ void k_low_fill(int nlow, const int* __restrict__ kk, const int* __restrict__ r, int2* __restrict__ lref) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nlow) return;
+    const int k = kk[q];
+    lref[q] = make_int2(k, r[k]);
 }
 
 // per-row entry list: transposed references, then the row's upper blocks
@@ -154,13 +148,26 @@ void bsr_build(Ctx& c) {
     S.K = K;
     S.nlow = nlow;
     int2* lref = S.lref.reserve((size_t)nlow + 1);
-    fill<int>(cnt, 0, nb, s);
     if (K > 0) {
-        hipLaunchKernelGGL(k_low_place, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, lptr, cnt,
-                           lref);
+        // the lower references of row j are the off-diagonal blocks of column
+        // j, in block order (= row order): one stable radix sort by column
+        int bc = 1;
+        while ((1ll << bc) <= (long long)nb) ++bc;
+        unsigned* key = S.lkey.reserve((size_t)K + 1);
+        unsigned* key2 = S.lkey2.reserve((size_t)K + 1);
+        int* kk = S.lidx.reserve((size_t)K + 1);
+        rocprim::counting_iterator<int> ids(0);
+        size_t bytes = 0;
+        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
+        char* tmp = S.ltmp.reserve(bytes + 1);
+        hipLaunchKernelGGL(k_low_keys, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, nb, key);
         SGIPC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_low_sort, dim3(div_up(nb, 256)), dim3(256), 0, s, nb, lptr, lref);
-        SGIPC_LAUNCH_CHECK();
+        SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
+        if (nlow > 0) {
+            hipLaunchKernelGGL(k_low_fill, dim3(div_up(nlow, 256)), dim3(256), 0, s, nlow, (const int*)kk, c.t_rows,
+                               lref);
+            SGIPC_LAUNCH_CHECK();
+        }
     }
     int* eptr = S.eptr.reserve((size_t)nb + 2);
     int2* ent = S.ent.reserve((size_t)nlow + (size_t)K + 1);
