@@ -762,6 +762,17 @@ __global__ void __launch_bounds__(64) k_bank_inverse_reg48(int nbanks, const dou
 }
 
 // ---- apply -----------------------------------------------------------------
+// value of v in lane j of the wavefront (j uniform)
+__device__ inline float lane_bcast(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double lane_bcast(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // address-space-qualified pointers for the direct global -> LDS load builtin
 typedef __attribute__((address_space(1))) const void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
@@ -781,7 +792,7 @@ __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_solve(const T* __re
     using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     constexpr int PER = (NV + NT - 1) / NT;
     __shared__ V4 As[PER * NT];  // padded to whole 1 KB wave chunks
-    __shared__ T rs[D];
+    __shared__ T rs[B == 16 ? 1 : D];
     const int b = blockIdx.x, i = threadIdx.x;
     const V4* src = reinterpret_cast<const V4*>(inv + (size_t)b * NP);
     // Direct global -> LDS loads (16 B per lane, one 1 KB chunk per wave and
@@ -795,13 +806,26 @@ __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_solve(const T* __re
         __builtin_amdgcn_global_load_lds((gptr_t)(src + min(q0 + lane, NV - 1)), (lptr_t)(As + q0), 16, 0, 0);
     }
     __builtin_amdgcn_s_waitcnt(0);
-    if (i < D) rs[i] = rv;
+    if constexpr (B != 16) {
+        if (i < D) rs[i] = rv;
+    }
     __syncthreads();
     if (i >= D) return;
     const T* A = reinterpret_cast<const T*>(As);
+    // Row i of the symmetric bank from the packed upper triangle: element
+    // (j, i) for j <= i, then (i, j); the packed index advances by D - 1 - j
+    // while j < i and by 1 after.  r_j comes from lane j's register when the
+    // bank fits one wavefront (B = 16), from LDS otherwise.
     T acc = 0;
+    int idx = i;  // (0, i)
 #pragma unroll 8
-    for (int j = 0; j < D; ++j) acc += A[j <= i ? sym_index<D>(j, i) : sym_index<D>(i, j)] * rs[j];
+    for (int j = 0; j < D; ++j) {
+        T rj;
+        if constexpr (B == 16) rj = lane_bcast(rv, j);
+        else rj = rs[j];
+        acc += A[idx] * rj;
+        idx += (j < i) ? (D - 1 - j) : 1;
+    }
     Z[(size_t)b * D + i] = acc;
 }
 
