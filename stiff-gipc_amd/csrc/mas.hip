@@ -871,6 +871,16 @@ __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, co
                                      const int* __restrict__ l1lo, const int* __restrict__ l1hi, T* __restrict__ R,
                                      T (*sv)[kTB], T (*l1v)[32], F value) {
     const int t = threadIdx.x;
+    // the level 1 -> 2 mapping of this bank's nodes depends only on b: load
+    // it first, so its latency overlaps the level 0 -> 1 pass
+    const int loc2 = b * B + t;
+    const bool valid = L >= 3 && t < B && loc2 < cnt1;
+    int q = -1;
+    unsigned mask2 = 0u;
+    if (valid) {
+        q = going_next[(size_t)beg1 + loc2];
+        mask2 = comp_mask[(size_t)beg1 + loc2];
+    }
     const int lo = l1lo[b] * B, hi = l1hi[b] * B;
     for (int c0 = lo; c0 < hi; c0 += kTB) {
         int sl = c0 + t;
@@ -896,14 +906,6 @@ __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, co
         __syncthreads();
     }
     if (L < 3) return;
-    int loc = b * B + t;
-    bool valid = t < B && loc < cnt1;
-    int q = -1;
-    unsigned mask = 0u;
-    if (valid) {
-        q = going_next[(size_t)beg1 + loc];
-        mask = comp_mask[(size_t)beg1 + loc];
-    }
     if (t < B) {
         sv[0][t] = valid ? l1v[0][t] : (T)0;
         sv[1][t] = valid ? l1v[1][t] : (T)0;
@@ -911,7 +913,7 @@ __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, co
     }
     __syncthreads();
     T s0, s1, s2;
-    if (valid && component_sum<T>(mask, sv[0], sv[1], sv[2], B, s0, s1, s2)) {
+    if (valid && component_sum<T>(mask2, sv[0], sv[1], sv[2], B, s0, s1, s2)) {
         R[3 * (size_t)q] = s0; R[3 * (size_t)q + 1] = s1; R[3 * (size_t)q + 2] = s2;
     }
     __syncthreads();
@@ -960,11 +962,11 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
     const unsigned* __restrict__ fine_mask, const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
     const int* __restrict__ l1lo, const int* __restrict__ l1hi, int nb1, int g1, T* __restrict__ R) {
     if (ctrl[C_STOP]) return;
-    __shared__ double sh[kTB / 64];
+    __shared__ double sh[2 * (kTB / 64)];
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
-    const double rz = sum_partials(part_rz, np_rz, sh);
-    const double pap = sum_partials(part_pap, np_pap, sh);
+    double rz, pap;
+    sum_partials2(part_rz, np_rz, part_pap, np_pap, sh, rz, pap);
     const double al = rz / pap;
     double acc = 0.0;
     if ((int)blockIdx.x < g1) {

@@ -66,12 +66,12 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap(int nb, const int* __rest
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
         return;
     }
-    __shared__ double sh[kThreads / 64];
+    __shared__ double sh[2 * (kThreads / 64)];
     const bool first = iter == 1;
     double be = 0.0;
     if (!first) {
-        const double rn = sum_partials(rz_last, np_rz, sh);
-        const double ro = sum_partials(rz_prev, np_rz, sh);
+        double rn, ro;
+        sum_partials2(rz_last, np_rz, rz_prev, np_rz, sh, rn, ro);
         be = rn / ro;
     }
     auto zv = [z](size_t i) { return z[i]; };
@@ -113,9 +113,9 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
                                                         const double* __restrict__ part_rz, int np_rz,
                                                         double* __restrict__ part_rr, int* __restrict__ ctrl) {
     if (ctrl[C_STOP]) return;
-    __shared__ double sh[kThreads / 64];
-    double rz = sum_partials(part_rz, np_rz, sh);
-    double pap = sum_partials(part_pap, np_pap, sh);
+    __shared__ double sh[2 * (kThreads / 64)];
+    double rz, pap;
+    sum_partials2(part_rz, np_rz, part_pap, np_pap, sh, rz, pap);
     double al = rz / pap;
     double v = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
@@ -305,12 +305,12 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* _
         if (blockIdx.x == 0 && threadIdx.x == 0) { ctrl[C_NOCONV] = 1; ctrl[C_STOP] = ctrl[C_MAXIT]; }
         return;
     }
-    __shared__ double sh[kThreads / 64];
+    __shared__ double sh[2 * (kThreads / 64)];
     const bool first = iter == 1;
     double be = 0.0;
     if (!first) {
-        const double rn = sum_partials(rz_last, np_rz, sh);
-        const double ro = sum_partials(rz_prev, np_rz, sh);
+        double rn, ro;
+        sum_partials2(rz_last, np_rz, rz_prev, np_rz, sh, rn, ro);
         be = rn / ro;
     }
     constexpr int RPB = kThreads / G;

@@ -28,6 +28,31 @@ __device__ inline double block_sum_all(double v, double* sh) {
     return s;
 }
 
+// two workgroup sums at once (each with the tree of block_sum_all, so the
+// same values), sharing the barriers; sh holds 2 * (waves) doubles
+__device__ inline void block_sum_all2(double& a, double& b, double* sh) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    __syncthreads();
+    if (lane == 0) { sh[w] = a; sh[nw + w] = b; }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < nw; ++i) { sa += sh[i]; sb += sh[nw + i]; }
+    a = sa;
+    b = sb;
+}
+
+// sums of two partial sets, as two sum_partials calls would give them
+__device__ inline void sum_partials2(const double* __restrict__ pa, int na, const double* __restrict__ pb, int nb,
+                                     double* sh, double& a, double& b) {
+    a = 0.0;
+    b = 0.0;
+    for (int i = threadIdx.x; i < na; i += blockDim.x) a += pa[i];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) b += pb[i];
+    block_sum_all2(a, b, sh);
+}
+
 // sum of np partials, computed redundantly (same order) by every workgroup
 __device__ inline double sum_partials(const double* __restrict__ part, int np, double* sh) {
     double v = 0.0;
