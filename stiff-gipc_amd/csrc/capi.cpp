@@ -78,6 +78,16 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         // callers (and torch's default stream) may use on the same buffers
         SGIPC_HIP(hipStreamCreate(&c->own_stream));
         c->stream = c->own_stream;
+        // mixed-precision PCG SpMV: one resident round of 1024-thread
+        // workgroups (two per CU) walking the row chunks; a second, partial
+        // round measured slower
+        int ncu = 0;
+        SGIPC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        if (ncu > 0) c->spmv_blocks = std::min(1024, 2 * ncu);
+        if (const char* g = std::getenv("SGIPC_SPMV_BLOCKS")) {
+            const int v = std::atoi(g);
+            if (v >= 8 && v <= 1024) c->spmv_blocks = v;
+        }
         if (const char* g = std::getenv("SGIPC_SPMV_G")) {
             int v = std::atoi(g);
             if (v == 4 || v == 8 || v == 16) c->spmv_group = v;

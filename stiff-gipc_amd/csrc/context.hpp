@@ -211,6 +211,7 @@ struct Ctx {
     SymBcoo bsr;
     bool have_bsr = false;
     int spmv_group = 8;  // lanes per block row in the PCG SpMV (env SGIPC_SPMV_G: 4, 8, 16)
+    int spmv_blocks = 0; // cap on the mixed PCG SpMV's workgroups: 2 per CU (env SGIPC_SPMV_BLOCKS; 0: one per chunk, at most 1024)
     int spmv_full32 = 1; // mixed precision: row-contiguous FP32 copy of both triangles (env SGIPC_SPMV_FULL32=0: upper BCOO)
     int deterministic = 0;  // 1: MAS projection and barrier gradient summed in a fixed order (bitwise repeatable)
     DBuf<double> grad_scr;               // ordered barrier gradient: per-pair rows (12 doubles per pair)

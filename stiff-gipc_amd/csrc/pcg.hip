@@ -757,7 +757,10 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
     // SpMV: chunks of kThreads / G block rows, a multiple of 8 workgroups (one range per XCD)
     const int G = c.spmv_group;
     const int nchunks = (int)div_up((long long)c.nb, kThreads / G);
-    const int gS = std::min(kPart, std::max(8, (int)div_up(nchunks, 8) * 8));
+    int gS = std::min(kPart, std::max(8, (int)div_up(nchunks, 8) * 8));
+    // the FP32 full-copy SpMV of mixed solves runs best as one resident round
+    // of workgroups walking the chunks (the FP64 kernel as one per chunk)
+    if (o.precision != 0 && c.spmv_blocks > 0) gS = std::min(gS, std::max(8, c.spmv_blocks / 8 * 8));
     st.refinements = 0;
     if (o.precision == 0) {
         auto run64 = [&](auto tag) {
