@@ -154,6 +154,8 @@ __device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_pr
     return stop;
 }
 
+constexpr int kLevelBatch = 8;  // MAS levels whose loads are in flight together in the collect
+
 constexpr int C_PUB = C_NUM + 2;  // publications of the progress words (pinned polling)
 
 // Progress words for the host's pinned-memory polling: the stop and
@@ -195,10 +197,21 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
         V z0, z1, z2;
         int vt = row - sub_off;
         if (vt >= 0 && vt < n_mas) {
+            // levels eight at a time: all node ids, then all Z values, are
+            // loaded before the sums (kept in level order)
             T s0 = 0, s1 = 0, s2 = 0;
-            for (int l = 0; l < L; ++l) {
-                size_t g = 3 * (size_t)node_of[(size_t)l * n_mas + vt];
-                s0 += Z[g]; s1 += Z[g + 1]; s2 += Z[g + 2];
+            for (int l0 = 0; l0 < L; l0 += kLevelBatch) {
+                size_t g[kLevelBatch];
+#pragma unroll
+                for (int k = 0; k < kLevelBatch; ++k)
+                    g[k] = (l0 + k < L) ? 3 * (size_t)node_of[(size_t)(l0 + k) * n_mas + vt] : 0;
+                T za[kLevelBatch], zb[kLevelBatch], zc[kLevelBatch];
+#pragma unroll
+                for (int k = 0; k < kLevelBatch; ++k)
+                    if (l0 + k < L) { za[k] = Z[g[k]]; zb[k] = Z[g[k] + 1]; zc[k] = Z[g[k] + 2]; }
+#pragma unroll
+                for (int k = 0; k < kLevelBatch; ++k)
+                    if (l0 + k < L) { s0 += za[k]; s1 += zb[k]; s2 += zc[k]; }
             }
             z0 = (V)s0; z1 = (V)s1; z2 = (V)s2;
         } else if (row < abd_rows) {
