@@ -16,14 +16,13 @@ namespace sgipc {
 struct SymBcoo {
     int nb = 0;         // block rows
     int K = 0;          // upper blocks
-    int nlow = 0;       // strictly-lower references
     DBuf<int> lptr;     // nb + 1 (build scratch: lower counts)
-    DBuf<int2> lref;    // nlow   (build scratch: lower references)
+    DBuf<int2> lref;    // <= K   (build scratch: lower references)
     DBuf<unsigned> lkey, lkey2;  // K (build scratch: column sort keys)
     DBuf<int> lidx;              // K (build scratch: block ids in column order)
     DBuf<char> ltmp;             // radix sort scratch
     DBuf<int> eptr;     // nb + 1
-    DBuf<int2> ent;     // nlow + K
+    DBuf<int2> ent;     // lower references + K (<= 2K)
     DBuf<float> valf;   // 12 * K FP32 copy of the BCOO values, padded columns (mixed precision)
     bool valf_ok = false;
     // row-contiguous FP32 copy of both triangles in entry order (mixed

@@ -537,7 +537,7 @@ struct Engine {
         auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
         const SymBcoo& S = c.bsr;
         mix((size_t)N); mix((size_t)gN); mix((size_t)gS); mix((size_t)G); mix((size_t)gU); mix((size_t)S.nb); mix((size_t)S.K);
-        mix((size_t)S.nlow); mix((size_t)S.eptr.p); mix((size_t)S.ent.p); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
+        mix((size_t)S.eptr.p); mix((size_t)S.ent.p); mix((size_t)val); mix((size_t)r); mix((size_t)z); mix((size_t)p);
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);

@@ -56,9 +56,10 @@ __global__ void k_low_keys(int K, const int* __restrict__ r, const int* __restri
 }
 
 // references in (column, block index) order: block ids from the stable sort
-__global__ void k_low_fill(int nlow, const int* __restrict__ kk, const int* __restrict__ r, int2* __restrict__ lref) {
+__global__ void k_low_fill(int K, const int* __restrict__ nlow, const int* __restrict__ kk, const int* __restrict__ r,
+                           int2* __restrict__ lref) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nlow) return;
+    if (q >= K || q >= *nlow) return;
     const int k = kk[q];
     lref[q] = make_int2(k, r[k]);
 }
@@ -141,13 +142,11 @@ void bsr_build(Ctx& c) {
         SGIPC_LAUNCH_CHECK();
     }
     exclusive_scan(c, cnt, lptr, nb, dint + 4, s);
-    int nlow = 0;
-    SGIPC_HIP(hipMemcpyAsync(&nlow, dint + 4, sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
+    // the count of lower references (dint[4]) stays on the device: every
+    // off-diagonal block has one, so K bounds it and sizes the buffers
     S.nb = nb;
     S.K = K;
-    S.nlow = nlow;
-    int2* lref = S.lref.reserve((size_t)nlow + 1);
+    int2* lref = S.lref.reserve((size_t)K + 1);
     if (K > 0) {
         // the lower references of row j are the off-diagonal blocks of column
         // j, in block order (= row order): one stable radix sort by column
@@ -163,14 +162,12 @@ void bsr_build(Ctx& c) {
         hipLaunchKernelGGL(k_low_keys, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, nb, key);
         SGIPC_LAUNCH_CHECK();
         SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
-        if (nlow > 0) {
-            hipLaunchKernelGGL(k_low_fill, dim3(div_up(nlow, 256)), dim3(256), 0, s, nlow, (const int*)kk, c.t_rows,
-                               lref);
-            SGIPC_LAUNCH_CHECK();
-        }
+        hipLaunchKernelGGL(k_low_fill, dim3(div_up(K, 256)), dim3(256), 0, s, K, (const int*)(dint + 4),
+                           (const int*)kk, c.t_rows, lref);
+        SGIPC_LAUNCH_CHECK();
     }
     int* eptr = S.eptr.reserve((size_t)nb + 2);
-    int2* ent = S.ent.reserve((size_t)nlow + (size_t)K + 1);
+    int2* ent = S.ent.reserve(2 * (size_t)K + 1);
     hipLaunchKernelGGL(k_entries, dim3(div_up(nb + 1, 256)), dim3(256), 0, s, nb, (const int*)lptr,
                        (const int2*)lref, (const int*)c.uniq_start.p, (const int*)c.t_cols, eptr, ent);
     SGIPC_LAUNCH_CHECK();
@@ -183,7 +180,7 @@ void bsr_build(Ctx& c) {
 const float* bsr_full_f32(Ctx& c, const int** cols) {
     if (!c.have_bsr) bsr_build(c);
     SymBcoo& S = c.bsr;
-    const size_t ne = (size_t)S.nlow + (size_t)S.K;
+    const size_t ne = 2 * (size_t)S.K;  // bound on the entries (lower references + upper blocks)
     float* vf = S.valfull.reserve(kF32FullStride * ne + 4);
     int* cf = S.colfull.reserve(ne + 1);
     if (!S.full_ok && S.nb > 0) {
