@@ -320,12 +320,13 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* _
     }
     __shared__ double sh[2 * (kThreads / 64)];
     const bool first = iter == 1;
+    // beta = rz_last / rz_prev is formed after the first chunk's row sums,
+    // so its partial-sum reduction overlaps the matrix loads (the chunk loop
+    // is uniform over the workgroup, so every thread reaches the barrier).
+    // The FP64 kernel keeps it first: holding its row state across the
+    // reduction cost it occupancy.
     double be = 0.0;
-    if (!first) {
-        double rn, ro;
-        sum_partials2(rz_last, np_rz, rz_prev, np_rz, sh, rn, ro);
-        be = rn / ro;
-    }
+    bool have_be = first;
     constexpr int RPB = kThreads / G;
     const int nchunks = (nb + RPB - 1) / RPB;
     ChunkWalk w = xcd_chunks(nchunks);
@@ -345,6 +346,12 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* _
             if (e < end) full_madd(load_full_entry(vf, e), load_z3(z, colf[e]), s0, s1, s2);
         }
         group_sum3<G, float>(s0, s1, s2);
+        if (!have_be) {
+            double rn, ro;
+            sum_partials2(rz_last, np_rz, rz_prev, np_rz, sh, rn, ro);
+            be = rn / ro;
+            have_be = true;
+        }
         if (sub < 3 && row < nb) {
             const size_t o = 3 * (size_t)row + sub;
             const V aw = (V)(sub == 0 ? s0 : (sub == 1 ? s1 : s2));
