@@ -864,27 +864,48 @@ __device__ inline bool component_sum(unsigned mask, const T* a0, const T* a1, co
 // level-0 value of vertex v at slot sl (own: this workgroup writes the
 // slot's bank).  fine_mask / comp_mask: component lanes of each level-0
 // vertex / coarse node.
+// The index loads of level-1 bank b that do not depend on the residual:
+// its level-0 slot range, the first slot chunk's vertices and the level
+// 1 -> 2 mapping of its nodes.  Issued before a kernel's partial-sum
+// reduction, their latency overlaps it.
+struct BankIdx {
+    int lo, hi, v0, q;
+    unsigned mask2;
+    bool valid;
+};
+__device__ inline BankIdx bank_index(int b, int B, int L, int cnt1, int beg1, const int* __restrict__ part_map,
+                                     const int* __restrict__ going_next, const unsigned* __restrict__ comp_mask,
+                                     const int* __restrict__ l1lo, const int* __restrict__ l1hi) {
+    const int t = threadIdx.x;
+    BankIdx x;
+    const int loc2 = b * B + t;
+    x.valid = L >= 3 && t < B && loc2 < cnt1;
+    x.q = -1;
+    x.mask2 = 0u;
+    if (x.valid) {
+        x.q = going_next[(size_t)beg1 + loc2];
+        x.mask2 = comp_mask[(size_t)beg1 + loc2];
+    }
+    x.lo = l1lo[b] * B;
+    x.hi = l1hi[b] * B;
+    const int sl = x.lo + t;
+    x.v0 = (sl < x.hi) ? part_map[sl] : -1;
+    return x;
+}
+
 template <typename T, typename F>
 __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, const int* __restrict__ part_map,
                                      const int* __restrict__ going_next, const unsigned* __restrict__ fine_mask,
                                      const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
-                                     const int* __restrict__ l1lo, const int* __restrict__ l1hi, T* __restrict__ R,
-                                     T (*sv)[kTB], T (*l1v)[32], F value) {
+                                     const BankIdx& pre, T* __restrict__ R, T (*sv)[kTB], T (*l1v)[32], F value) {
     const int t = threadIdx.x;
-    // the level 1 -> 2 mapping of this bank's nodes depends only on b: load
-    // it first, so its latency overlaps the level 0 -> 1 pass
-    const int loc2 = b * B + t;
-    const bool valid = L >= 3 && t < B && loc2 < cnt1;
-    int q = -1;
-    unsigned mask2 = 0u;
-    if (valid) {
-        q = going_next[(size_t)beg1 + loc2];
-        mask2 = comp_mask[(size_t)beg1 + loc2];
-    }
-    const int lo = l1lo[b] * B, hi = l1hi[b] * B;
+    const bool valid = pre.valid;
+    const int q = pre.q;
+    const unsigned mask2 = pre.mask2;
+    const int lo = pre.lo, hi = pre.hi;
     for (int c0 = lo; c0 < hi; c0 += kTB) {
         int sl = c0 + t;
-        int v = (sl < hi) ? part_map[sl] : -1;
+        int v = c0 == lo ? pre.v0 : ((sl < hi) ? part_map[sl] : -1);
         int p = -1;
         unsigned mask = 0u;
         T v0 = 0, v1 = 0, v2 = 0;
@@ -942,8 +963,9 @@ __global__ void __launch_bounds__(kTB) k_restrict_all(const V* __restrict__ r, l
         const V* rv = r + sub3 + 3 * (size_t)v;
         v0 = (T)rv[0]; v1 = (T)rv[1]; v2 = (T)rv[2];
     };
-    restrict_bank<T>(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, fine_mask, comp_mask, psum0, l1lo, l1hi, R,
-                     sv, l1v, value);
+    const BankIdx pre = bank_index(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, comp_mask, l1lo, l1hi);
+    restrict_bank<T>(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, fine_mask, comp_mask, psum0, pre, R, sv, l1v,
+                     value);
 }
 
 // One PCG step fused with the restriction of its new residual:
@@ -965,6 +987,10 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
     __shared__ double sh[2 * (kTB / 64)];
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
+    // this workgroup's first bank indices, loaded while the dots are reduced
+    BankIdx pre{0, 0, -1, -1, 0u, false};
+    if ((int)blockIdx.x < g1 && (int)blockIdx.x < nb1)
+        pre = bank_index(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, comp_mask, l1lo, l1hi);
     double rz, pap;
     sum_partials2(part_rz, np_rz, part_pap, np_pap, sh, rz, pap);
     const double al = rz / pap;
@@ -984,8 +1010,11 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
             v0 = (T)r0; v1 = (T)r1; v2 = (T)r2;
         };
         for (int b = blockIdx.x; b < nb1; b += g1)
-            restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask, psum0, l1lo, l1hi, R,
-                             sv, l1v, value);
+            restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask, psum0,
+                             b == (int)blockIdx.x ? pre
+                                                  : bank_index(b, B, L, cnt1, beg1, part_map, going_next, comp_mask,
+                                                               l1lo, l1hi),
+                             R, sv, l1v, value);
     } else {
         const long long m0 = 3LL * sub_off, m1 = 3LL * ((long long)sub_off + n_mas), nout = N - (m1 - m0);
         for (long long q = (long long)(blockIdx.x - g1) * kTB + threadIdx.x; q < nout;
