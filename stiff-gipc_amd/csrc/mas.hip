@@ -700,20 +700,23 @@ __device__ __forceinline__ double lane_value(double v) {
 // barrier).  Each entry k of the owner's column is read before the owner
 // overwrites it in the same iteration; the owner starts from the unit
 // vector e_J, as in gj_step, so the arithmetic is identical.
+// (The owner's base is its entry times 0.0 -- one multiply instead of a
+// 64-bit select; the same value up to the sign of a zero.)
 template <int J, int D, int K>
-__device__ __forceinline__ void gj_rl_entry(double (&S)[D], double pv, bool own) {
+__device__ __forceinline__ void gj_rl_entry(double (&S)[D], double pv, double keep) {
     if constexpr (K != J) {
         const double c = lane_value<J>(S[K]);
-        const double base = own ? 0.0 : S[K];
+        const double base = S[K] * keep;
         S[K] = base + -c * pv;
     }
 }
 template <int J, int D, int... K>
 __device__ __forceinline__ void gj_rl_step(double (&S)[D], int i, std::integer_sequence<int, K...>) {
     const bool own = i == J;
+    const double keep = own ? 0.0 : 1.0;
     const double rt = lane_value<J>(S[J]);
     const double pv = (own ? 1.0 : S[J]) / rt;
-    (gj_rl_entry<J, D, K>(S, pv, own), ...);
+    (gj_rl_entry<J, D, K>(S, pv, keep), ...);
     S[J] = pv;
 }
 template <int D, int... J>
