@@ -132,26 +132,45 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
 // follows the x/r update (so no block depends on another's flag write):
 // rule 0 (reference): |rz_prev| <= tol * rz0 on the pre-update rz;
 // rule 1: ||r|| <= tol * ||b|| on the updated residual.  rule < 0: none.
-__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, int np_rz,
-                                 const double* __restrict__ part_rr, int np_rr, double* __restrict__ sc,
-                                 int* __restrict__ ctrl, double* sh) {
+// This thread's share of the stop test's partial sum (the strided part of
+// sum_partials), so a kernel can issue its loads early and reduce late.
+__device__ inline double stop_share(int rule, const double* __restrict__ part_rz_prev, int np_rz,
+                                    const double* __restrict__ part_rr, int np_rr) {
+    double v = 0.0;
+    if (rule == 0) {
+        for (int i = threadIdx.x; i < np_rz; i += blockDim.x) v += part_rz_prev[i];
+    } else if (rule > 0) {
+        for (int i = threadIdx.x; i < np_rr; i += blockDim.x) v += part_rr[i];
+    }
+    return v;
+}
+
+// Stop test from the shares: rule 0 |r.z| <= tol * r0.z0 (previous r.z),
+// rule 1 ||r|| <= tol * ||b||; block 0 records the progress and the stop.
+__device__ inline bool stop_test_share(int rule, double share, double* __restrict__ sc, int* __restrict__ ctrl,
+                                       double* sh) {
     if (rule < 0) return false;
+    const double sum = block_sum_all(share, sh);
     bool stop;
     double q;  // progress: the ratio compared with the tolerance
     if (rule == 0) {
-        const double rz = sum_partials(part_rz_prev, np_rz, sh);
-        stop = fabs(rz) <= sc[S_TOL] * sc[S_RZ0];
-        q = fabs(rz) / sc[S_RZ0];
+        stop = fabs(sum) <= sc[S_TOL] * sc[S_RZ0];
+        q = fabs(sum) / sc[S_RZ0];
     } else {
-        const double rr = sum_partials(part_rr, np_rr, sh);
-        stop = sqrt(rr) <= sc[S_TOL] * sqrt(sc[S_BB]);
-        q = sqrt(rr) / sqrt(sc[S_BB]);
+        stop = sqrt(sum) <= sc[S_TOL] * sqrt(sc[S_BB]);
+        q = sqrt(sum) / sqrt(sc[S_BB]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc[S_PROG] = q;
         if (stop) ctrl[C_STOP] = ctrl[C_ITER] - 1;
     }
     return stop;
+}
+
+__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, int np_rz,
+                                 const double* __restrict__ part_rr, int np_rr, double* __restrict__ sc,
+                                 int* __restrict__ ctrl, double* sh) {
+    return stop_test_share(rule, stop_share(rule, part_rz_prev, np_rz, part_rr, np_rr), sc, ctrl, sh);
 }
 
 constexpr int kLevelBatch = 8;  // MAS levels whose loads are in flight together in the collect
@@ -190,7 +209,9 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
     }
     __shared__ double sh[kThreads / 64];
     double v = 0.0;
-    // z first (its loads do not depend on the stop test), then the test
+    // the stop test's partials are loaded first and reduced after z (whose
+    // loads do not depend on the test), so the two latencies overlap
+    const double share = stop_share(rule, part_rz_prev, np_rz, part_rr, np_rr);
     for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < nb; row += gridDim.x * blockDim.x) {
         size_t o = 3 * (size_t)row;
         V r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
@@ -231,7 +252,7 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
         z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
         v += (double)r0 * (double)z0 + (double)r1 * (double)z1 + (double)r2 * (double)z2;
     }
-    const bool stop = stop_test(rule, part_rz_prev, np_rz, part_rr, np_rr, sc, ctrl, sh);
+    const bool stop = stop_test_share(rule, share, sc, ctrl, sh);
     // the thread that ran the test's writes publishes its outcome
     if (hp && blockIdx.x == 0 && threadIdx.x == 0) publish_progress(ctrl, sc, hp, hq);
     if (stop) return;
