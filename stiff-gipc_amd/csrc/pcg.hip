@@ -843,7 +843,9 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
                                       rf, w.zf.reserve(N + 1), w.pf.reserve(N + 1),
                                       w.Apf.reserve(N + 1),
                                       d64, {parts, parts + kPart}, parts + 2 * kPart, parts + 3 * kPart,
-                                      parts + 4 * kPart, sc, ctrl, bsr_values_f32(c), mas,
+                                      parts + 4 * kPart, sc, ctrl,
+                                      c.spmv_full32 ? nullptr : bsr_values_f32(c),  // the full copy replaces it
+                                      mas,
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
                                       c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
         E.G = G;
