@@ -489,6 +489,9 @@ def main():
         "reference_rule": ref_rule,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("stored counter measurement, profiles/spmv_traffic.json (rocprofv3 --pmc "
+                                        "FETCH_SIZE / WRITE_SIZE runs of this kernel, scaled by a device copy); "
+                                        "not measured in this run") if traffic is not None else None,
                      "avg_ms": round(r["spmv_ms"], 5), "launches": r["spmv_calls"], "alg_bytes": alg,
                      "cache_note": "the SpMV's matrix copy (about 76 MB FP32 / 68 MB FP64) fits MI355X's 256 MB "
                                    "Infinity Cache; FETCH_SIZE counts L2 misses, which it may serve instead of HBM"},

@@ -398,6 +398,8 @@ int sgipc_malloc(sgipc_ctx* ctx, void** d_ptr, size_t bytes);
 int sgipc_free(sgipc_ctx* ctx, void* d_ptr);
 int sgipc_memcpy_h2d(sgipc_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
 int sgipc_memcpy_d2h(sgipc_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+/* device to device, enqueued on the context's stream (no synchronisation) */
+int sgipc_memcpy_d2d(sgipc_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ def lib():
         L.or_classify.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double, i32p, i32p, i32p]
         L.or_classify.restype = C.c_int
         L.or_set_fem_projection.argtypes = [C.c_int]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_threads.restype = C.c_int
         L.or_accd.argtypes = [C.c_int, f64p, f64p, C.c_double]
         L.or_accd.restype = C.c_double
         L.or_ccd_step_bound.argtypes = [C.c_int, i32p, C.c_int, i32p, f64p, f64p, C.c_double]
@@ -317,6 +319,30 @@ class fem_projection:
 
     def __exit__(self, *a):
         lib().or_set_fem_projection(1)
+
+
+def set_threads(t):
+    """Threads of the heavy stages (FEM assembly, convert, MAS build, PCG);
+    1 runs the serial restatement."""
+    lib().or_set_threads(int(t))
+
+
+def get_threads():
+    return lib().or_get_threads()
+
+
+class threads:
+    """with threads(t): ... runs the oracle's heavy stages on t threads."""
+
+    def __init__(self, t):
+        self.t = int(t)
+
+    def __enter__(self):
+        self.prev = get_threads()
+        set_threads(self.t)
+
+    def __exit__(self, *a):
+        set_threads(self.prev)
 
 
 def ground_detect(cand, x, normal, offset, dhat):

@@ -23,7 +23,7 @@
  *                    no penetration over [0, toc] on random motions
  *   exact SNK1 mode  (projection off) Hessian vs central FD of the gradient
  */
-#include "sgipc_oracle.h"
+#include "sgipc_oracle_internal.h"
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -400,7 +400,10 @@ void or_fem_precompute(int n, int T, const double* xr, const uint32_t* tets, dou
 void or_fem_assemble(int n, int T, const double* x, const uint32_t* tets, const double* DmInv,
                      const double* vol, const double* mu, const double* lam, double dt,
                      const int* btype, int off, double* grad, double* vals, int* rows, int* cols) {
-    (void)n;
+    if (or_threads > 1) {
+        or_fem_assemble_mt(n, T, x, tets, DmInv, vol, mu, lam, dt, btype, off, grad, vals, rows, cols);
+        return;
+    }
     double xt[12], g[12], H[144];
     for (int t = 0; t < T; ++t) {
         const uint32_t* v = tets + 4 * t;
@@ -458,6 +461,7 @@ static int kt_cmp(const void* a, const void* b) {
 int or_convert(int nt, const int* rows, const int* cols, const double* vals, int* orow, int* ocol,
                double* oval) {
     if (nt < 1) return 0;
+    if (or_threads > 1) return or_convert_mt(nt, rows, cols, vals, orow, ocol, oval);
     kt_t* e = (kt_t*)malloc(sizeof(kt_t) * (size_t)nt);
     for (int i = 0; i < nt; ++i) {
         e[i].key = ((uint64_t)(uint32_t)rows[i] << 32) + (uint64_t)(uint32_t)cols[i];
@@ -589,23 +593,6 @@ int or_mas_num_levels(int nn, int bank) {
 /* ======================================================================== */
 /* MAS: aggregation tables, Galerkin bank matrices, inverses, apply         */
 /* ======================================================================== */
-struct or_mas {
-    int n, M, B, L, total, nbanks_alloc;
-    int *part_map, *real_map;
-    uint32_t *nstart, *nnum0, *nlist0, *nnum, *nlist;
-    int nlist_len;
-    int* level_size; /* 2*(L+1) : (count, begin) */
-    int* coarse_space; /* L*n */
-    int* going_next; /* cap */
-    int going_cap;
-    int* coarse_table; /* 6*n */
-    uint32_t* fine_mask; /* n */
-    int* prefix0; /* M/B */
-    double* A; /* banks * (3B)^2 dense Galerkin matrices */
-    double* Ainv; /* symmetrised inverse */
-    float* AinvF;
-    int fp32_store;
-};
 
 or_mas* or_mas_create(int n, int M, const int* part_map, const int* real_map,
                       const uint32_t* nstart, const uint32_t* nnum, const uint32_t* nlist,
@@ -643,6 +630,7 @@ void or_mas_destroy(or_mas* m) {
     free(m->nlist0); free(m->nlist); free(m->level_size); free(m->coarse_space);
     free(m->going_next); free(m->coarse_table); free(m->fine_mask); free(m->prefix0);
     free(m->A); free(m->Ainv); free(m->AinvF);
+    free(m->node_vptr); free(m->node_vlist);
     free(m);
 }
 
@@ -866,8 +854,37 @@ static void reorder(or_mas* m, const int* pairs, int np, int node_offset) {
 }
 
 /* coarse node (global id) of vertex v at level l (l=0: slot) */
-static int node_of(const or_mas* m, int v, int l) {
+int or__node_of(const or_mas* m, int v, int l) {
     return l == 0 ? m->real_map[v] : m->coarse_table[6 * (size_t)v + l - 1];
+}
+static int node_of(const or_mas* m, int v, int l) { return or__node_of(m, v, l); }
+
+/* unpivoted in-place Gauss-Jordan inverse of bank b (:630-726); S: (3B)^2
+ * scratch, colm: 3B scratch */
+void or__invert_bank(or_mas* m, int b, double* S, double* colm) {
+    int D = 3 * m->B;
+    size_t bs = (size_t)D * D;
+    memcpy(S, m->A + bs * b, sizeof(double) * bs);
+    for (int i = 0; i < D; ++i) if (S[(size_t)i * D + i] == 0.0) S[(size_t)i * D + i] = 1.0;
+    for (int j = 0; j < D; ++j) {
+        double rt = S[(size_t)j * D + j];
+        for (int i = 0; i < D; ++i) colm[i] = S[(size_t)i * D + j];
+        for (int i = 0; i < D; ++i) S[(size_t)i * D + j] = (i == j) ? 1.0 : 0.0;
+        for (int i = 0; i < D; ++i) S[(size_t)j * D + i] /= rt;
+        for (int k = 0; k < D; ++k) {
+            if (k == j) continue;
+            double rate = -colm[k];
+            for (int i = 0; i < D; ++i) S[(size_t)k * D + i] += rate * S[(size_t)j * D + i];
+        }
+    }
+    double* Ai = m->Ainv + bs * b;
+    float* Af = m->AinvF + bs * b;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            double v = i <= j ? S[(size_t)i * D + j] : S[(size_t)j * D + i];
+            Ai[(size_t)i * D + j] = v;
+            Af[(size_t)i * D + j] = (float)v;
+        }
 }
 
 int or_mas_build(or_mas* m, int K, const int* rows, const int* cols, const double* vals,
@@ -885,6 +902,11 @@ int or_mas_build(or_mas* m, int K, const int* rows, const int* cols, const doubl
     }
     memset(m->A, 0, sizeof(double) * bs * nbanks);
     m->fp32_store = fp32_store;
+    m->node_lists_ok = 0;
+    if (or_threads > 1) {
+        or_mas_project_invert_mt(m, K, rows, cols, vals, offset);
+        return m->total;
+    }
     /* Galerkin projection A_l = R_l A R_l^T restricted to same-bank blocks
      * (PrepareHessian_bcoo, MASPreconditioner.cu:1816-2062) */
     for (int l = 0; l < m->L; ++l) {
@@ -909,29 +931,7 @@ int or_mas_build(or_mas* m, int K, const int* rows, const int* cols, const doubl
     /* unpivoted in-place Gauss-Jordan inverse per bank (:630-726) */
     double* S = (double*)malloc(sizeof(double) * bs);
     double* colm = (double*)malloc(sizeof(double) * D);
-    for (int b = 0; b < nbanks; ++b) {
-        memcpy(S, m->A + bs * b, sizeof(double) * bs);
-        for (int i = 0; i < D; ++i) if (S[(size_t)i * D + i] == 0.0) S[(size_t)i * D + i] = 1.0;
-        for (int j = 0; j < D; ++j) {
-            double rt = S[(size_t)j * D + j];
-            for (int i = 0; i < D; ++i) colm[i] = S[(size_t)i * D + j];
-            for (int i = 0; i < D; ++i) S[(size_t)i * D + j] = (i == j) ? 1.0 : 0.0;
-            for (int i = 0; i < D; ++i) S[(size_t)j * D + i] /= rt;
-            for (int k = 0; k < D; ++k) {
-                if (k == j) continue;
-                double rate = -colm[k];
-                for (int i = 0; i < D; ++i) S[(size_t)k * D + i] += rate * S[(size_t)j * D + i];
-            }
-        }
-        double* Ai = m->Ainv + bs * b;
-        float* Af = m->AinvF + bs * b;
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j < D; ++j) {
-                double v = i <= j ? S[(size_t)i * D + j] : S[(size_t)j * D + i];
-                Ai[(size_t)i * D + j] = v;
-                Af[(size_t)i * D + j] = (float)v;
-            }
-    }
+    for (int b = 0; b < nbanks; ++b) or__invert_bank(m, b, S, colm);
     free(S); free(colm);
     return m->total;
 }
@@ -1050,6 +1050,8 @@ static void apply_pre(int precond, int nb, const double* Dinv, or_mas* mas, int 
 int or_pcg(int K, const int* rows, const int* cols, const double* vals, int nb, const double* b,
            double* x, int precond, or_mas* mas, int mas_fp32, double tol, int rule, int max_iter,
            double* rel_res_out) {
+    if (or_threads > 1)
+        return or_pcg_mt(K, rows, cols, vals, nb, b, x, precond, mas, mas_fp32, tol, rule, max_iter, rel_res_out);
     int N = 3 * nb;
     double *r = (double*)malloc(sizeof(double) * N), *z = (double*)malloc(sizeof(double) * N);
     double *p = (double*)malloc(sizeof(double) * N), *Ap = (double*)malloc(sizeof(double) * N);

@@ -53,6 +53,11 @@ double or_ccd_step_bound(int npt, const int* pt, int nee, const int* ee, const d
                          double slack);
 extern int or_fem_project;
 void or_set_fem_projection(int on);
+/* Threads of the CPU baseline: t > 1 runs or_fem_assemble, or_convert,
+ * or_mas_build and or_pcg on t threads (sgipc_oracle_mt.c); 1 (default) the
+ * serial code. */
+void or_set_threads(int t);
+int or_get_threads(void);
 /* ground plane n.x = o (sgipc_oracle_ground.c) */
 int or_ground_detect(int m, const uint32_t* cand, const double* x, const double nrm[3], double o, double dhat,
                      uint32_t* out);

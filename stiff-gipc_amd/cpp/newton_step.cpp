@@ -1,7 +1,11 @@
-// newton_step -- one Newton step of a FEM scene through the C++ facade
-// (include/sgipc_linear_system.hpp), driving the library the way GIPC's
-// Newton loop drives its linear system: kinetic gradient, SNK1 assembly,
-// mass diagonal, right-hand side, then solve_linear_system().
+// newton_step -- one Newton step of a FEM scene through the reference-shaped
+// C++ API (include/gipc_linear_system.hpp): an FEM subsystem, the PCG
+// solver and the configured preconditioner registered with
+// GlobalLinearSystem::create<T>(), the assembly written into the caller's
+// triplet store (kinetic gradient, SNK1 terms, mass diagonal), then
+// solve_linear_system(), which assembles the right-hand side through the
+// subsystem, converts, builds the preconditioner, solves and hands the
+// subsystem its solution.
 //
 // usage: newton_step <scene.bin> <out.bin>
 // scene.bin (little endian): "SGIPCSC1", int32 n, T, bank, precond, stop_rule,
@@ -14,7 +18,7 @@
 #include <cstring>
 #include <vector>
 
-#include "sgipc_linear_system.hpp"
+#include "gipc_linear_system.hpp"
 
 namespace {
 
@@ -33,18 +37,53 @@ struct Reader {
     }
 };
 
-// device copy of a host vector, freed with the context
-template <typename T>
-T* upload(sgipc::GlobalLinearSystem& sys, const std::vector<T>& h, std::vector<void*>& owned) {
-    void* d = nullptr;
-    size_t bytes = sizeof(T) * (h.empty() ? 1 : h.size());
-    sgipc::check(sgipc_malloc(sys.handle(), &d, bytes), sys.handle(), "sgipc_malloc");
-    owned.push_back(d);
-    if (!h.empty()) sgipc::check(sgipc_memcpy_h2d(sys.handle(), d, h.data(), bytes), sys.handle(), "sgipc_memcpy_h2d");
-    return static_cast<T*>(d);
-}
+// device buffers freed with the context
+struct DeviceArena {
+    sgipc_ctx* ctx;
+    std::vector<void*> owned;
+    template <typename T>
+    T* upload(const std::vector<T>& h) {
+        void* d = nullptr;
+        size_t bytes = sizeof(T) * (h.empty() ? 1 : h.size());
+        gipc::check(sgipc_malloc(ctx, &d, bytes), ctx, "sgipc_malloc");
+        owned.push_back(d);
+        if (!h.empty()) gipc::check(sgipc_memcpy_h2d(ctx, d, h.data(), bytes), ctx, "sgipc_memcpy_h2d");
+        return static_cast<T*>(d);
+    }
+    ~DeviceArena() {
+        for (void* p : owned) sgipc_free(ctx, p);
+    }
+};
+
+// the FEM state of the simulator (device arrays)
+struct FemState {
+    int n = 0;
+    const double* fb = nullptr;  // barrier gradient (none in this scene)
+    const double* shape = nullptr;
+    const int* btype = nullptr;
+    double* move_dir = nullptr;
+};
 
 }  // namespace
+
+namespace gipc {
+// The simulator-side subsystem (the reference's
+// linear_system/subsystem/fem_linear_subsystem.cu:56-102) over the library.
+class FEMLinearSubsystem : public DiagonalSubsystem {
+public:
+    explicit FEMLinearSubsystem(FemState& s) : m_s(s) {}
+    void report_subsystem_info() override { right_hand_side_dof(3 * (SizeT)m_s.n); }
+    void assemble(DenseVectorView gradient) override {
+        check(sgipc_fem_rhs(ctx(), m_s.n, m_s.fb, m_s.shape, m_s.btype, gradient.data()), ctx(), "sgipc_fem_rhs");
+    }
+    void retrieve_solution(CDenseVectorView dx) override {
+        check(sgipc_memcpy_d2d(ctx(), m_s.move_dir, dx.data(), sizeof(Float) * dx.size()), ctx(), "sgipc_memcpy_d2d");
+    }
+
+private:
+    FemState& m_s;
+};
+}  // namespace gipc
 
 int main(int argc, char** argv) {
     if (argc != 3) {
@@ -70,6 +109,7 @@ int main(int argc, char** argv) {
         precond = rd.one<int>(); stop_rule = rd.one<int>(); max_iter = rd.one<int>();
         dt = rd.one<double>(); tol = rd.one<double>();
         if (n <= 0 || T <= 0) throw std::runtime_error("empty scene");
+        if (precond < SGIPC_PRECOND_NONE || precond > SGIPC_PRECOND_MAS_FP32) throw std::runtime_error("bad preconditioner");
         rd.get(x, 3 * (size_t)n); rd.get(xt, 3 * (size_t)n); rd.get(mass, n); rd.get(btype, n);
         rd.get(tets, 4 * (size_t)T); rd.get(DmInv, 9 * (size_t)T); rd.get(vol, T); rd.get(mu, T); rd.get(lam, T);
         rd.get(part, n);
@@ -80,60 +120,72 @@ int main(int argc, char** argv) {
     }
     std::fclose(f);
 
-    std::vector<void*> owned;
     int rc = 0;
     try {
-        sgipc::GlobalLinearSystem sys(0);
-        try {
-            sys.set_fem_mesh(n, T, tets.data());
-            sys.set_mas_partition(n, T, tets.data(), part.data(), bank);
-            sys.set_preconditioner(precond);
-            sys.config().global_tol_rate = tol;
-            sys.config().stop_rule = stop_rule;
-            sys.config().max_iter = max_iter;
-            const long long cap = 10LL * T + n;
-            double* d_vals = upload(sys, std::vector<double>(9 * (size_t)cap, 0.0), owned);
-            int* d_rows = upload(sys, std::vector<int>((size_t)cap, 0), owned);
-            int* d_cols = upload(sys, std::vector<int>((size_t)cap, 0), owned);
-            sys.set_triplet_matrix(d_vals, d_rows, d_cols, cap);
-            double* d_x = upload(sys, x, owned);
-            double* d_xt = upload(sys, xt, owned);
-            double* d_mass = upload(sys, mass, owned);
-            int* d_btype = upload(sys, btype, owned);
-            uint32_t* d_tets = upload(sys, tets, owned);
-            double* d_Dm = upload(sys, DmInv, owned);
-            double* d_vol = upload(sys, vol, owned);
-            double* d_mu = upload(sys, mu, owned);
-            double* d_lam = upload(sys, lam, owned);
-            std::vector<double> zeros(3 * (size_t)n, 0.0);
-            double* d_shape = upload(sys, zeros, owned);
-            double* d_b = upload(sys, zeros, owned);
-            double* d_dx = upload(sys, zeros, owned);
+        gipc::GlobalLinearSystem sys(0);
+        DeviceArena mem{sys.handle(), {}};
+        gipc::check(sgipc_set_fem_mesh(sys.handle(), n, T, tets.data()), sys.handle(), "sgipc_set_fem_mesh");
+        MASPreconditioner mp;
+        mp.init_partition(n, T, tets.data(), part.data(), bank);
+        uint32_t cp_num[5] = {0, 0, 0, 0, 0};
 
-            sys.begin_assembly();
-            sys.kinetic_gradient(n, d_x, d_xt, d_mass, d_shape);
-            sys.assemble_fem(T, d_x, d_tets, d_Dm, d_vol, d_mu, d_lam, dt, d_btype, d_shape);
-            sys.mass_diagonal(n, d_mass);
-            sys.assemble_rhs(n, nullptr, d_shape, d_btype, d_b);
-            const std::size_t iters = sys.solve_linear_system(d_b, d_dx);
-            std::vector<double> dx(3 * (size_t)n);
-            sgipc::check(sgipc_memcpy_d2h(sys.handle(), dx.data(), d_dx, sizeof(double) * dx.size()), sys.handle(),
-                         "sgipc_memcpy_d2h");
-            FILE* o = std::fopen(argv[2], "wb");
-            if (!o) throw std::runtime_error("cannot write output");
-            int head[2] = {(int)iters, sys.last_stats().converged};
-            double rr = sys.last_stats().rel_residual;
-            bool ok = std::fwrite(head, sizeof(int), 2, o) == 2 && std::fwrite(&rr, sizeof(double), 1, o) == 1 &&
-                      std::fwrite(dx.data(), sizeof(double), dx.size(), o) == dx.size();
-            std::fclose(o);
-            if (!ok) throw std::runtime_error("short write");
-            std::printf("newton_step: n=%d T=%d iterations=%zu rel_residual=%.3e\n", n, T, iters, rr);
-        } catch (...) {
-            for (void* p : owned) sgipc_free(sys.handle(), p);
-            throw;
-        }
-        for (void* p : owned) sgipc_free(sys.handle(), p);
-    } catch (const sgipc::Error& e) {
+        const long long cap = 10LL * T + n;
+        gipc::GIPCTripletMatrix trip(mem.upload(std::vector<double>(9 * (size_t)cap, 0.0)),
+                                     mem.upload(std::vector<int>((size_t)cap, 0)),
+                                     mem.upload(std::vector<int>((size_t)cap, 0)), cap);
+        sys.gipc_global_triplet = &trip;
+        double* d_x = mem.upload(x);
+        double* d_xt = mem.upload(xt);
+        double* d_mass = mem.upload(mass);
+        int* d_btype = mem.upload(btype);
+        uint32_t* d_tets = mem.upload(tets);
+        double* d_Dm = mem.upload(DmInv);
+        double* d_vol = mem.upload(vol);
+        double* d_mu = mem.upload(mu);
+        double* d_lam = mem.upload(lam);
+        std::vector<double> zeros(3 * (size_t)n, 0.0);
+        FemState fs;
+        fs.n = n;
+        double* d_shape = mem.upload(zeros);
+        fs.shape = d_shape;
+        fs.btype = d_btype;
+        fs.move_dir = mem.upload(zeros);
+
+        // linear system set-up, as GIPC::create_LinearSystem does it
+        auto& fem = sys.create<gipc::FEMLinearSubsystem>(fs);
+        gipc::PCGSolverConfig cfg;
+        cfg.global_tol_rate = tol;
+        cfg.stop_rule = stop_rule;
+        cfg.max_iter = max_iter;
+        auto& pcg = sys.create<gipc::PCGSolver>(cfg);
+        if (precond == SGIPC_PRECOND_MAS || precond == SGIPC_PRECOND_MAS_FP32)
+            sys.create<gipc::MAS_Preconditioner>(fem, mp, d_mass, cp_num, precond);
+        else if (precond == SGIPC_PRECOND_DIAG)
+            sys.create<gipc::DiagPreconditioner>();
+
+        // GIPC::computeGradientAndHessian
+        gipc::Assembler as(sys);
+        as.begin();
+        as.kinetic_gradient(n, d_x, d_xt, d_mass, d_shape);
+        as.fem(T, d_x, d_tets, d_Dm, d_vol, d_mu, d_lam, dt, d_btype, d_shape);
+        as.mass_diagonal(n, d_mass);
+        // GIPC::calculateMovingDirection
+        const gipc::SizeT iters = sys.solve_linear_system();
+
+        std::vector<double> dx(3 * (size_t)n);
+        gipc::check(sgipc_memcpy_d2h(sys.handle(), dx.data(), fs.move_dir, sizeof(double) * dx.size()), sys.handle(),
+              "sgipc_memcpy_d2h");
+        FILE* o = std::fopen(argv[2], "wb");
+        if (!o) throw std::runtime_error("cannot write output");
+        int head[2] = {(int)iters, pcg.last_stats().converged};
+        double rr = pcg.last_stats().rel_residual;
+        bool ok = std::fwrite(head, sizeof(int), 2, o) == 2 && std::fwrite(&rr, sizeof(double), 1, o) == 1 &&
+                  std::fwrite(dx.data(), sizeof(double), dx.size(), o) == dx.size();
+        std::fclose(o);
+        if (!ok) throw std::runtime_error("short write");
+        std::printf("newton_step: n=%d T=%d K=%d iterations=%zu rel_residual=%.3e\n", n, T, trip.h_unique_key_number,
+                    iters, rr);
+    } catch (const gipc::Error& e) {
         std::fprintf(stderr, "sgipc error %d: %s\n", e.code(), e.what());
         rc = 3;
     } catch (const std::exception& e) {

@@ -547,17 +547,7 @@ int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts
     CHECK_CTX(c);
     return guard(c, [&] {
         if (!b || !x) throw sgipc::arg_error("solve: null vector");
-        sgipc::SolveOpts so;
-        if (o) {
-            so.precision = o->precision;
-            so.stop_rule = o->stop_rule;
-            so.tol = o->tol;
-            so.max_iter_ratio = o->max_iter_ratio;
-            so.max_iter = o->max_iter;
-            so.check_every = o->check_every;
-        }
-        if (so.stop_rule != 0 && so.stop_rule != 1) throw sgipc::arg_error("solve: unknown stop rule");
-        if (!(so.tol > 0.0)) throw sgipc::arg_error("solve: tolerance must be positive");
+        const sgipc::SolveOpts so = solve_opts_of(o);
         set_device(c);
         sgipc::SolveStats s2;
         sgipc::prof_begin(*c, sgipc::PROF_SOLVE);
@@ -696,6 +686,14 @@ int sgipc_memcpy_d2h(sgipc_ctx* c, void* h, const void* d, size_t bytes) {
         set_device(c);
         SGIPC_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
         SGIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+int sgipc_memcpy_d2d(sgipc_ctx* c, void* dst, const void* src, size_t bytes) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (bytes && (!dst || !src)) throw sgipc::arg_error("memcpy_d2d: null pointer");
+        set_device(c);
+        if (bytes) SGIPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
     });
 }
 

@@ -864,9 +864,14 @@ __device__ inline bool component_sum(unsigned mask, const T* a0, const T* a1, co
 
 // level-0 slots -> level-1 sums of the nodes of level-1 bank b, then
 // level 1 -> 2 inside that bank.  `value(v, sl, own, ...)` gives the
-// level-0 value of vertex v at slot sl (own: this workgroup writes the
-// slot's bank).  fine_mask / comp_mask: component lanes of each level-0
-// vertex / coarse node.
+// level-0 value of vertex v at slot sl.  A vertex is owned by exactly one
+// workgroup: the one whose level-1 bank holds the vertex's level-1 node.
+// Only the owner reads or writes the vertex's data (value() is called with
+// own = false for the others and their values are never summed: a
+// component's lanes all share one level-1 node), so a level-0 bank whose
+// coarse nodes straddle two level-1 banks is split between their
+// workgroups with no slot touched twice.  fine_mask / comp_mask: component
+// lanes of each level-0 vertex / coarse node.
 // The index loads of level-1 bank b that do not depend on the residual:
 // its level-0 slot range, the first slot chunk's vertices and the level
 // 1 -> 2 mapping of its nodes.  Issued before a kernel's partial-sum
@@ -899,8 +904,8 @@ __device__ inline BankIdx bank_index(int b, int B, int L, int cnt1, int beg1, co
 template <typename T, typename F>
 __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, const int* __restrict__ part_map,
                                      const int* __restrict__ going_next, const unsigned* __restrict__ fine_mask,
-                                     const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
-                                     const BankIdx& pre, T* __restrict__ R, T (*sv)[kTB], T (*l1v)[32], F value) {
+                                     const unsigned* __restrict__ comp_mask, const BankIdx& pre, T* __restrict__ R,
+                                     T (*sv)[kTB], T (*l1v)[32], F value) {
     const int t = threadIdx.x;
     const bool valid = pre.valid;
     const int q = pre.q;
@@ -913,11 +918,14 @@ __device__ inline void restrict_bank(int b, int B, int L, int cnt1, int beg1, co
         unsigned mask = 0u;
         T v0 = 0, v1 = 0, v2 = 0;
         if (v >= 0) {
-            bool own = psum0[sl / B] / B == b;
+            const int loc = going_next[v] - beg1;
+            const bool own = loc / B == b;
             value(v, sl, own, v0, v1, v2);
-            if (own) { R[3 * (size_t)sl] = v0; R[3 * (size_t)sl + 1] = v1; R[3 * (size_t)sl + 2] = v2; }
-            int loc = going_next[v] - beg1;
-            if (loc / B == b) { p = loc; mask = fine_mask[v]; }
+            if (own) {
+                R[3 * (size_t)sl] = v0; R[3 * (size_t)sl + 1] = v1; R[3 * (size_t)sl + 2] = v2;
+                p = loc;
+                mask = fine_mask[v];
+            }
         }
         sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
         __syncthreads();
@@ -956,18 +964,19 @@ __global__ void __launch_bounds__(kTB) k_restrict_all(const V* __restrict__ r, l
                                                        const int* __restrict__ going_next,
                                                        const unsigned* __restrict__ fine_mask,
                                                        const unsigned* __restrict__ comp_mask,
-                                                       const int* __restrict__ psum0, const int* __restrict__ l1lo,
-                                                       const int* __restrict__ l1hi, T* __restrict__ R,
+                                                       const int* __restrict__ l1lo, const int* __restrict__ l1hi,
+                                                       T* __restrict__ R,
                                                        const int* __restrict__ ctrl) {
     if (ctrl && ctrl[0]) return;
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
-    auto value = [&](int v, int, bool, T& v0, T& v1, T& v2) {
+    auto value = [&](int v, int, bool own, T& v0, T& v1, T& v2) {
+        if (!own) return;
         const V* rv = r + sub3 + 3 * (size_t)v;
         v0 = (T)rv[0]; v1 = (T)rv[1]; v2 = (T)rv[2];
     };
     const BankIdx pre = bank_index(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, comp_mask, l1lo, l1hi);
-    restrict_bank<T>(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, fine_mask, comp_mask, psum0, pre, R, sv, l1v,
+    restrict_bank<T>(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, fine_mask, comp_mask, pre, R, sv, l1v,
                      value);
 }
 
@@ -984,8 +993,8 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
     const double* __restrict__ part_pap, int np_pap, const double* __restrict__ part_rz, int np_rz,
     double* __restrict__ part_rr, int* __restrict__ ctrl, int sub_off, int n_mas, int B, int L,
     const int* __restrict__ ls, const int* __restrict__ part_map, const int* __restrict__ going_next,
-    const unsigned* __restrict__ fine_mask, const unsigned* __restrict__ comp_mask, const int* __restrict__ psum0,
-    const int* __restrict__ l1lo, const int* __restrict__ l1hi, int nb1, int g1, T* __restrict__ R) {
+    const unsigned* __restrict__ fine_mask, const unsigned* __restrict__ comp_mask, const int* __restrict__ l1lo,
+    const int* __restrict__ l1hi, int nb1, int g1, T* __restrict__ R) {
     if (ctrl[C_STOP]) return;
     __shared__ double sh[2 * (kTB / 64)];
     __shared__ T sv[3][kTB];
@@ -1000,20 +1009,20 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
     double acc = 0.0;
     if ((int)blockIdx.x < g1) {
         const int cnt1 = ls[2], beg1 = ls[3];
+        // only the vertex's owner reads and updates r in place (see restrict_bank)
         auto value = [&](int v, int, bool own, T& v0, T& v1, T& v2) {
+            if (!own) return;
             const size_t o = 3 * ((size_t)sub_off + v);
             V r0 = (V)((double)r[o] - al * (double)Ap[o]);
             V r1 = (V)((double)r[o + 1] - al * (double)Ap[o + 1]);
             V r2 = (V)((double)r[o + 2] - al * (double)Ap[o + 2]);
-            if (own) {
-                x[o] += al * (double)p[o]; x[o + 1] += al * (double)p[o + 1]; x[o + 2] += al * (double)p[o + 2];
-                r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
-                acc += (double)r0 * (double)r0 + (double)r1 * (double)r1 + (double)r2 * (double)r2;
-            }
+            x[o] += al * (double)p[o]; x[o + 1] += al * (double)p[o + 1]; x[o + 2] += al * (double)p[o + 2];
+            r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
+            acc += (double)r0 * (double)r0 + (double)r1 * (double)r1 + (double)r2 * (double)r2;
             v0 = (T)r0; v1 = (T)r1; v2 = (T)r2;
         };
         for (int b = blockIdx.x; b < nb1; b += g1)
-            restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask, psum0,
+            restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask,
                              b == (int)blockIdx.x ? pre
                                                   : bank_index(b, B, L, cnt1, beg1, part_map, going_next, comp_mask,
                                                                l1lo, l1hi),
@@ -1107,7 +1116,7 @@ void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl) {
     if (m.nb1 <= 0) return;
     hipLaunchKernelGGL((k_restrict_all<V, T>), dim3(m.nb1), dim3(kTB), 0, c.stream, r, sub3, m.bank, m.L,
                        (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
-                       (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                       (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p,
                        (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
     SGIPC_LAUNCH_CHECK();
     if (m.L >= 4) {
@@ -1420,14 +1429,14 @@ static void update_restrict_t(Ctx& c, double* x, V* r, const V* p, const V* Ap, 
         hipLaunchKernelGGL((k_update_restrict<V, float>), dim3(g1 + gx), dim3(kTB), 0, c.stream, N, x, r, p, Ap,
                            part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, c.sub_offset, m.n, m.bank, m.L,
                            (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
-                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p,
                            (const int*)m.l1lo.p, (const int*)m.l1hi.p, m.nb1, g1, m.R32.p);
     } else {
         if (sizeof(V) != 8) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
         hipLaunchKernelGGL((k_update_restrict<V, double>), dim3(g1 + gx), dim3(kTB), 0, c.stream, N, x, r, p, Ap,
                            part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, c.sub_offset, m.n, m.bank, m.L,
                            (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
-                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p, (const int*)m.psum0.p,
+                           (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p,
                            (const int*)m.l1lo.p, (const int*)m.l1hi.p, m.nb1, g1, m.R64.p);
     }
     SGIPC_LAUNCH_CHECK();

@@ -575,6 +575,7 @@ struct Engine {
             mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
             mix((size_t)m.nb1); mix((size_t)m.going_next.p); mix((size_t)m.node_of.p); mix((size_t)m.level_size.p);
             mix((size_t)m.part_map.p); mix((size_t)m.psum0.p); mix((size_t)m.l1lo.p); mix((size_t)m.l1hi.p);
+            mix((size_t)m.comp_mask.p); mix((size_t)m.fine_mask.p);
             mix((size_t)m.inv64.p); mix((size_t)m.inv32.p); mix((size_t)m.R64.p); mix((size_t)m.R32.p);
             mix((size_t)m.Z64.p); mix((size_t)m.Z32.p);
         }

@@ -98,6 +98,7 @@ SIGNATURES = {
     "sgipc_free": ([P, P], I),
     "sgipc_memcpy_h2d": ([P, P, P, SZ], I),
     "sgipc_memcpy_d2h": ([P, P, P, SZ], I),
+    "sgipc_memcpy_d2d": ([P, P, P, SZ], I),
 }
 
 

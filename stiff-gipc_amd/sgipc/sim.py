@@ -261,7 +261,8 @@ def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, pr
     (fd, adjoint, relative error) and the loss.  With richardson_above set,
     directions whose error exceeds it are also differenced with h/2 and the
     Richardson value (4 fd(h/2) - fd(h)) / 3, which cancels the O(h^2)
-    truncation term, is compared too: entries then carry (fd_r, rel_r)."""
+    truncation term (a fourth-order stencil), is compared too: entries then
+    carry (fd_r, rel_r).  richardson_above=0 does so for every direction."""
     L, gE, gv = loss_and_gradient(sim, log_E, v0, target, steps, ckpt)
     rng = np.random.default_rng(seed)
     free = sim.free.reshape(-1, 3)
@@ -282,7 +283,7 @@ def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, pr
         fd = central(h)
         ad = float((gE * dE).sum() + (gv * dv).sum())
         rel = abs(fd - ad) / max(abs(ad), 1e-300)
-        if richardson_above is not None and rel > richardson_above:
+        if richardson_above is not None and (richardson_above <= 0 or rel > richardson_above):
             fr = (4.0 * central(0.5 * h) - fd) / 3.0
             out.append((fd, ad, rel, fr, abs(fr - ad) / max(abs(ad), 1e-300)))
         else:
