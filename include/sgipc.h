@@ -93,6 +93,20 @@ int sgipc_build_vertex_neighbors(int n, int T, const uint32_t* h_tets, uint32_t*
  * (StiffGIPC/gl_main.cu:1066-1093).  part_map needs bank*(last part+1). */
 int sgipc_build_bank_map(int n, const int* h_part_id, int bank, int* h_part_map,
                          int part_map_capacity, int* map_size, int* h_real_map);
+/* Vertex partition into subdomains of at most `bank` (16 or 32) vertices;
+ * replaces metis_sort (MeshProcess/metis_partition/src/metis_sort.cpp:25-383:
+ * nParts = ceil(n / (bank - off)), off raised until no part exceeds bank) with
+ * this library's own multilevel recursive bisection of the vertex graph
+ * (edge weight = tets sharing the edge, mesh.cpp:486-549) in place of
+ * METIS_PartGraphKway (node_edge_model.cpp:5-28).  h_part[v] = part of vertex
+ * v; h_order (optional, n entries) = vertex ids sorted by part, ascending id
+ * within a part (TetMesh::sort_index, mesh.cpp:65-88, which sorts unstably):
+ * new vertex i is old vertex h_order[i].  offset (optional) = the off at
+ * which the loop stopped (it starts at 1: one vertex of slack per part). */
+int sgipc_partition_mesh(int n, int T, const uint32_t* h_tets, int bank, int* h_part, int* h_order, int* n_parts,
+                         int* offset);
+/* Sum of the weights of the vertex-graph edges cut by a partition. */
+int sgipc_partition_edge_cut(int n, int T, const uint32_t* h_tets, const int* h_part, long long* cut);
 /* Number of MAS levels for a node count; MASPreconditioner::computeNumLevels
  * (StiffGIPC/MASPreconditioner.cu:1724-1743). */
 int sgipc_mas_num_levels(int n_nodes, int bank);
@@ -335,6 +349,19 @@ int sgipc_autotune(sgipc_ctx* ctx, const double* d_b, int n_scenes, const int* h
 int sgipc_set_deterministic(sgipc_ctx* ctx, int on);
 /* Set the schedule directly (group 4, 8 or 16; 1..64 iterations per graph). */
 int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
+/* How the MAS build inverts its bank matrices (the reference:
+ * __inverse6_P96x96, StiffGIPC/MASPreconditioner.cu:630-726, scalar FP64
+ * Gauss-Jordan, stored FP32).  Applies to the next sgipc_precond_build.
+ *  VALU64: the scalar elimination on the vector ALUs, FP64;
+ *  MFMA64 (default): blocked Gauss-Jordan over 16x16 tiles on FP64 MFMA;
+ *  MFMA32_REFINED: blocked Gauss-Jordan on FP32 MFMA, then one refinement
+ *    step X + X(I - A X) with the residual accumulated in FP64 MFMA; FP32
+ *    inverses only, so it applies to SGIPC_PRECOND_MAS_FP32 builds (an
+ *    SGIPC_PRECOND_MAS build uses MFMA64). */
+#define SGIPC_BANK_INVERSE_VALU64 0
+#define SGIPC_BANK_INVERSE_MFMA64 1
+#define SGIPC_BANK_INVERSE_MFMA32_REFINED 2
+int sgipc_set_bank_inverse(sgipc_ctx* ctx, int mode);
 
 /* ---- FP64 energies (line search) ------------------------------------------ */
 /* Fixed-order FP64 reductions, result on the host (synchronises the stream);
@@ -390,8 +417,11 @@ int sgipc_mas_info(sgipc_ctx* ctx, int* levelnum, int* total_clusters, int* h_le
  * table 6*n (itable), fine masks n, prefix per level-0 bank M/bank */
 int sgipc_mas_tables(sgipc_ctx* ctx, int* h_coarse_space, int* h_going_next, int* h_coarse_table,
                      uint32_t* h_fine_mask, int* h_prefix0);
-/* dense FP64 inverse (symmetric, 3b x 3b row-major) of one bank, to host */
+/* dense FP64 inverse (symmetric, 3b x 3b row-major) of one bank, to host
+ * (SGIPC_ERR_STATE after a build that made FP32 inverses only) */
 int sgipc_mas_bank_inverse(sgipc_ctx* ctx, int bank_index, double* h_inv);
+/* the same from the FP32 inverses the FP32 MAS apply reads */
+int sgipc_mas_bank_inverse_f32(sgipc_ctx* ctx, int bank_index, float* h_inv);
 
 /* ---- device memory helpers for callers without a framework ---------------- */
 int sgipc_malloc(sgipc_ctx* ctx, void** d_ptr, size_t bytes);

@@ -10,6 +10,8 @@ namespace sgipc {
 int build_vertex_neighbors(int n, int T, const uint32_t* tets, uint32_t* start, uint32_t* num, uint32_t* list,
                            long long cap);
 int build_bank_map(int n, const int* part, int bank, int* part_map, long long cap, int* real_map);
+int partition_mesh(int n, int T, const uint32_t* tets, int bank, int* part, int* order, int* offset);
+long long partition_edge_cut(int n, int T, const uint32_t* tets, const int* part);
 int mas_num_levels(int nn, int bank);
 }  // namespace sgipc
 
@@ -93,6 +95,10 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
             if (v == 4 || v == 8 || v == 16) c->spmv_group = v;
         }
         if (const char* f = std::getenv("SGIPC_SPMV_FULL32")) c->spmv_full32 = std::atoi(f) != 0;
+        if (const char* f = std::getenv("SGIPC_BANK_INVERSE")) {
+            const int v = std::atoi(f);
+            if (v >= SGIPC_BANK_INVERSE_VALU64 && v <= SGIPC_BANK_INVERSE_MFMA32_REFINED) c->bank_inverse = v;
+        }
     });
     if (rc != SGIPC_OK) { delete c; return rc; }
     *out = c;
@@ -142,6 +148,25 @@ int sgipc_build_bank_map(int n, const int* part, int bank, int* part_map, int ca
 }
 
 int sgipc_mas_num_levels(int nn, int bank) { return sgipc::mas_num_levels(nn, bank); }
+
+int sgipc_partition_mesh(int n, int T, const uint32_t* tets, int bank, int* part, int* order, int* n_parts,
+                         int* offset) {
+    return guard(nullptr, [&] {
+        if (n < 0 || T < 0 || (T > 0 && !tets) || (n > 0 && !part) || !n_parts || (bank != 16 && bank != 32))
+            throw sgipc::arg_error("partition_mesh: invalid argument");
+        int off = 0;
+        *n_parts = sgipc::partition_mesh(n, T, tets, bank, part, order, &off);
+        if (offset) *offset = off;
+    });
+}
+
+int sgipc_partition_edge_cut(int n, int T, const uint32_t* tets, const int* part, long long* cut) {
+    return guard(nullptr, [&] {
+        if (n < 0 || T < 0 || (T > 0 && !tets) || (n > 0 && !part) || !cut)
+            throw sgipc::arg_error("partition_edge_cut: invalid argument");
+        *cut = sgipc::partition_edge_cut(n, T, tets, part);
+    });
+}
 
 int sgipc_set_fem_mesh(sgipc_ctx* c, int n, int T, const uint32_t* h_tets) {
     CHECK_CTX(c);
@@ -647,6 +672,7 @@ int sgipc_mas_bank_inverse(sgipc_ctx* c, int b, double* inv) {
     return guard(c, [&] {
         sgipc::MasState& m = c->mas;
         if (!m.ready) throw sgipc::state_error("mas_bank_inverse: MAS not built");
+        if (!m.inv64_ok) throw sgipc::state_error("mas_bank_inverse: only FP32 inverses were built");
         if (b < 0 || b >= m.nbanks || !inv) throw sgipc::arg_error("mas_bank_inverse: bad bank");
         set_device(c);
         SGIPC_HIP(hipStreamSynchronize(c->stream));
@@ -657,6 +683,32 @@ int sgipc_mas_bank_inverse(sgipc_ctx* c, int b, double* inv) {
         size_t e = 0;
         for (size_t r = 0; r < D; ++r)
             for (size_t q = r; q < D; ++q, ++e) inv[r * D + q] = inv[q * D + r] = up[e];
+    });
+}
+
+int sgipc_mas_bank_inverse_f32(sgipc_ctx* c, int b, float* inv) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        sgipc::MasState& m = c->mas;
+        if (!m.ready) throw sgipc::state_error("mas_bank_inverse_f32: MAS not built");
+        if (b < 0 || b >= m.nbanks || !inv) throw sgipc::arg_error("mas_bank_inverse_f32: bad bank");
+        set_device(c);
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+        const size_t D = 3 * (size_t)m.bank, NP = D * (D + 1) / 2;
+        std::vector<float> up(NP);
+        SGIPC_HIP(hipMemcpy(up.data(), m.inv32.p + (size_t)b * NP, sizeof(float) * NP, hipMemcpyDeviceToHost));
+        size_t e = 0;
+        for (size_t r = 0; r < D; ++r)
+            for (size_t q = r; q < D; ++q, ++e) inv[r * D + q] = inv[q * D + r] = up[e];
+    });
+}
+
+int sgipc_set_bank_inverse(sgipc_ctx* c, int mode) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (mode < SGIPC_BANK_INVERSE_VALU64 || mode > SGIPC_BANK_INVERSE_MFMA32_REFINED)
+            throw sgipc::arg_error("set_bank_inverse: unknown mode");
+        c->bank_inverse = mode;
     });
 }
 

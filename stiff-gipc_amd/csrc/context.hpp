@@ -74,6 +74,7 @@ struct MasState {
     DBuf<double> bank_upper; // nbanks * npack * 9 (Galerkin, packed upper, col-major blocks)
     DBuf<double> inv64;      // nbanks * D(D+1)/2 symmetric inverse, upper triangle packed by rows (FP64)
     DBuf<float> inv32;       // same, FP32
+    bool inv64_ok = true;    // inv64 holds the FP64 inverses (not after an FP32-only build)
     DBuf<int> l1lo, l1hi;    // per level-1 bank: range [lo, hi) of level-0 banks holding its children
     DBuf<int> own_lo, own_hi;  // per level-1 bank: owned vertex range (slot-ordered vertices)
     DBuf<double> proj_scratch; // per-workgroup partials of the top-level bank matrices
@@ -219,6 +220,7 @@ struct Ctx {
 
     // preconditioner
     int precond_kind = 0;
+    int bank_inverse = 0;   // MAS bank inverses: BANK_INV_* (default: scalar FP64 Gauss-Jordan)
     int sub_offset = 0, sub_nb = 0;
     MasState mas;
 
@@ -283,6 +285,12 @@ void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, co
                       const unsigned* nlist, int nlist_len, const int* part_map, int M,
                       const int* real_map, int bank);
 void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs);
+// MAS bank inverses: scalar FP64 Gauss-Jordan on the vector ALUs, blocked
+// FP64 Gauss-Jordan on MFMA, or blocked FP32 on MFMA + one refinement step
+// with the residual accumulated in FP64 (FP32 inverses only)
+enum { BANK_INV_VALU64 = 0, BANK_INV_MFMA64 = 1, BANK_INV_MFMA32_REFINED = 2 };
+void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, double* inv64, float* inv32,
+                       int mode);
 void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
 // fused multilevel restriction of r[sub3 .. sub3 + 3n) into R_l (FP64 or FP32

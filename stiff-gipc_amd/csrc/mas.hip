@@ -1375,13 +1375,21 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
                            L, m.node_of.p, m.bank_upper.p, lds_lo, lds_nb);
         SGIPC_LAUNCH_CHECK();
     }
-    if (B == 16)
-        hipLaunchKernelGGL(k_bank_inverse_reg48, dim3(m.nbanks), dim3(64), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,
-                           m.inv32.p);
-    else
-        hipLaunchKernelGGL(k_bank_inverse<32>, dim3(m.nbanks), dim3(128), 0, s, m.nbanks, m.bank_upper.p, m.inv64.p,
-                           m.inv32.p);
-    SGIPC_LAUNCH_CHECK();
+    // the FP32-with-refinement inverses serve FP32 applies only
+    int inv_mode = c.bank_inverse;
+    if (inv_mode == BANK_INV_MFMA32_REFINED && c.precond_kind != 3) inv_mode = BANK_INV_MFMA64;
+    m.inv64_ok = inv_mode != BANK_INV_MFMA32_REFINED;
+    if (inv_mode != BANK_INV_VALU64) {
+        bank_inverse_mfma(c, B, m.nbanks, m.bank_upper.p, m.inv64.p, m.inv32.p, inv_mode);
+    } else {
+        if (B == 16)
+            hipLaunchKernelGGL(k_bank_inverse_reg48, dim3(m.nbanks), dim3(64), 0, s, m.nbanks, m.bank_upper.p,
+                               m.inv64.p, m.inv32.p);
+        else
+            hipLaunchKernelGGL(k_bank_inverse<32>, dim3(m.nbanks), dim3(128), 0, s, m.nbanks, m.bank_upper.p,
+                               m.inv64.p, m.inv32.p);
+        SGIPC_LAUNCH_CHECK();
+    }
     // apply work vectors: allocated here so their addresses are stable for
     // captured PCG graphs; padding entries stay zero (never written)
     m.R64.reserve(3 * (size_t)m.total + 3);
@@ -1395,6 +1403,7 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
 
 void mas_apply64(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
+    if (!c.mas.inv64_ok) throw state_error("MAS apply: only FP32 inverses were built");
     mas_apply_t<double>(c, r, z, c.mas.R64.p, c.mas.Z64.p, c.mas.inv64.p);
 }
 void mas_apply32(Ctx& c, const double* r, double* z) {
@@ -1464,7 +1473,10 @@ void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const floa
 
 void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl) {
     if (fp32) mas_solve_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
-    else mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+    else {
+        if (!c.mas.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
+        mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+    }
 }
 
 }  // namespace sgipc

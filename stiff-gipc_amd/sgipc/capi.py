@@ -14,6 +14,7 @@ ERR = {1: "invalid argument", 2: "HIP error", 3: "capacity exceeded", 4: "invali
 PRECOND_NONE, PRECOND_DIAG, PRECOND_MAS, PRECOND_MAS_FP32 = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 STOP_REF_RZ, STOP_TRUE_REL = 0, 1
+BANK_INVERSE_VALU64, BANK_INVERSE_MFMA64, BANK_INVERSE_MFMA32_REFINED = 0, 1, 2
 PROF_SPMV, PROF_MAS_SOLVE, PROF_FEM, PROF_CONVERT, PROF_PRECOND_BUILD, PROF_SOLVE, PROF_BARRIER = range(7)
 
 
@@ -53,6 +54,8 @@ SIGNATURES = {
     "sgipc_build_vertex_neighbors": ([I, I, P, P, P, P, I, C.POINTER(I)], I),
     "sgipc_build_bank_map": ([I, P, I, P, I, C.POINTER(I), P], I),
     "sgipc_mas_num_levels": ([I, I], I),
+    "sgipc_partition_mesh": ([I, I, P, I, P, P, C.POINTER(I), C.POINTER(I)], I),
+    "sgipc_partition_edge_cut": ([I, I, P, P, C.POINTER(LL)], I),
     "sgipc_set_fem_mesh": ([P, I, I, P], I),
     "sgipc_set_mas_topology": ([P, I, I, P, P, P, I, P, I, P, I], I),
     "sgipc_triplets_view": ([P, P, P, P, LL], I),
@@ -94,6 +97,8 @@ SIGNATURES = {
     "sgipc_mas_info": ([P, C.POINTER(I), C.POINTER(I), P], I),
     "sgipc_mas_tables": ([P, P, P, P, P, P], I),
     "sgipc_mas_bank_inverse": ([P, I, P], I),
+    "sgipc_mas_bank_inverse_f32": ([P, I, P], I),
+    "sgipc_set_bank_inverse": ([P, I], I),
     "sgipc_malloc": ([P, C.POINTER(P), SZ], I),
     "sgipc_free": ([P, P], I),
     "sgipc_memcpy_h2d": ([P, P, P, SZ], I),

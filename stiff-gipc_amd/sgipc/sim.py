@@ -255,14 +255,45 @@ def final_positions(sim, log_E, v0, steps):
     return xT.reshape(-1, 3)
 
 
-def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, progress=None, richardson_above=None):
+def ridders(D, h0, con=2.0, levels=6, safe=2.0):
+    """Derivative from central differences D(h) by Ridders' extrapolation
+    (C. J. F. Ridders, Adv. Eng. Software 4(2), 1982): D(h) is taken at
+    h0, h0/con, h0/con^2, ... and extrapolated to h -> 0 in a Neville
+    tableau, each column cancelling the next even power of h.  Every entry
+    carries an error estimate (its distance to the two entries it came
+    from); the entry with the smallest estimate is returned, and the tableau
+    stops growing once its diagonal moves by more than `safe` times that
+    estimate (round-off has taken over).  Returns (estimate, error estimate,
+    smallest h used)."""
+    c2 = con * con
+    prev = [D(h0)]
+    best, err, h = prev[0], float("inf"), h0
+    for i in range(1, levels):
+        h /= con
+        row = [D(h)]
+        fac = c2
+        for j in range(1, i + 1):
+            row.append((row[j - 1] * fac - prev[j - 1]) / (fac - 1.0))
+            fac *= c2
+            e = max(abs(row[j] - row[j - 1]), abs(row[j] - prev[j - 1]))
+            if e <= err:
+                best, err = row[j], e
+        if abs(row[i] - prev[i - 1]) >= safe * err:
+            break
+        prev = row
+    return best, err, h
+
+
+def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, progress=None, ridders_above=None,
+             ridders_h0=1e-4):
     """Central differences of L along n_dir seeded unit directions in
     (ln E, free v0) space against the adjoint gradient.  Returns a list of
-    (fd, adjoint, relative error) and the loss.  With richardson_above set,
-    directions whose error exceeds it are also differenced with h/2 and the
-    Richardson value (4 fd(h/2) - fd(h)) / 3, which cancels the O(h^2)
-    truncation term (a fourth-order stencil), is compared too: entries then
-    carry (fd_r, rel_r).  richardson_above=0 does so for every direction."""
+    (fd, adjoint, relative error) and the loss.  With ridders_above set,
+    directions whose error exceeds it are also differentiated by Ridders'
+    extrapolation of central differences from ridders_h0 down (the step
+    chosen per direction by the tableau's error estimate): entries then carry
+    (fd_r, rel_r, err_est_r, h_min_r).  ridders_above=0 does so for every
+    direction."""
     L, gE, gv = loss_and_gradient(sim, log_E, v0, target, steps, ckpt)
     rng = np.random.default_rng(seed)
     free = sim.free.reshape(-1, 3)
@@ -283,9 +314,9 @@ def fd_check(sim, log_E, v0, target, steps, ckpt=5, n_dir=32, h=1e-6, seed=0, pr
         fd = central(h)
         ad = float((gE * dE).sum() + (gv * dv).sum())
         rel = abs(fd - ad) / max(abs(ad), 1e-300)
-        if richardson_above is not None and (richardson_above <= 0 or rel > richardson_above):
-            fr = (4.0 * central(0.5 * h) - fd) / 3.0
-            out.append((fd, ad, rel, fr, abs(fr - ad) / max(abs(ad), 1e-300)))
+        if ridders_above is not None and (ridders_above <= 0 or rel > ridders_above):
+            fr, er, hmin = ridders(central, ridders_h0)
+            out.append((fd, ad, rel, fr, abs(fr - ad) / max(abs(ad), 1e-300), er, hmin))
         else:
             out.append((fd, ad, rel))
         if progress:

@@ -244,3 +244,9 @@ class NewtonSystem:
         out = np.zeros(D * D)
         self.ctx.call("sgipc_mas_bank_inverse", b, out.ctypes.data)
         return out.reshape(D, D)
+
+    def bank_inverse_f32(self, b):
+        D = 3 * self.sc.bank
+        out = np.zeros(D * D, np.float32)
+        self.ctx.call("sgipc_mas_bank_inverse_f32", b, out.ctypes.data)
+        return out.reshape(D, D)

@@ -126,7 +126,7 @@ def test_batched_256_scenes_each_converged(torch_cuda):
     print("  worst scene residual %.3e, iterations %d-%d" % (worst, min(its), max(its)), flush=True)
 
 
-def test_cantilever_60x12x12_adjoint_vs_fourth_order_fd(torch_cuda):
+def test_cantilever_60x12x12_adjoint_vs_extrapolated_fd(torch_cuda):
     from sgipc import scene
     from sgipc.sim import GroundSim, fd_check
     torch = torch_cuda
@@ -140,10 +140,12 @@ def test_cantilever_60x12x12_adjoint_vs_fourth_order_fd(torch_cuda):
     print("case: cantilever 60x12x12 n=%d T=%d, 20 steps, 4 directions" % (sc.n, sc.T), flush=True)
 
     def progress(i, fd, ad, rel):
-        print("  direction %d: fd %.9e adjoint %.9e rel %.2e" % (i, fd, ad, rel), flush=True)
+        print("  direction %d: fd(h=1e-6) %.9e adjoint %.9e rel %.2e" % (i, fd, ad, rel), flush=True)
+    # every direction also by Ridders' extrapolation of central differences
     res, L = fd_check(sim, logE, v0, target, 20, ckpt=5, n_dir=4, h=1e-6, seed=0, progress=progress,
-                      richardson_above=0)
+                      ridders_above=0)
     assert sim.n_close > 0                      # the bar reaches the ground barrier
-    for fd, ad, rel, fr, rr in res:
-        print("  fourth-order fd %.9e rel %.2e" % (fr, rr), flush=True)
+    for fd, ad, rel, fr, rr, er, hmin in res:
+        print("  extrapolated fd %.9e rel %.2e (error estimate %.1e, smallest h %.1e)" % (fr, rr, er, hmin),
+              flush=True)
         assert rr <= 1e-4
