@@ -375,6 +375,18 @@ int sgipc_kinetic_energy(sgipc_ctx* ctx, int n, const double* d_x, const double*
  * (StiffGIPC/femEnergy.cu:1032-1050) */
 int sgipc_fem_energy(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d_tets, const double* d_DmInv,
                      const double* d_vol, const double* d_mu, const double* d_lambda, double dt, double* h_energy);
+/* The same SNK1 energy without allocating: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight line-search loops, harnesses that forbid hipMalloc).  The result goes
+ * to the caller's device double d_energy on the context's stream (no host
+ * synchronisation); the per-workgroup partials live in a static device array
+ * of 1024 doubles per GPU (the reduction grid is capped at 1024 workgroups,
+ * so any T fits).  Same grid and summation order as sgipc_fem_energy, so the
+ * same bits.  Calls that overlap on one GPU (any contexts, any streams) share
+ * that array and race. */
+int sgipc_fem_energy_noalloc(sgipc_ctx* ctx, int T, const double* d_x, const uint32_t* d_tets, const double* d_DmInv,
+                             const double* d_vol, const double* d_mu, const double* d_lambda, double dt,
+                             double* d_energy);
 /* sum over int4 pairs of the RANK-2 barrier (mollified for parallel edges);
  * _getBarrierEnergy_Reduction_3D (StiffGIPC/GIPC.cu:7768-7800) with
  * __cal_Barrier_energy (:701-957).  dhat squared, as GIPC::dHat. */

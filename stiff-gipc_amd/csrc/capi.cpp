@@ -76,6 +76,8 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         if (device < 0 || device >= ndev) throw sgipc::arg_error("invalid device index");
         c->device = device;
         SGIPC_HIP(hipSetDevice(device));
+        // this device's address of the allocation-free energy's scratch
+        sgipc::energy_scratch_setup(device);
         // blocking stream: ordered with work on the legacy null stream, which
         // callers (and torch's default stream) may use on the same buffers
         SGIPC_HIP(hipStreamCreate(&c->own_stream));
@@ -606,6 +608,17 @@ int sgipc_fem_energy(sgipc_ctx* c, int T, const double* x, const uint32_t* tets,
             throw sgipc::arg_error("fem_energy: invalid argument");
         set_device(c);
         *e = sgipc::fem_energy(*c, T, x, tets, DmInv, vol, mu, lam, dt);
+    });
+}
+
+int sgipc_fem_energy_noalloc(sgipc_ctx* c, int T, const double* x, const uint32_t* tets, const double* DmInv,
+                             const double* vol, const double* mu, const double* lam, double dt, double* d_e) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (T < 0 || !d_e || (T > 0 && (!x || !tets || !DmInv || !vol || !mu || !lam)))
+            throw sgipc::arg_error("fem_energy_noalloc: invalid argument");
+        set_device(c);
+        sgipc::fem_energy_noalloc(*c, T, x, tets, DmInv, vol, mu, lam, dt, d_e);
     });
 }
 

@@ -276,6 +276,11 @@ double fem_energy(Ctx& c, int T, const double* x, const uint32_t* tets, const do
                   const double* mu, const double* lam, double dt);
 double barrier_energy(Ctx& c, int np, const int* pairs, const double* x, const double* xr, double dhat,
                       double kappa);
+// allocation-free SNK1 energy: partials in a static device array (looked up
+// once per device by energy_scratch_setup), result to device memory
+void energy_scratch_setup(int device);
+void fem_energy_noalloc(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                        const double* mu, const double* lam, double dt, double* out);
 // energy reductions: workgroups for n items (256 threads), partial buffer,
 // fixed-order sum of g partials to the host
 int energy_grid(long long n);

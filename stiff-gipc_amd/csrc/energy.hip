@@ -13,6 +13,7 @@
 // (:701-957, RANK 2).
 #include "context.hpp"
 #include "pcg_dev.hpp"
+#include <mutex>
 
 namespace sgipc {
 namespace {
@@ -76,7 +77,55 @@ __global__ void __launch_bounds__(kEThreads) k_energy_total(const double* __rest
     if (threadIdx.x == 0) *out = s;
 }
 
+// fixed-order sum of the partials, times `scale`, to device memory
+__global__ void __launch_bounds__(kEThreads) k_energy_total_scaled(const double* __restrict__ part, int np,
+                                                                   double scale, double* __restrict__ out) {
+    __shared__ double sh[kEThreads / 64];
+    double s = sum_partials(part, np, sh);
+    if (threadIdx.x == 0) *out = scale * s;
+}
+
+// Scratch of the allocation-free FP64 SNK1 energy: one partial per
+// workgroup (at most kEParts).  Each device holds its own copy at its own
+// address, looked up once per device by energy_scratch_setup.
+static __device__ __align__(256) double scratch[kEParts];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
+
+// Device address of this device's energy scratch (device current).
+void energy_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) throw arg_error("energy scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    SGIPC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw hip_error("energy scratch: no device address");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+// dt^2 sum_t SNK1 energy to device memory, partials in the static scratch:
+// no allocation, no synchronisation.  Same grid and reduction order as
+// fem_energy, so the same bits.
+void fem_energy_noalloc(Ctx& c, int T, const double* x, const uint32_t* tets, const double* DmInv, const double* vol,
+                        const double* mu, const double* lam, double dt, double* out) {
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c.device >= 0 && c.device < kMaxDevices) part = g_scratch_ptr[c.device];
+    }
+    if (!part) throw state_error("fem_energy_noalloc: scratch not set up on this device");
+    const int g = energy_grid(T);
+    if (g > kEParts) throw capacity_error("fem_energy_noalloc: more partials than the scratch holds");
+    hipLaunchKernelGGL(k_fem_energy, dim3(g), dim3(kEThreads), 0, c.stream, T, x, (const uint4*)tets, DmInv, vol, mu,
+                       lam, part);
+    SGIPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_energy_total_scaled, dim3(1), dim3(kEThreads), 0, c.stream, (const double*)part, g, dt * dt,
+                       out);
+    SGIPC_LAUNCH_CHECK();
+}
 
 int energy_grid(long long n) {
     long long g = div_up(n, kEThreads);

@@ -91,6 +91,7 @@ SIGNATURES = {
     "sgipc_solve": ([P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)], I),
     "sgipc_kinetic_energy": ([P, I, P, P, P, C.POINTER(D)], I),
     "sgipc_fem_energy": ([P, I, P, P, P, P, P, P, D, C.POINTER(D)], I),
+    "sgipc_fem_energy_noalloc": ([P, I, P, P, P, P, P, P, D, P], I),
     "sgipc_barrier_energy": ([P, I, P, P, P, D, D, C.POINTER(D)], I),
     "sgipc_profile_enable": ([P, I], I),
     "sgipc_profile_read": ([P, I, C.POINTER(LL), C.POINTER(D), I], I),
