@@ -22,6 +22,7 @@
 //    refinement step X1 = X0 + X0 (I - A X0) with the residual I - A X0
 //    accumulated in FP64 MFMA from the FP64 bank matrix; X1 is stored FP32.
 #include "context.hpp"
+#include <utility>
 
 namespace sgipc {
 namespace {
@@ -211,6 +212,145 @@ __global__ void __launch_bounds__(64) k_bank_inverse_mfma(int nbanks, const doub
     }
 }
 
+// Value of v in lane LANE (a compile-time constant) of the wavefront,
+// through scalar registers.
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), lane);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+    }
+}
+
+// Step J of the scalar Gauss-Jordan of a 16 x 16 tile held column per lane
+// (lane i < 16 owns column i in S[0..15]): the pivot column comes from lane
+// J through scalar registers, read entry by entry before lane J overwrites
+// it; the owner starts from the unit vector e_J (its base is multiplied by
+// 0), so every lane runs the same arithmetic:
+//   a_JJ <- 1 / a_JJ, a_Ji <- a_Ji / a_JJ, a_KJ <- -a_KJ / a_JJ,
+//   a_Ki <- a_Ki - a_KJ a_Ji / a_JJ.
+template <typename T, int J, int... K>
+__device__ __forceinline__ void gj16_step(T (&S)[16], int i, std::integer_sequence<int, K...>) {
+    const bool own = i == J;
+    const T keep = own ? T(0) : T(1);
+    const T rt = read_lane(S[J], J);
+    const T pv = (own ? T(1) : S[J]) / rt;
+    ((K != J ? (void)(S[K] = S[K] * keep + -read_lane(S[K], J) * pv) : (void)0), ...);
+    S[J] = pv;
+}
+template <typename T, int... J>
+__device__ __forceinline__ void gj16(T (&S)[16], int i, std::integer_sequence<int, J...>) {
+    (gj16_step<T, J>(S, i, std::make_integer_sequence<int, 16>{}), ...);
+}
+
+// The 48 x 48 (16-vertex bank) elimination with the nine tiles held in
+// registers in accumulator layout; LDS only stages the tiles that serve as
+// MFMA operands (the pivot inverse, the row tiles A_kt, the column tiles
+// A_tk and the new row tiles), 10.9 KB per wavefront for FP64.  The pivot
+// tile is inverted column per lane (16 lanes) by scalar Gauss-Jordan with the
+// pivot column broadcast through scalar registers.
+template <typename T>
+__global__ void __launch_bounds__(64) k_bank_inverse_mfma48(int nbanks, const double* __restrict__ bank_upper,
+                                                            double* __restrict__ inv64, float* __restrict__ inv32) {
+    typedef typename Mf<T>::acc acc;
+    constexpr int D = 48, B = 16, npack = B * (B + 1) / 2, NP = D * (D + 1) / 2, LDW = 17;
+    __shared__ T sw[5][16 * LDW];  // pivot inverse, row tiles (2), column tiles (2)
+    const Tiles<T, LDW> W{sw[0]}, R0{sw[1]}, R1{sw[2]}, C0{sw[3]}, C1{sw[4]};
+    const int b = blockIdx.x, l = threadIdx.x, q = l >> 4, j = l & 15;
+    const double* src = bank_upper + (size_t)b * npack * 9;
+    acc A[3][3];
+#pragma unroll
+    for (int ti = 0; ti < 3; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 3; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) A[ti][tj][r] = (T)bank_elem<D>(src, 16 * ti + Mf<T>::row(q, r), 16 * tj + j);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int t0 = (k + 1) % 3, t1 = (k + 2) % 3;
+        // pivot tile: accumulator layout -> LDS -> column per lane -> inverse
+        W.store_c(0, 0, l, A[k][k]);
+        __syncthreads();
+        T col[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) col[i] = l < 16 ? W.at(i, l) : T(0);
+        __syncthreads();
+        gj16<T>(col, l, std::make_integer_sequence<int, 16>{});
+        if (l < 16) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) W.at(i, l) = col[i];
+        }
+        R0.store_c(0, 0, l, A[k][t0]);
+        R1.store_c(0, 0, l, A[k][t1]);
+        C0.store_c(0, 0, l, A[t0][k]);
+        C1.store_c(0, 0, l, A[t1][k]);
+        __syncthreads();
+        T pa[4], pb[4], rb0[4], rb1[4], ca0[4], ca1[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            pa[c] = W.opa(0, 0, c, l);
+            pb[c] = W.opb(0, 0, c, l);
+            rb0[c] = R0.opb(0, 0, c, l);
+            rb1[c] = R1.opb(0, 0, c, l);
+            ca0[c] = C0.opa(0, 0, c, l);
+            ca1[c] = C1.opa(0, 0, c, l);
+        }
+        const acc P = W.load_c(0, 0, l);
+        acc Tr0 = {0, 0, 0, 0}, Tr1 = {0, 0, 0, 0}, Tc0 = {0, 0, 0, 0}, Tc1 = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            Tr0 = Mf<T>::mac(pa[c], rb0[c], Tr0);  // P^-1 A_k,t0
+            Tr1 = Mf<T>::mac(pa[c], rb1[c], Tr1);
+            Tc0 = Mf<T>::mac(ca0[c], pb[c], Tc0);  // A_t0,k P^-1 (old column tile)
+            Tc1 = Mf<T>::mac(ca1[c], pb[c], Tc1);
+        }
+        __syncthreads();
+        R0.store_c(0, 0, l, Tr0);
+        R1.store_c(0, 0, l, Tr1);
+        __syncthreads();
+        T trb0[4], trb1[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            trb0[c] = R0.opb(0, 0, c, l);
+            trb1[c] = R1.opb(0, 0, c, l);
+        }
+        // trailing update with the old column tiles and the new row tiles
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            A[t0][t0] = Mf<T>::mac(-ca0[c], trb0[c], A[t0][t0]);
+            A[t0][t1] = Mf<T>::mac(-ca0[c], trb1[c], A[t0][t1]);
+            A[t1][t0] = Mf<T>::mac(-ca1[c], trb0[c], A[t1][t0]);
+            A[t1][t1] = Mf<T>::mac(-ca1[c], trb1[c], A[t1][t1]);
+        }
+        A[k][t0] = Tr0;
+        A[k][t1] = Tr1;
+        A[t0][k] = -Tc0;
+        A[t1][k] = -Tc1;
+        A[k][k] = P;
+        __syncthreads();
+    }
+    // upper triangle (row <= column) packed by rows
+    double* o64 = inv64 + (size_t)b * NP;
+    float* o32 = inv32 + (size_t)b * NP;
+#pragma unroll
+    for (int ti = 0; ti < 3; ++ti)
+#pragma unroll
+        for (int tj = ti; tj < 3; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + Mf<T>::row(q, r), c = 16 * tj + j;
+                if (row <= c) {
+                    const int e = row * (2 * D - row + 1) / 2 + (c - row);
+                    if constexpr (sizeof(T) == 8) o64[e] = A[ti][tj][r];
+                    o32[e] = (float)A[ti][tj][r];
+                }
+            }
+}
+
 // One refinement step of the FP32 inverse X0 (inv32, packed upper):
 // R = I - A X0 in FP64 MFMA (A the FP64 bank matrix, X0 exact in FP64),
 // X1 = X0 + X0 R (upper tiles, FP64 MFMA), stored back to inv32.
@@ -291,7 +431,7 @@ void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, d
     if (bank != 16 && bank != 32) throw arg_error("bank inverse: bank size must be 16 or 32");
     if (mode == BANK_INV_MFMA64) {
         if (bank == 16)
-            hipLaunchKernelGGL((k_bank_inverse_mfma<double, 3>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
+            hipLaunchKernelGGL((k_bank_inverse_mfma48<double>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
                                inv64, inv32);
         else
             hipLaunchKernelGGL((k_bank_inverse_mfma<double, 6>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
@@ -299,7 +439,7 @@ void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, d
         SGIPC_LAUNCH_CHECK();
     } else if (mode == BANK_INV_MFMA32_REFINED) {
         if (bank == 16) {
-            hipLaunchKernelGGL((k_bank_inverse_mfma<float, 3>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
+            hipLaunchKernelGGL((k_bank_inverse_mfma48<float>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
                                inv64, inv32);
             SGIPC_LAUNCH_CHECK();
             hipLaunchKernelGGL((k_bank_refine<3>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper, inv32);

@@ -220,7 +220,7 @@ struct Ctx {
 
     // preconditioner
     int precond_kind = 0;
-    int bank_inverse = 0;   // MAS bank inverses: BANK_INV_* (default: scalar FP64 Gauss-Jordan)
+    int bank_inverse = 1;   // MAS bank inverses: BANK_INV_* (default: FP64 MFMA)
     int sub_offset = 0, sub_nb = 0;
     MasState mas;
 
