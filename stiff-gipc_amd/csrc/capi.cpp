@@ -97,6 +97,7 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
             if (v == 4 || v == 8 || v == 16) c->spmv_group = v;
         }
         if (const char* f = std::getenv("SGIPC_SPMV_FULL32")) c->spmv_full32 = std::atoi(f) != 0;
+        if (const char* f = std::getenv("SGIPC_MAS_FUSED")) c->mas_fused = std::atoi(f) != 0;
         if (const char* f = std::getenv("SGIPC_BANK_INVERSE")) {
             const int v = std::atoi(f);
             if (v >= SGIPC_BANK_INVERSE_VALU64 && v <= SGIPC_BANK_INVERSE_MFMA32_REFINED) c->bank_inverse = v;

@@ -221,6 +221,7 @@ struct Ctx {
     // preconditioner
     int precond_kind = 0;
     int bank_inverse = 1;   // MAS bank inverses: BANK_INV_* (default: FP64 MFMA)
+    int mas_fused = 1;      // top restriction beside the bank solves (16-vertex banks; env SGIPC_MAS_FUSED=0: own launch)
     int sub_offset = 0, sub_nb = 0;
     MasState mas;
 
@@ -300,16 +301,24 @@ void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
 // fused multilevel restriction of r[sub3 .. sub3 + 3n) into R_l (FP64 or FP32
 // preconditioner data); bank solves of all levels; both skip when ctrl[0] != 0
-void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl);
-void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl);
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top = true);
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top = true);
 void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl);
+// PCG iteration with 16-vertex banks: the top-level restriction (levels
+// 2 -> L-1) runs in the bank-solve launch, beside the banks of levels 0..2,
+// instead of its own launch before it (the x/r update then restricts levels
+// 0..2 only: mas_update_restrict(..., top = false))
+bool mas_fused(const Ctx& c);
+void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl);
 // PCG x/r update fused with the restriction of the new residual (levels
 // 0..L-1); g1 + gx workgroups, each writing one r.r partial
 void mas_update_layout(const Ctx& c, int& g1, int& gx);
 void mas_update_restrict(Ctx& c, double* x, double* r, const double* p, const double* Ap, const double* part_pap,
-                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32);
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32,
+                         bool top = true);
 void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const float* Ap, const double* part_pap,
-                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32);
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32,
+                         bool top = true);
 void diag_build(Ctx& c);
 
 void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, const int counts[3], const double* x,

@@ -1086,6 +1086,113 @@ __global__ void __launch_bounds__(kTop) k_restrict_top(int B, int L, const int* 
     }
 }
 
+// ---- top-level restriction overlapped with the bank solves (16-vertex banks) -
+// k_mas_solve_top replaces k_restrict_top + k_bank_solve in a PCG iteration
+// (the x/r update before it restricts levels 0..2; k_collect_dot follows):
+// workgroup 0 restricts levels 2 -> L-1 (the sums of k_restrict_top) and
+// then solves the banks of levels >= 3 on its own results; the other
+// workgroups solve every bank of levels 0..2, whose restriction is complete
+// at launch, at the same time.  No workgroup waits for another.
+constexpr int kCW = 8;               // wavefronts per workgroup of k_mas_coarse
+constexpr int kCT = 64 * kCW;
+
+// One 48 x 48 bank solve by one wavefront: the packed FP32/FP64 inverse is
+// staged into this wavefront's LDS area by direct global -> LDS loads, lane
+// i < 48 forms row i (the arithmetic of k_bank_solve<16>).
+template <typename T>
+__device__ inline T wave_bank_solve(const T* __restrict__ inv, const T* __restrict__ R, int b, void* lds) {
+    constexpr int D = 48, NP = D * (D + 1) / 2;
+    using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    constexpr int NV = NP * (int)sizeof(T) / 16;
+    constexpr int PER = (NV + 63) / 64;
+    V4* As = static_cast<V4*>(lds);
+    const int i = threadIdx.x & 63;
+    const V4* src = reinterpret_cast<const V4*>(inv + (size_t)b * NP);
+    const T rv = R[(size_t)b * D + min(i, D - 1)];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q0 = k * 64;
+        __builtin_amdgcn_global_load_lds((gptr_t)(src + min(q0 + i, NV - 1)), (lptr_t)(As + q0), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    const T* A = reinterpret_cast<const T*>(As);
+    T acc = 0;
+    if (i < D) {
+        int idx = i;
+#pragma unroll 8
+        for (int j = 0; j < D; ++j) {
+            acc += A[idx] * lane_bcast(rv, j);
+            idx += (j < i) ? (D - 1 - j) : 1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();  // every lane's reads of this bank done before the area is reused
+    return acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCT) k_mas_solve_top(int L, const int* __restrict__ ls,
+                                                     const int* __restrict__ going_next,
+                                                     const unsigned* __restrict__ comp_mask, int top_bank0,
+                                                     int nbanks, const T* __restrict__ inv, T* __restrict__ R,
+                                                     T* __restrict__ Z, const int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
+    constexpr int D = 48;
+    using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    constexpr int PER = ((D * (D + 1) / 2) * (int)sizeof(T) / 16 + 63) / 64;
+    static_assert(kCW * PER * 64 * sizeof(V4) >= 6 * kCT * sizeof(T), "LDS union too small");
+    __shared__ V4 lds[kCW][PER * 64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (blockIdx.x == 0) {
+        // levels 2 -> L-1, one level at a time (the sums of k_restrict_top)
+        T(*sv)[kCT] = reinterpret_cast<T(*)[kCT]>(&lds[0][0]);
+        T(*nx)[kCT] = reinterpret_cast<T(*)[kCT]>(reinterpret_cast<char*>(&lds[0][0]) + 3 * kCT * sizeof(T));
+        const int t = threadIdx.x;
+        bool in_lds = false;
+        for (int l = 2; l + 1 < L; ++l) {
+            const int cnt = ls[2 * l], beg = ls[2 * l + 1], cntn = ls[2 * l + 2], begn = ls[2 * l + 3];
+            const bool next_fits = cntn <= kCT;
+            for (int c0 = 0; c0 < cnt; c0 += kCT) {
+                const int cc = c0 + t;
+                const bool valid = cc < cnt;
+                const size_t g = (size_t)beg + cc;
+                const int pp = valid ? going_next[g] : -1;
+                const unsigned mask = valid ? comp_mask[g] : 0u;
+                T v0 = 0, v1 = 0, v2 = 0;
+                if (valid) {
+                    if (in_lds) { v0 = nx[0][cc]; v1 = nx[1][cc]; v2 = nx[2][cc]; }
+                    else { v0 = R[3 * g]; v1 = R[3 * g + 1]; v2 = R[3 * g + 2]; }
+                }
+                __syncthreads();
+                sv[0][t] = v0; sv[1][t] = v1; sv[2][t] = v2;
+                __syncthreads();
+                T s0, s1, s2;
+                if (valid && component_sum<T>(mask, sv[0], sv[1], sv[2], 16, s0, s1, s2)) {
+                    const size_t q = (size_t)pp;
+                    R[3 * q] = s0; R[3 * q + 1] = s1; R[3 * q + 2] = s2;
+                    if (next_fits) { const int j = (int)(q - begn); nx[0][j] = s0; nx[1][j] = s1; nx[2][j] = s2; }
+                }
+                __syncthreads();
+            }
+            in_lds = next_fits;
+        }
+        // the top banks, on this workgroup's restriction (R written above;
+        // the barrier orders this workgroup's global writes and reads)
+        __threadfence_block();
+        __syncthreads();
+        for (int b = top_bank0 + wv; b < nbanks; b += kCW) {
+            const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
+            if (lane < D) Z[(size_t)b * D + lane] = zb;
+        }
+        return;
+    }
+    const int b = (blockIdx.x - 1) * kCW + wv;  // levels 0 .. 2
+    if (b < top_bank0) {
+        const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
+        if (lane < D) Z[(size_t)b * D + lane] = zb;
+    }
+}
+
 // level-0 bank range of each level-1 bank (children of its nodes)
 __global__ void k_l1_ranges(int nb0, int B, const int* __restrict__ prefix0, const int* __restrict__ psum0,
                             int* __restrict__ lo, int* __restrict__ hi) {
@@ -1111,7 +1218,7 @@ int num_levels(int nn, int B) {
 }
 
 template <typename V, typename T>
-void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl) {
+void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl, bool top = true) {
     MasState& m = c.mas;
     if (m.nb1 <= 0) return;
     hipLaunchKernelGGL((k_restrict_all<V, T>), dim3(m.nb1), dim3(kTB), 0, c.stream, r, sub3, m.bank, m.L,
@@ -1119,7 +1226,7 @@ void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl) {
                        (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p,
                        (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
     SGIPC_LAUNCH_CHECK();
-    if (m.L >= 4) {
+    if (top && m.L >= 4) {
         hipLaunchKernelGGL((k_restrict_top<T>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
                            (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p, R, ctrl);
         SGIPC_LAUNCH_CHECK();
@@ -1410,14 +1517,16 @@ void mas_apply32(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
     mas_apply_t<float>(c, r, z, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p);
 }
-void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl) {
-    if (fp32) mas_restrict_t<double, float>(c, r, sub3, c.mas.R32.p, ctrl);
-    else mas_restrict_t<double, double>(c, r, sub3, c.mas.R64.p, ctrl);
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top) {
+    if (fp32) mas_restrict_t<double, float>(c, r, sub3, c.mas.R32.p, ctrl, top);
+    else mas_restrict_t<double, double>(c, r, sub3, c.mas.R64.p, ctrl, top);
 }
-void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl) {
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top) {
     if (!fp32) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
-    mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl);
+    mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl, top);
 }
+
+
 // workgroups of the fused update: level-1 banks (grid-stride) plus a few for
 // the dofs outside the preconditioned range; one r.r partial each
 void mas_update_layout(const Ctx& c, int& g1, int& gx) {
@@ -1429,7 +1538,7 @@ void mas_update_layout(const Ctx& c, int& g1, int& gx) {
 
 template <typename V>
 static void update_restrict_t(Ctx& c, double* x, V* r, const V* p, const V* Ap, const double* part_pap, int np_pap,
-                              const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
+                              const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32, bool top) {
     MasState& m = c.mas;
     int g1, gx;
     mas_update_layout(c, g1, gx);
@@ -1449,7 +1558,7 @@ static void update_restrict_t(Ctx& c, double* x, V* r, const V* p, const V* Ap, 
                            (const int*)m.l1lo.p, (const int*)m.l1hi.p, m.nb1, g1, m.R64.p);
     }
     SGIPC_LAUNCH_CHECK();
-    if (m.L >= 4) {
+    if (m.L >= 4 && top) {
         if (fp32)
             hipLaunchKernelGGL((k_restrict_top<float>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
                                (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
@@ -1463,12 +1572,36 @@ static void update_restrict_t(Ctx& c, double* x, V* r, const V* p, const V* Ap, 
 }
 
 void mas_update_restrict(Ctx& c, double* x, double* r, const double* p, const double* Ap, const double* part_pap,
-                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
-    update_restrict_t<double>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32);
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32,
+                         bool top) {
+    update_restrict_t<double>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32, top);
 }
 void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const float* Ap, const double* part_pap,
-                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32) {
-    update_restrict_t<float>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32);
+                         int np_pap, const double* part_rz, int np_rz, double* part_rr, int* ctrl, bool fp32,
+                         bool top) {
+    update_restrict_t<float>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32, top);
+}
+
+template <typename T>
+static void solve_top_t(Ctx& c, const T* inv, T* R, T* Z, const int* ctrl) {
+    MasState& m = c.mas;
+    const int top_bank0 = m.L >= 4 ? m.level_size_h[7] / m.bank : m.nbanks;
+    prof_begin(c, PROF_MAS_SOLVE);
+    hipLaunchKernelGGL((k_mas_solve_top<T>), dim3(1 + div_up(top_bank0, kCW)), dim3(kCT), 0, c.stream, m.L,
+                       (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
+                       top_bank0, m.nbanks, inv, R, Z, ctrl);
+    SGIPC_LAUNCH_CHECK();
+    prof_end(c, PROF_MAS_SOLVE);
+}
+
+bool mas_fused(const Ctx& c) { return c.mas_fused && c.mas.bank == 16 && c.mas.ready; }
+
+void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl) {
+    if (fp32) solve_top_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
+    else {
+        if (!c.mas.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
+        solve_top_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+    }
 }
 
 void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl) {

@@ -128,69 +128,6 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[C_ITER] += 1;
 }
 
-// Stop tests, evaluated identically by every block of the kernel that
-// follows the x/r update (so no block depends on another's flag write):
-// rule 0 (reference): |rz_prev| <= tol * rz0 on the pre-update rz;
-// rule 1: ||r|| <= tol * ||b|| on the updated residual.  rule < 0: none.
-// This thread's share of the stop test's partial sum (the strided part of
-// sum_partials), so a kernel can issue its loads early and reduce late.
-__device__ inline double stop_share(int rule, const double* __restrict__ part_rz_prev, int np_rz,
-                                    const double* __restrict__ part_rr, int np_rr) {
-    double v = 0.0;
-    if (rule == 0) {
-        for (int i = threadIdx.x; i < np_rz; i += blockDim.x) v += part_rz_prev[i];
-    } else if (rule > 0) {
-        for (int i = threadIdx.x; i < np_rr; i += blockDim.x) v += part_rr[i];
-    }
-    return v;
-}
-
-// Stop test from the shares: rule 0 |r.z| <= tol * r0.z0 (previous r.z),
-// rule 1 ||r|| <= tol * ||b||; block 0 records the progress and the stop.
-__device__ inline bool stop_test_share(int rule, double share, double* __restrict__ sc, int* __restrict__ ctrl,
-                                       double* sh) {
-    if (rule < 0) return false;
-    const double sum = block_sum_all(share, sh);
-    bool stop;
-    double q;  // progress: the ratio compared with the tolerance
-    if (rule == 0) {
-        stop = fabs(sum) <= sc[S_TOL] * sc[S_RZ0];
-        q = fabs(sum) / sc[S_RZ0];
-    } else {
-        stop = sqrt(sum) <= sc[S_TOL] * sqrt(sc[S_BB]);
-        q = sqrt(sum) / sqrt(sc[S_BB]);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc[S_PROG] = q;
-        if (stop) ctrl[C_STOP] = ctrl[C_ITER] - 1;
-    }
-    return stop;
-}
-
-__device__ inline bool stop_test(int rule, const double* __restrict__ part_rz_prev, int np_rz,
-                                 const double* __restrict__ part_rr, int np_rr, double* __restrict__ sc,
-                                 int* __restrict__ ctrl, double* sh) {
-    return stop_test_share(rule, stop_share(rule, part_rz_prev, np_rz, part_rr, np_rr), sc, ctrl, sh);
-}
-
-constexpr int kLevelBatch = 8;  // MAS levels whose loads are in flight together in the collect
-
-constexpr int C_PUB = C_NUM + 2;  // publications of the progress words (pinned polling)
-
-// Progress words for the host's pinned-memory polling: the stop and
-// iteration words and the progress ratio, then (after a system-scope fence)
-// the publication count the host waits on.  One thread.
-__device__ inline void publish_progress(int* __restrict__ ctrl, const double* __restrict__ sc, volatile int* hp,
-                                        volatile double* hq) {
-    const int pub = ctrl[C_PUB] + 1;
-    ctrl[C_PUB] = pub;
-    hq[0] = sc[S_PROG];
-    hp[0] = ctrl[C_STOP];
-    hp[1] = ctrl[C_ITER];
-    __threadfence_system();
-    hp[2] = pub;
-}
-
 // Stop test, then z = sum_l Z_l[node_l(v)] on the MAS range, the 12x12
 // block-Jacobi of the affine bodies on block rows < abd_rows (when abd_inv is
 // set), identity elsewhere, and partials of r.z; one thread per block row.
@@ -467,6 +404,8 @@ struct Engine {
     int maxit = 0;
     double tol_now = 0.0;  // tolerance of the current run (progress prediction)
     int gU = 1;  // workgroups (and r.r partials) of the x/r update
+    int nRZ = 1; // r.z partials written by the preconditioner apply
+    bool fused = false;  // MAS apply as one launch
 
     int G = 8;  // lanes per block row in the SpMV (4, 8 or 16)
     // mixed precision: the row-contiguous FP32 copy (null: the upper BCOO)
@@ -482,14 +421,15 @@ struct Engine {
     void precond_dot(double* rz_out, const double* rz_prev, int rule_now) {
         hipStream_t s = c.stream;
         if (mas) {
-            mas_bank_solve(c, kFp32Mas, ctrl);
+            if (fused) mas_solve_with_top(c, kFp32Mas, ctrl);
+            else mas_bank_solve(c, kFp32Mas, ctrl);
             hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
                                c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, abd_inv(c), abd_rows(c), rz_out,
-                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now, pub_h, pub_q);
+                               rz_prev, nRZ, part_rr, gU, sc, ctrl, rule_now, pub_h, pub_q);
             SGIPC_LAUNCH_CHECK();
         } else {
             hipLaunchKernelGGL((k_simple_precond_dot<V>), dim3(gN), dim3(kThreads), 0, s, N, r, z, Dinv, rz_out,
-                               rz_prev, gN, part_rr, gU, sc, ctrl, rule_now);
+                               rz_prev, nRZ, part_rr, gU, sc, ctrl, rule_now);
             SGIPC_LAUNCH_CHECK();
         }
     }
@@ -503,12 +443,12 @@ struct Engine {
         prof_begin(c, PROF_SPMV);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, S.ent.p, val, (const V*)z, p, Ap,
-                               (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
+                               (const double*)rz_last, (const double*)rz_this, nRZ, part_pap, ctrl);
         };
         if (vfull) {
             auto launchf = [&](auto kern) {
                 hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, cfull, vfull, (const V*)z, p,
-                                   Ap, (const double*)rz_last, (const double*)rz_this, gN, part_pap, ctrl);
+                                   Ap, (const double*)rz_last, (const double*)rz_this, nRZ, part_pap, ctrl);
             };
             if (G == 4) launchf(k_spmv_pap_full<4, V>);
             else if (G == 16) launchf(k_spmv_pap_full<16, V>);
@@ -519,10 +459,10 @@ struct Engine {
         SGIPC_LAUNCH_CHECK();
         prof_end(c, PROF_SPMV);
         if (mas) {
-            mas_update_restrict(c, x, r, p, Ap, part_pap, gS, rz_last, gN, part_rr, ctrl, kFp32Mas);
+            mas_update_restrict(c, x, r, p, Ap, part_pap, gS, rz_last, nRZ, part_rr, ctrl, kFp32Mas, !fused);
         } else {
             hipLaunchKernelGGL((k_update_xr<V>), dim3(gN), dim3(kThreads), 0, s, N, x, r, (const V*)p, (const V*)Ap,
-                               (const double*)part_pap, gS, (const double*)rz_last, gN, part_rr, ctrl);
+                               (const double*)part_pap, gS, (const double*)rz_last, nRZ, part_rr, ctrl);
             SGIPC_LAUNCH_CHECK();
         }
         precond_dot(rz_this, rz_last, rule);
@@ -541,6 +481,8 @@ struct Engine {
         } else {
             gU = gN;
         }
+        fused = mas && mas_fused(c);
+        nRZ = gN;
         hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, max_iter);
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL, tol);
@@ -550,9 +492,9 @@ struct Engine {
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_bb, gN, sc, (int)S_BB);
         SGIPC_LAUNCH_CHECK();
-        if (mas) mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl);
+        if (mas) mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl, !fused);
         precond_dot(part_rz[0], part_rz[1], -1);
-        hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], gN, sc, (int)S_RZ0);
+        hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], nRZ, sc, (int)S_RZ0);
         SGIPC_LAUNCH_CHECK();
         double rz0 = 0.0;
         SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -570,6 +512,7 @@ struct Engine {
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
         mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c)); mix((size_t)vfull); mix((size_t)cfull);
+        mix((size_t)fused); mix((size_t)nRZ);
         if (mas) {
             const MasState& m = c.mas;
             mix((size_t)m.nbanks); mix((size_t)m.L); mix((size_t)m.M); mix((size_t)m.n); mix((size_t)m.bank);
