@@ -16,6 +16,7 @@ baseline (the oracle's FP64 restatement of the same step, one thread).
 """
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -135,12 +136,20 @@ def broad_phase_timing(sc, sysm, reps=5):
     return out
 
 
-def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5):
-    """The oracle's FP64 C restatement of the same Newton step, one thread,
-    repeated until at least min_secs of CPU work (bounded by max_steps)."""
+def cpu_baseline(sc, pairs, mat_index, counts, min_secs=10.0, max_steps=5, threads=1):
+    """The oracle's FP64 C restatement of the same Newton step on `threads`
+    host threads (FEM assembly, convert, MAS build and PCG split over
+    std::thread-style pthreads; 1 = the serial restatement), repeated until
+    at least min_secs of CPU work (bounded by max_steps)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    with O.threads(threads):
+        return _cpu_baseline(O, sc, pairs, mat_index, counts, min_secs, max_steps)
+
+
+def _cpu_baseline(O, sc, pairs, mat_index, counts, min_secs, max_steps):
+    import numpy as np
     ns, nn, nl = O.vertex_neighbors(sc.n, sc.tets)  # topology setup: one-time, untimed (as on the GPU)
     pm, rm = O.bank_map(sc.part, sc.bank)
     total, steps, it = 0.0, 0, 0
@@ -336,16 +345,58 @@ def reference_rule(sysm, steps):
 
 
 def host_cpu():
-    model = None
+    """The host's CPU: model, logical CPUs, sockets and physical cores (from
+    /proc/cpuinfo; lscpu where present), and the CPUs this process may use."""
+    info = {"model": None, "logical_cpus": os.cpu_count(), "sockets": None, "physical_cores": None}
     try:
+        phys, cores_of = set(), set()
+        cur = {}
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and info["model"] is None:
+                    info["model"] = v
+                elif k == "physical id":
+                    cur["p"] = v
+                    phys.add(v)
+                elif k == "core id":
+                    cur["c"] = v
+                elif not k and cur:
+                    if "p" in cur and "c" in cur:
+                        cores_of.add((cur["p"], cur["c"]))
+                    cur = {}
+        if "p" in cur and "c" in cur:
+            cores_of.add((cur["p"], cur["c"]))
+        info["sockets"] = len(phys) or None
+        info["physical_cores"] = len(cores_of) or None
     except OSError:
         pass
-    return {"model": model, "logical_cpus": os.cpu_count()}
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["usable_cpus"] = os.cpu_count()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket" and info.get("sockets"):
+                info["physical_cores"] = int(v) * info["sockets"]
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def baseline_threads():
+    """Threads for the all-cores CPU baseline: the CPUs this process may use,
+    at most 16 (the GPU box's CPU share for one GPU)."""
+    h = host_cpu()
+    return max(1, min(16, h.get("usable_cpus") or os.cpu_count() or 1))
 
 
 def mas_banks(sysm, bank):
@@ -439,12 +490,19 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        csecs, csteps, cit = cpu_baseline(sc, sysm.pairs.cpu().numpy().reshape(-1, 4) if sysm.n_pairs else None,
-                                          sysm.mat_index.cpu().numpy() if sysm.n_pairs else None, sysm.counts)
-        cpu = {"value": round(csteps / csecs, 5), "unit": "Newton solves/s", "cores": 1, "kind": "port",
+        cp = sysm.pairs.cpu().numpy().reshape(-1, 4) if sysm.n_pairs else None
+        cm = sysm.mat_index.cpu().numpy() if sysm.n_pairs else None
+        nt = baseline_threads()
+        print("CPU baseline (oracle): %d threads ..." % nt, file=sys.stderr, flush=True)
+        csecs, csteps, cit = cpu_baseline(sc, cp, cm, sysm.counts, min_secs=8.0, threads=nt)
+        print("CPU baseline (oracle): 1 thread ...", file=sys.stderr, flush=True)
+        ssecs, ssteps, sit = cpu_baseline(sc, cp, cm, sysm.counts, min_secs=5.0, max_steps=2, threads=1)
+        cpu = {"value": round(csteps / csecs, 5), "unit": "Newton solves/s", "cores": nt, "kind": "port",
                "sample": "%d full Newton step(s) of the same workload (barrier + FEM assembly, convert, MAS build, "
-                         "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement, 1 thread, "
-                         "%.1f s" % (csteps, cit, csecs),
+                         "MAS-PCG to 1e-6 true rel. residual: %d iterations), oracle FP64 C restatement on %d "
+                         "threads, %.1f s" % (csteps, cit, nt, csecs),
+               "single_thread": {"value": round(ssteps / ssecs, 5), "unit": "Newton solves/s", "cores": 1,
+                                 "sample": "%d step(s), %d iterations, %.1f s" % (ssteps, sit, ssecs)},
                "label": "CPU restatement of the reference algorithm (baseline only)", "host": host_cpu()}
 
     g = os.environ.get("SGIPC_SPMV_G", "8")
