@@ -45,6 +45,11 @@ def parse():
                          "64 affine cubes (kappa 1e11) on a 100x20x100-cell soft slab, one coupled Newton step, "
                          "mixed vs FP64")
     ap.add_argument("--scenes", type=int, default=256, help="batched: number of scenes")
+    ap.add_argument("--scene-vertices", type=int, nargs=2, default=[500, 20000], metavar=("MIN", "MAX"),
+                    help="batched: range of the log-uniform per-scene vertex counts")
+    ap.add_argument("--batch-form", choices=["auto", "sequence", "persistent"], default="auto",
+                    help="batched: the autotuner's choice, the graph-captured launch sequence, or the persistent "
+                         "launch (a workgroup per scene)")
     ap.add_argument("--fd-dirs", type=int, default=32, help="cantilever_adjoint: FD directions")
     ap.add_argument("--rollout-steps", type=int, default=20, help="cantilever_adjoint: time steps")
     ap.add_argument("--newton-tol", type=float, default=1e-13,
@@ -789,11 +794,32 @@ def batched(a):
     from sgipc import scene, capi
     from sgipc.batch import BatchedNewton, SequentialNewton
     print("batched: generating %d scenes ..." % a.scenes, file=sys.stderr, flush=True)
-    scs = scene.batch_scenes(a.scenes)
+    scs = scene.batch_scenes(a.scenes, n_min=a.scene_vertices[0], n_max=a.scene_vertices[1])
     bn = BatchedNewton(scs, "cuda:0", precond=capi.PRECOND_MAS_FP32)
     print("batched: %d vertices, %d tets; warm-up + autotune ..." % (bn.sc.n, bn.sc.T), file=sys.stderr, flush=True)
     bn.newton_step(precision=capi.PRECISION_MIXED)
     tuned = bn.autotune(capi.PRECISION_MIXED)
+    tuned_form = bn.mode()
+    # the solve alone (same assembled system) in each form at the tuned lanes per row
+    forms = {}
+    for name, pers, threads in (("sequence", False, 1024), ("persistent_512", True, 512),
+                                ("persistent_1024", True, 1024)):
+        bn.set_mode(pers, threads)
+        bn.solve(precision=capi.PRECISION_MIXED)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            fst = bn.solve(precision=capi.PRECISION_MIXED)
+        torch.cuda.synchronize()
+        forms[name] = {"solve_ms": round(1e3 * (time.perf_counter() - t0) / 3, 3),
+                       "all_converged": all(fst[i].converged for i in range(bn.S)),
+                       "pcg_iters_max": max(fst[i].iterations for i in range(bn.S))}
+        print("  %s: %.2f ms per batched solve" % (name, forms[name]["solve_ms"]), file=sys.stderr, flush=True)
+    if a.batch_form == "auto":
+        bn.set_mode(*tuned_form)
+    else:
+        bn.set_mode(a.batch_form == "persistent", tuned_form[1])
+    form = bn.mode()
     for _ in range(max(a.warmup - 1, 0)):
         bn.newton_step(precision=capi.PRECISION_MIXED)
     torch.cuda.synchronize()
@@ -852,7 +878,11 @@ def batched(a):
                                "vertices), E log-U[1e4,1e7], nu U[0.3,0.48], dt=0.01" %
                                (bn.S, bn.sc.n, bn.sc.T, min(s.n for s in scs), max(s.n for s in scs)),
                    "precision": "FP32 inner PCG, per-scene FP64 refinement", "preconditioner": "MAS, 16-vertex banks",
-                   "schedule": {"spmv_lanes_per_row": tuned[0], "iterations_per_graph": tuned[1]}},
+                   "schedule": {"spmv_lanes_per_row": tuned[0], "iterations_per_graph": tuned[1],
+                                "form": ("persistent launch, %d threads per workgroup" % form[1]) if form[0]
+                                else "graph-captured launch sequence",
+                                "autotuned_form": "persistent" if tuned_form[0] else "sequence"}},
+        "batched_solve_by_form": forms,
         "pcg_iters_per_scene": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
         "rel_residual_max": rel,
         "all_converged": bool(conv),

@@ -349,6 +349,18 @@ int sgipc_autotune(sgipc_ctx* ctx, const double* d_b, int n_scenes, const int* h
 int sgipc_set_deterministic(sgipc_ctx* ctx, int on);
 /* Set the schedule directly (group 4, 8 or 16; 1..64 iterations per graph). */
 int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
+/* Batched solver form; no reference counterpart.  persistent = 0: the
+ * graph-captured launch sequence above; 1: one persistent launch in which
+ * each workgroup of `threads` (512 or 1024) takes scenes, largest first, and
+ * runs a scene's whole solve -- every PCG iteration, the MAS apply at every
+ * level and, mixed, the FP64 refinement -- with no other synchronisation
+ * than its own barriers (needs 16-vertex MAS banks; otherwise the launch
+ * sequence runs).  Per scene the same loop, stop rules and iteration
+ * counts; dot products sum in another order (round-off).  max_workgroups > 0
+ * caps the workgroups (0: as many as are resident at once).  The autotuner
+ * includes both forms among its candidates and leaves the faster set. */
+int sgipc_set_batch_mode(sgipc_ctx* ctx, int persistent, int threads, int max_workgroups);
+int sgipc_get_batch_mode(sgipc_ctx* ctx, int* h_persistent, int* h_threads);
 /* How the MAS build inverts its bank matrices (the reference:
  * __inverse6_P96x96, StiffGIPC/MASPreconditioner.cu:630-726, scalar FP64
  * Gauss-Jordan, stored FP32).  Applies to the next sgipc_precond_build.

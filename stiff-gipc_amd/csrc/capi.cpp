@@ -571,6 +571,26 @@ int sgipc_set_schedule(sgipc_ctx* c, int group, int iters_per_graph) {
     });
 }
 
+int sgipc_set_batch_mode(sgipc_ctx* c, int persistent, int threads, int max_workgroups) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (persistent != 0 && persistent != 1) throw sgipc::arg_error("set_batch_mode: persistent must be 0 or 1");
+        if (threads != 512 && threads != 1024) throw sgipc::arg_error("set_batch_mode: threads must be 512 or 1024");
+        if (max_workgroups < 0) throw sgipc::arg_error("set_batch_mode: max_workgroups must be >= 0");
+        c->batch_persistent = persistent;
+        c->batch_threads = threads;
+        c->batch_max_wg = max_workgroups;
+    });
+}
+
+int sgipc_get_batch_mode(sgipc_ctx* c, int* persistent, int* threads) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (persistent) *persistent = c->batch_persistent;
+        if (threads) *threads = c->batch_threads;
+    });
+}
+
 int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts* o, sgipc_solve_stats* st) {
     CHECK_CTX(c);
     return guard(c, [&] {

@@ -117,6 +117,10 @@ struct BatchWork {
     DBuf<int> stop, iters, noconv;                  // per scene
     DBuf<int> ctrl, bad, rstart;                    // C_* words; coupling check counter; row starts
     PcgGraph graph64, graph32, graph_mixed;
+    // persistent solver (pcg_persist.hip)
+    DBuf<int> order, queue, refine;                 // scenes largest first; scene queue; refinements
+    DBuf<int> mlo, mhi, node_scene, bank_scene;     // per scene and MAS level node ranges; scene of each node / bank
+    DBuf<double> relres;                            // per-scene true relative residuals
 };
 
 // Affine bodies coupled to the FEM system (abd.hip)
@@ -230,6 +234,9 @@ struct Ctx {
     BatchWork batch;
     Lbvh lbvh;
     int batch_iters_per_graph = 4;  // batched PCG iterations captured per graph launch
+    int batch_persistent = 0;       // batched PCG: 1 = one persistent launch (a workgroup per scene, 16-vertex banks)
+    int batch_threads = 1024;       // its workgroup size (512 or 1024)
+    int batch_max_wg = 0;           // cap on its workgroups (0: as many as fit the device at once)
     DBuf<double> tune_x;            // autotuner output scratch
     DBuf<int> scan_tmp;
     // contact classification workspaces
@@ -301,9 +308,23 @@ void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
 // fused multilevel restriction of r[sub3 .. sub3 + 3n) into R_l (FP64 or FP32
 // preconditioner data); bank solves of all levels; both skip when ctrl[0] != 0
-void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top = true);
-void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top = true);
-void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl);
+// Batched solves: banks whose nodes all belong to one stopped scene are
+// skipped (bank_scene: the scene of every bank's nodes, -1 when they belong
+// to several or none; sstop: per-scene stop words).  Default: no skipping.
+struct SceneSkip {
+    const int* bank_scene = nullptr;
+    const int* sstop = nullptr;
+    __device__ bool stopped(int bank) const {
+        if (!bank_scene) return false;
+        const int s = bank_scene[bank];
+        return s >= 0 && sstop[s] != 0;
+    }
+};
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top = true,
+                  SceneSkip skip = SceneSkip{});
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top = true,
+                  SceneSkip skip = SceneSkip{});
+void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip = SceneSkip{});
 // PCG iteration with 16-vertex banks: the top-level restriction (levels
 // 2 -> L-1) runs in the bank-solve launch, beside the banks of levels 0..2,
 // instead of its own launch before it (the x/r update then restricts levels
@@ -347,6 +368,13 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
 void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, double* x, const SolveOpts& o,
                        SolveStats* st);
 void batch_check_coupling(Ctx& c, const std::vector<int>& row_start);
+// the batched solve as one persistent launch: each workgroup takes scenes
+// (largest first) and runs a scene's whole solve (validated arguments)
+// scene of every MAS node and bank (BatchWork node_scene, bank_scene) and,
+// when lo is given, every scene's node range [lo, hi) at every level (S x L)
+void batch_scene_maps(Ctx& c, int S, const int* d_row_start, int* lo, int* hi);
+void pcg_solve_batched_persistent(Ctx& c, int S, const std::vector<int>& row_start, const double* b, double* x,
+                                  const SolveOpts& o, SolveStats* st);
 void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precision, int& group, int& per_graph);
 double ccd_step_bound(Ctx& c, int npt, const int* pt, int nee, const int* ee, const double* x, const double* dir,
                       double slack);

@@ -785,8 +785,10 @@ typedef __attribute__((address_space(3))) void* lptr_t;
 // bytes), then lane i forms row i, summing over columns in order.
 template <int B, typename T>
 __global__ void __launch_bounds__(B == 16 ? 64 : 128) k_bank_solve(const T* __restrict__ inv, const T* __restrict__ R,
-                                                                    T* __restrict__ Z, const int* __restrict__ ctrl) {
+                                                                    T* __restrict__ Z, const int* __restrict__ ctrl,
+                                                                    SceneSkip skip) {
     if (ctrl && ctrl[0]) return;
+    if (skip.stopped(blockIdx.x)) return;
     constexpr int D = 3 * B;
     constexpr int NP = D * (D + 1) / 2;
     constexpr int NT = (B == 16) ? 64 : 128;
@@ -966,8 +968,9 @@ __global__ void __launch_bounds__(kTB) k_restrict_all(const V* __restrict__ r, l
                                                        const unsigned* __restrict__ comp_mask,
                                                        const int* __restrict__ l1lo, const int* __restrict__ l1hi,
                                                        T* __restrict__ R,
-                                                       const int* __restrict__ ctrl) {
+                                                       const int* __restrict__ ctrl, SceneSkip skip) {
     if (ctrl && ctrl[0]) return;
+    if (skip.bank_scene && skip.stopped(ls[3] / B + (int)blockIdx.x)) return;
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
     auto value = [&](int v, int, bool own, T& v0, T& v1, T& v2) {
@@ -1218,13 +1221,14 @@ int num_levels(int nn, int B) {
 }
 
 template <typename V, typename T>
-void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl, bool top = true) {
+void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl, bool top = true,
+                    SceneSkip skip = SceneSkip{}) {
     MasState& m = c.mas;
     if (m.nb1 <= 0) return;
     hipLaunchKernelGGL((k_restrict_all<V, T>), dim3(m.nb1), dim3(kTB), 0, c.stream, r, sub3, m.bank, m.L,
                        (const int*)m.level_size.p, (const int*)m.part_map.p, (const int*)m.going_next.p,
                        (const unsigned*)m.fine_mask.p, (const unsigned*)m.comp_mask.p,
-                       (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl);
+                       (const int*)m.l1lo.p, (const int*)m.l1hi.p, R, ctrl, skip);
     SGIPC_LAUNCH_CHECK();
     if (top && m.L >= 4) {
         hipLaunchKernelGGL((k_restrict_top<T>), dim3(1), dim3(kTop), 0, c.stream, m.bank, m.L,
@@ -1234,13 +1238,13 @@ void mas_restrict_t(Ctx& c, const V* r, long long sub3, T* R, const int* ctrl, b
 }
 
 template <typename T>
-void mas_solve_t(Ctx& c, const T* inv, const T* R, T* Z, const int* ctrl) {
+void mas_solve_t(Ctx& c, const T* inv, const T* R, T* Z, const int* ctrl, SceneSkip skip = SceneSkip{}) {
     MasState& m = c.mas;
     prof_begin(c, PROF_MAS_SOLVE);
     if (m.bank == 16)
-        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, c.stream, inv, R, Z, ctrl);
+        hipLaunchKernelGGL((k_bank_solve<16, T>), dim3(m.nbanks), dim3(64), 0, c.stream, inv, R, Z, ctrl, skip);
     else
-        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, c.stream, inv, R, Z, ctrl);
+        hipLaunchKernelGGL((k_bank_solve<32, T>), dim3(m.nbanks), dim3(128), 0, c.stream, inv, R, Z, ctrl, skip);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_MAS_SOLVE);
 }
@@ -1517,13 +1521,13 @@ void mas_apply32(Ctx& c, const double* r, double* z) {
     if (!c.mas.ready) throw state_error("MAS apply: build first");
     mas_apply_t<float>(c, r, z, c.mas.R32.p, c.mas.Z32.p, c.mas.inv32.p);
 }
-void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top) {
-    if (fp32) mas_restrict_t<double, float>(c, r, sub3, c.mas.R32.p, ctrl, top);
-    else mas_restrict_t<double, double>(c, r, sub3, c.mas.R64.p, ctrl, top);
+void mas_restrict(Ctx& c, const double* r, long long sub3, bool fp32, const int* ctrl, bool top, SceneSkip skip) {
+    if (fp32) mas_restrict_t<double, float>(c, r, sub3, c.mas.R32.p, ctrl, top, skip);
+    else mas_restrict_t<double, double>(c, r, sub3, c.mas.R64.p, ctrl, top, skip);
 }
-void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top) {
+void mas_restrict(Ctx& c, const float* r, long long sub3, bool fp32, const int* ctrl, bool top, SceneSkip skip) {
     if (!fp32) throw state_error("MAS: FP32 residual needs the FP32 preconditioner data");
-    mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl, top);
+    mas_restrict_t<float, float>(c, r, sub3, c.mas.R32.p, ctrl, top, skip);
 }
 
 
@@ -1604,11 +1608,11 @@ void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl) {
     }
 }
 
-void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl) {
-    if (fp32) mas_solve_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
+void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip) {
+    if (fp32) mas_solve_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl, skip);
     else {
         if (!c.mas.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
-        mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+        mas_solve_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl, skip);
     }
 }
 

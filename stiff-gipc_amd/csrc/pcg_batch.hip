@@ -321,8 +321,9 @@ struct BEngine {
 
     void precond() {
         // z = M^-1 r over the union, then r.z partials per chunk
-        mas_restrict(c, r, 0, fp32mas, w.ctrl.p);
-        mas_bank_solve(c, fp32mas, w.ctrl.p);
+        const SceneSkip skip{w.bank_scene.p, w.stop.p};
+        mas_restrict(c, r, 0, fp32mas, w.ctrl.p, true, skip);
+        mas_bank_solve(c, fp32mas, w.ctrl.p, skip);
         hipLaunchKernelGGL((k_bcollect<V, T>), dim3(nch), dim3(kBT), 0, c.stream, (const int*)w.chunk_row.p,
                            (const int*)w.chunk_scene.p, (const V*)r, z, c.mas.n, c.mas.L, (const int*)c.mas.node_of.p,
                            (const T*)Zm(), (const int*)w.stop.p, w.part_rz.p, (const int*)w.ctrl.p);
@@ -385,7 +386,7 @@ struct BEngine {
                               (const void*)w.bb.p, (const void*)w.rz.p, (const void*)w.rz0.p, (const void*)w.alpha.p,
                               (const void*)w.beta.p, (const void*)w.tol.p, (const void*)w.segout.p,
                               (const void*)w.stop.p, (const void*)w.iters.p, (const void*)w.noconv.p,
-                              (const void*)w.ctrl.p})
+                              (const void*)w.ctrl.p, (const void*)w.bank_scene.p})
             mix((size_t)q);
         mix((size_t)c.stream);
         const MasState& m = c.mas;
@@ -502,11 +503,17 @@ void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, d
     for (int s = 0; s < S; ++s)
         if (rs[s + 1] < rs[s]) throw arg_error("solve_batched: row_start must be non-decreasing");
     if (!c.have_bsr) bsr_build(c);
+    if (c.batch_persistent && c.mas.bank == 16) {
+        batch_check_coupling(c, rs);
+        pcg_solve_batched_persistent(c, S, rs, b, x, o, st);
+        return;
+    }
     hipStream_t s = c.stream;
     BatchWork& w = c.batch;
     const int G = c.spmv_group;
     build_chunks(c, w, rs, 1024 / G);
     batch_check_coupling(c, rs);
+    batch_scene_maps(c, S, w.rstart.p, nullptr, nullptr);  // banks of stopped scenes are skipped
     for (DBuf<double>* q : {&w.bb, &w.rz, &w.rz0, &w.alpha, &w.beta, &w.tol, &w.segout}) q->reserve(S + 1);
     for (DBuf<int>* q : {&w.stop, &w.iters, &w.noconv}) q->reserve(S + 1);
     w.ctrl.reserve(C_NUM + 8);
@@ -627,8 +634,10 @@ namespace sgipc {
 // Schedule autotuner: times a fixed number of PCG iterations (unreachable
 // tolerance) for each candidate schedule on the current system and keeps the
 // fastest.  Candidates: lanes per block row in the SpMV (4, 8, 16) and, for
-// the batched solver, iterations captured per graph launch (2, 4, 8).
-// Returns the chosen (group, iterations per graph).
+// the batched solver, iterations captured per graph launch (2, 4, 8) of the
+// launch sequence, and (16-vertex banks) the persistent form with 512 or
+// 1024 threads per workgroup.  Returns the chosen (group, iterations per
+// graph); the chosen batched form stays in the context.
 void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precision, int& group, int& per_graph) {
     if (!c.have_bcoo) throw state_error("autotune: convert first");
     const long long N = 3LL * c.nb;
@@ -640,45 +649,61 @@ void autotune(Ctx& c, const double* b, int S, const int* h_row_start, int precis
     o.max_iter = 48;
     o.inner_tol = 0.0;
     o.check_every = 48;
-    const int groups[3] = {4, 8, 16};
-    const int pgs[3] = {2, 4, 8};
-    const int npg = S > 0 ? 3 : 1;
-    const int save_g = c.spmv_group, save_pg = c.batch_iters_per_graph;
+    struct Cand {
+        int group, per_graph, persistent, threads;
+    };
+    std::vector<Cand> cands;
+    for (int g : {4, 8, 16}) {
+        if (S > 0)
+            for (int pg : {2, 4, 8}) cands.push_back({g, pg, 0, c.batch_threads});
+        else
+            cands.push_back({g, c.batch_iters_per_graph, 0, c.batch_threads});
+    }
+    if (S > 0 && c.mas.ready && c.mas.bank == 16)
+        for (int g : {4, 8, 16})
+            for (int th : {512, 1024}) cands.push_back({g, c.batch_iters_per_graph, 1, th});
+    const Cand save{c.spmv_group, c.batch_iters_per_graph, c.batch_persistent, c.batch_threads};
+    auto apply = [&](const Cand& k) {
+        c.spmv_group = k.group;
+        c.batch_iters_per_graph = k.per_graph;
+        c.batch_persistent = k.persistent;
+        c.batch_threads = k.threads;
+    };
     double best = 1e300;
-    int bg = save_g, bpg = save_pg;
+    Cand pick = save;
+    int best_pg = save.per_graph;  // iterations per graph of the fastest launch-sequence candidate
+    double best_seq = 1e300;
     std::vector<SolveStats> st(S > 0 ? S : 1);
     hipEvent_t e0, e1;
     SGIPC_HIP(hipEventCreate(&e0));
     SGIPC_HIP(hipEventCreate(&e1));
     try {
-        for (int gi = 0; gi < 3; ++gi)
-            for (int pi = 0; pi < npg; ++pi) {
-                c.spmv_group = groups[gi];
-                c.batch_iters_per_graph = pgs[pi];
-                float ms = 0.f;
-                for (int rep = 0; rep < 2; ++rep) {  // first run builds the graphs
-                    SGIPC_HIP(hipEventRecord(e0, c.stream));
-                    if (S > 0) pcg_solve_batched(c, S, h_row_start, b, x, o, st.data());
-                    else pcg_solve(c, b, x, o, st[0]);
-                    SGIPC_HIP(hipEventRecord(e1, c.stream));
-                    SGIPC_HIP(hipEventSynchronize(e1));
-                    SGIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
-                }
-                if (ms < best) { best = ms; bg = groups[gi]; bpg = pgs[pi]; }
+        for (const Cand& k : cands) {
+            apply(k);
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {  // first run builds the graphs
+                SGIPC_HIP(hipEventRecord(e0, c.stream));
+                if (S > 0) pcg_solve_batched(c, S, h_row_start, b, x, o, st.data());
+                else pcg_solve(c, b, x, o, st[0]);
+                SGIPC_HIP(hipEventRecord(e1, c.stream));
+                SGIPC_HIP(hipEventSynchronize(e1));
+                SGIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
             }
+            if (ms < best) { best = ms; pick = k; }
+            if (!k.persistent && ms < best_seq) { best_seq = ms; best_pg = k.per_graph; }
+        }
     } catch (...) {
-        c.spmv_group = save_g;
-        c.batch_iters_per_graph = save_pg;
+        apply(save);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c.spmv_group = bg;
-    c.batch_iters_per_graph = bpg;
-    group = bg;
-    per_graph = bpg;
+    if (pick.persistent) pick.per_graph = best_pg;
+    apply(pick);
+    group = pick.group;
+    per_graph = pick.per_graph;
 }
 
 }  // namespace sgipc

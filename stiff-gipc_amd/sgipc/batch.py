@@ -43,6 +43,16 @@ class BatchedNewton:
                           C.byref(g), C.byref(pg))
         return g.value, pg.value
 
+    def set_mode(self, persistent, threads=1024, max_workgroups=0):
+        """Batched form: the graph-captured launch sequence (False) or one
+        persistent launch, a workgroup of `threads` per scene (True)."""
+        self.sys.ctx.call("sgipc_set_batch_mode", int(bool(persistent)), threads, max_workgroups)
+
+    def mode(self):
+        p, t = C.c_int(0), C.c_int(0)
+        self.sys.ctx.call("sgipc_get_batch_mode", C.byref(p), C.byref(t))
+        return bool(p.value), t.value
+
     def solve(self, precision=capi.PRECISION_MIXED, tol=1e-6, stop_rule=capi.STOP_TRUE_REL, max_iter=0):
         o = capi.SolveOpts(precision, stop_rule, tol, 0.3, max_iter, 8)
         self.sys.ctx.call("sgipc_solve_batched", self.S, self.row_start.ctypes.data, _p(self.sys.b),

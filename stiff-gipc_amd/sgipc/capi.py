@@ -81,6 +81,8 @@ SIGNATURES = {
     "sgipc_solve_batched": ([P, I, P, P, P, P, P], I),
     "sgipc_autotune": ([P, P, I, P, I, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_set_schedule": ([P, I, I], I),
+    "sgipc_set_batch_mode": ([P, I, I, I], I),
+    "sgipc_get_batch_mode": ([P, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_fem_param_adjoint": ([P, I, P, P, P, P, D, P, P, P, P], I),
     "sgipc_classify_contacts": ([P, I, P, I, P, P, P, D, P, P, I, C.POINTER(I), P], I),
     "sgipc_assemble_barrier": ([P, I, P, P, P, P, P, D, D, P, I, LL, P], I),

@@ -1,8 +1,12 @@
-"""GPU parity of the batched PCG (independent scenes, one launch sequence)
-through the C ABI: one scene against the oracle's PCG, several scenes
+"""GPU parity of the batched PCG (independent scenes) through the C ABI, in
+both forms -- the graph-captured launch sequence and the persistent launch
+(a workgroup per scene): one scene against the oracle's PCG, several scenes
 against a per-scene PCG written here over the oracle's MAS of the union
 (its block-diagonality per scene checked on the way), the mixed-precision
-form against a direct solve, the coupling check, and the autotuner."""
+form against a direct solve, the persistent form against the launch
+sequence and against itself with fewer workgroups than scenes, the coupling
+check, and the autotuner."""
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -45,13 +49,29 @@ def _solve_batched(sysm, row_start, precision, tol, rule, max_iter=0):
     return st, sysm.dx.cpu().numpy()
 
 
-def test_batched_single_scene_matches_oracle_pcg(torch_cuda, oracle, cube10):
+@contextlib.contextmanager
+def _mode(sysm, form, threads=1024, max_wg=0):
+    """Run the enclosed solves in the given batched form ("sequence" or
+    "persistent"), then restore the launch sequence."""
+    sysm.ctx.call("sgipc_set_batch_mode", int(form == "persistent"), threads, max_wg)
+    try:
+        yield
+    finally:
+        sysm.ctx.call("sgipc_set_batch_mode", 0, 1024, 0)
+
+
+FORMS = ["sequence", "persistent"]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_batched_single_scene_matches_oracle_pcg(torch_cuda, oracle, cube10, form):
     from sgipc import capi
     from sgipc.system import NewtonSystem
-    print("case: batched S=1 cube10 vs oracle pcg", flush=True)
+    print("case: batched S=1 cube10 vs oracle pcg (%s)" % form, flush=True)
     ref = oracle_system(oracle, cube10)
     sysm = NewtonSystem(cube10, "cuda:0", precond=capi.PRECOND_MAS)
     sysm.assemble(); sysm.convert(); sysm.build_precond()
+    sysm.ctx.call("sgipc_set_batch_mode", int(form == "persistent"), 1024, 0)
     kr, kc, kv = ref["bcoo"]
     mas = oracle.Mas(cube10.n, sysm.part_map, sysm.real_map, sysm.ns, sysm.nn, sysm.nl, cube10.bank)
     mas.build(kr, kc, kv)
@@ -77,11 +97,12 @@ def five_scenes(torch_cuda):
     return scs, bn
 
 
-def test_batched_scenes_match_per_scene_pcg(torch_cuda, oracle, five_scenes):
+@pytest.mark.parametrize("form", FORMS)
+def test_batched_scenes_match_per_scene_pcg(torch_cuda, oracle, five_scenes, form):
     from sgipc import capi
     scs, bn = five_scenes
     sysm = bn.sys
-    print("case: batched S=5 vs per-scene PCG over the union MAS", flush=True)
+    print("case: batched S=5 vs per-scene PCG over the union MAS (%s)" % form, flush=True)
     kr, kc, kv = sysm.bcoo()
     n = bn.sc.n
     A = _csr(kr, kc, kv, n)
@@ -90,7 +111,8 @@ def test_batched_scenes_match_per_scene_pcg(torch_cuda, oracle, five_scenes):
     mas = oracle.Mas(n, sysm.part_map, sysm.real_map, sysm.ns, sysm.nn, sysm.nl, bn.sc.bank)
     mas.build(kr, kc, kv)
     tol = 1e-4
-    st, x = _solve_batched(sysm, bn.row_start, capi.PRECISION_FP64, tol, 0)
+    with _mode(sysm, form):
+        st, x = _solve_batched(sysm, bn.row_start, capi.PRECISION_FP64, tol, 0)
     for s in range(bn.S):
         sl = bn.scene_slice(s)
         # per-scene PCG (the reference loop) with the union MAS restricted to the scene
@@ -115,13 +137,15 @@ def test_batched_scenes_match_per_scene_pcg(torch_cuda, oracle, five_scenes):
     assert len(set(st[s].iterations for s in range(bn.S))) > 1    # scenes stop independently
 
 
-def test_batched_mixed_reaches_tolerance_per_scene(torch_cuda, five_scenes):
+@pytest.mark.parametrize("form", FORMS)
+def test_batched_mixed_reaches_tolerance_per_scene(torch_cuda, five_scenes, form):
     import scipy.sparse.linalg as spla
     from sgipc import capi
     scs, bn = five_scenes
-    print("case: batched S=5 mixed FP32/FP64 to 1e-6", flush=True)
+    print("case: batched S=5 mixed FP32/FP64 to 1e-6 (%s)" % form, flush=True)
     bn.sys.build_precond(capi.PRECOND_MAS_FP32)
-    st, x = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_MIXED, 1e-6, 1)
+    with _mode(bn.sys, form):
+        st, x = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_MIXED, 1e-6, 1)
     kr, kc, kv = bn.sys.bcoo()
     A = _csr(kr, kc, kv, bn.sc.n)
     b = bn.sys.b.cpu().numpy()
@@ -132,6 +156,57 @@ def test_batched_mixed_reaches_tolerance_per_scene(torch_cuda, five_scenes):
         xs = spla.spsolve(A[sl, sl].tocsc(), b[sl])
         assert np.linalg.norm(x[sl] - xs) <= 1e-3 * np.linalg.norm(xs)
     bn.sys.build_precond(capi.PRECOND_MAS)
+
+
+@pytest.mark.parametrize("precond", ["MAS", "MAS_FP32"])
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_persistent_matches_launch_sequence(torch_cuda, five_scenes, precond, precision):
+    from sgipc import capi
+    scs, bn = five_scenes
+    if precond == "MAS" and precision == "MIXED":
+        pytest.skip("mixed precision needs the FP32 MAS inverses")
+    print("case: persistent vs launch sequence, S=5, %s, %s" % (precond, precision), flush=True)
+    bn.sys.build_precond(getattr(capi, "PRECOND_" + precond))
+    prec = getattr(capi, "PRECISION_" + precision)
+    rule = 1 if precision == "MIXED" else 0
+    tol = 1e-6 if precision == "MIXED" else 1e-5
+    st0, x0 = _solve_batched(bn.sys, bn.row_start, prec, tol, rule)
+    ref = [(st0[s].iterations, st0[s].converged) for s in range(bn.S)]
+    with _mode(bn.sys, "persistent", 512):
+        st1, x1 = _solve_batched(bn.sys, bn.row_start, prec, tol, rule)
+        # fewer workgroups than scenes: each workgroup takes several scenes
+        # from the queue; a scene's arithmetic does not depend on which
+        with _mode(bn.sys, "persistent", 512, 2):
+            st2, x2 = _solve_batched(bn.sys, bn.row_start, prec, tol, rule)
+    kr, kc, kv = bn.sys.bcoo()
+    A = _csr(kr, kc, kv, bn.sc.n)
+    b = bn.sys.b.cpu().numpy()
+    for s in range(bn.S):
+        sl = bn.scene_slice(s)
+        assert st1[s].converged and ref[s][1]
+        assert abs(st1[s].iterations - ref[s][0]) <= 1, (s, st1[s].iterations, ref[s][0])
+        # the true residual the kernel reports is the scene's FP64 b - A x
+        rel = np.linalg.norm(b[sl] - A[sl, sl] @ x1[sl]) / np.linalg.norm(b[sl])
+        assert abs(st1[s].rel_residual - rel) <= 1e-6 * rel + 1e-15, (s, st1[s].rel_residual, rel)
+        if precision == "MIXED":
+            assert rel <= 1.5 * tol
+        scale = np.linalg.norm(x0[sl])
+        assert np.linalg.norm(x1[sl] - x0[sl]) <= (1e-4 if precision == "MIXED" else 1e-8) * scale
+        assert st2[s].iterations == st1[s].iterations and st2[s].refinements == st1[s].refinements
+    assert np.array_equal(x1, x2)                                    # bitwise: same arithmetic per scene
+    bn.sys.build_precond(capi.PRECOND_MAS)
+
+
+def test_persistent_budget_cut(torch_cuda, five_scenes):
+    from sgipc import capi
+    scs, bn = five_scenes
+    print("case: persistent iteration budget", flush=True)
+    st0, _ = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_FP64, 1e-30, 0, max_iter=7)
+    with _mode(bn.sys, "persistent"):
+        st1, _ = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_FP64, 1e-30, 0, max_iter=7)
+    for s in range(bn.S):
+        assert st0[s].iterations == st1[s].iterations == 7
+        assert not st0[s].converged and not st1[s].converged
 
 
 def test_batched_rejects_coupled_split(torch_cuda, five_scenes):
@@ -153,9 +228,14 @@ def test_autotune_then_batched_solve(torch_cuda, five_scenes):
     it0 = [st0[s].iterations for s in range(bn.S)]
     g, pg = bn.autotune(capi.PRECISION_FP64)
     assert g in (4, 8, 16) and pg in (2, 4, 8)
+    persistent, threads = bn.mode()
+    assert threads in (512, 1024)
+    print("autotune: group %d, iterations per graph %d, persistent %s (%d threads)" % (g, pg, persistent, threads),
+          flush=True)
     st, x = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_FP64, 1e-4, 0)
     # another lane split of the SpMV only reorders row sums (round-off)
     for s in range(bn.S):
         assert abs(st[s].iterations - it0[s]) <= 1 and st[s].converged
     assert np.linalg.norm(x - x0) <= 1e-3 * np.linalg.norm(x0)
     bn.sys.ctx.call("sgipc_set_schedule", 8, 4)
+    bn.set_mode(False)
