@@ -360,6 +360,24 @@ int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
  * caps the workgroups (0: as many as are resident at once).  The autotuner
  * includes both forms among its candidates and leaves the faster set. */
 int sgipc_set_batch_mode(sgipc_ctx* ctx, int persistent, int threads, int max_workgroups);
+
+/* ---- sorting ------------------------------------------------------------------
+ * No reference counterpart (the reference sorts with thrust/CUB inside its
+ * convert, linear_system/utils/converter.cu).  The library sorts with its own
+ * stable LSD radix sort (8-bit digits, one look-back launch per digit) and
+ * its own exclusive scan; SGIPC_SORT_ROCPRIM switches every sort and scan of
+ * the step (convert, SpMV structure, LBVH, ABD contact map, ordered sums) to
+ * rocPRIM's, the cross-check (also env SGIPC_SORT=rocprim).  Both give the
+ * same bits: the sorts are stable. */
+#define SGIPC_SORT_OWN 0
+#define SGIPC_SORT_ROCPRIM 1
+int sgipc_set_sort(sgipc_ctx* ctx, int impl);
+/* Stable sort of n (key, int32 value) pairs by the low `bits` bits of the
+ * keys (key_bytes 4 or 8), d_keys_in -> d_keys_out, values along; a null
+ * d_vals_in sorts the values 0 .. n-1.  Input and output must not overlap.
+ * Synchronises (checks the look-back).  Exposed for cross-checks and timing. */
+int sgipc_sort_pairs(sgipc_ctx* ctx, const void* d_keys_in, void* d_keys_out, const int* d_vals_in, int* d_vals_out,
+                     long long n, int key_bytes, int bits, int impl);
 int sgipc_get_batch_mode(sgipc_ctx* ctx, int* h_persistent, int* h_threads);
 /* How the MAS build inverts its bank matrices (the reference:
  * __inverse6_P96x96, StiffGIPC/MASPreconditioner.cu:630-726, scalar FP64

@@ -592,10 +592,14 @@ long long abd_map_contacts(Ctx& c, long long src, long long m, long long dst) {
     hipLaunchKernelGGL(k_abd_pair_keys, dim3(div_up(m, 256)), dim3(256), 0, s, m, sv, sr, sc, a.nav,
                        (const int*)a.body_of.p, (const int*)a.fixed.p, key, idx);
     SGIPC_LAUNCH_CHECK();
-    size_t bytes = 0;
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
-    char* tmp = a.sort_tmp.reserve(bytes + 1);
-    SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
+    if (c.sort_impl == SORT_OWN) {
+        radix_sort_pairs(c, key, key + m, idx, 0, idx + m, m, 64, s);
+    } else {  // rocPRIM as the cross-check
+        size_t bytes = 0;
+        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
+        char* tmp = a.sort_tmp.reserve(bytes + 1);
+        SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key + m, idx, idx + m, (size_t)m, 0, 64, s));
+    }
     const unsigned long long* skey = key + m;
     const int* sidx = idx + m;
     hipLaunchKernelGGL(k_abd_run_heads, dim3(div_up(m, 256)), dim3(256), 0, s, m, skey, head);

@@ -560,11 +560,16 @@ void barrier_assemble(Ctx& c, int np, const int* pairs, const int* mat_index, co
                        gkey, grec, sentinel);
     SGIPC_LAUNCH_CHECK();
     if (c.deterministic) {
-        size_t bytes = 0;
-        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32,
-                                            c.stream));
-        void* tmp = c.grad_sort_tmp.reserve(bytes + 1);
-        SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32, c.stream));
+        if (c.sort_impl == SORT_OWN) {
+            radix_sort_pairs(c, gkey, gkey + m, (const int*)grec, 0, (int*)(grec + m), m, 32, c.stream);
+        } else {  // rocPRIM as the cross-check
+            size_t bytes = 0;
+            SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32,
+                                                c.stream));
+            void* tmp = c.grad_sort_tmp.reserve(bytes + 1);
+            SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, gkey, gkey + m, grec, grec + m, (size_t)m, 0, 32,
+                                                c.stream));
+        }
         hipLaunchKernelGGL(k_grad_gather, dim3(div_up(m, 256)), dim3(256), 0, c.stream, m,
                            (const unsigned*)(gkey + m), (const unsigned*)(grec + m), (const double*)gscr, sentinel,
                            fb);

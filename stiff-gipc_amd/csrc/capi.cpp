@@ -98,6 +98,7 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         }
         if (const char* f = std::getenv("SGIPC_SPMV_FULL32")) c->spmv_full32 = std::atoi(f) != 0;
         if (const char* f = std::getenv("SGIPC_MAS_FUSED")) c->mas_fused = std::atoi(f) != 0;
+        if (const char* f = std::getenv("SGIPC_SORT")) c->sort_impl = std::string(f) == "rocprim" ? 1 : 0;
         if (const char* f = std::getenv("SGIPC_BANK_INVERSE")) {
             const int v = std::atoi(f);
             if (v >= SGIPC_BANK_INVERSE_VALU64 && v <= SGIPC_BANK_INVERSE_MFMA32_REFINED) c->bank_inverse = v;
@@ -580,6 +581,30 @@ int sgipc_set_batch_mode(sgipc_ctx* c, int persistent, int threads, int max_work
         c->batch_persistent = persistent;
         c->batch_threads = threads;
         c->batch_max_wg = max_workgroups;
+    });
+}
+
+int sgipc_set_sort(sgipc_ctx* c, int impl) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (impl != SGIPC_SORT_OWN && impl != SGIPC_SORT_ROCPRIM) throw sgipc::arg_error("set_sort: unknown sort");
+        c->sort_impl = impl;
+    });
+}
+
+int sgipc_sort_pairs(sgipc_ctx* c, const void* d_keys_in, void* d_keys_out, const int* d_vals_in, int* d_vals_out,
+                     long long n, int key_bytes, int bits, int impl) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || (n > 0 && (!d_keys_in || !d_keys_out || !d_vals_out)))
+            throw sgipc::arg_error("sort_pairs: invalid argument");
+        if (key_bytes != 4 && key_bytes != 8) throw sgipc::arg_error("sort_pairs: keys are 4 or 8 bytes");
+        if (bits < 1 || bits > 8 * key_bytes) throw sgipc::arg_error("sort_pairs: bits must be in [1, 8 key_bytes]");
+        if (impl != SGIPC_SORT_OWN && impl != SGIPC_SORT_ROCPRIM) throw sgipc::arg_error("sort_pairs: unknown sort");
+        set_device(c);
+        sgipc::sort_pairs_dispatch(*c, d_keys_in, d_keys_out, d_vals_in, d_vals_out, n, key_bytes, bits, impl);
+        if (impl == SGIPC_SORT_OWN && sgipc::radix_sort_error(*c))
+            throw std::runtime_error("sort_pairs: radix sort look-back failed");
     });
 }
 

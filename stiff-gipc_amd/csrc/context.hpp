@@ -239,6 +239,9 @@ struct Ctx {
     int batch_max_wg = 0;           // cap on its workgroups (0: as many as fit the device at once)
     DBuf<double> tune_x;            // autotuner output scratch
     DBuf<int> scan_tmp;
+    int sort_impl = 0;              // SORT_OWN, or SORT_ROCPRIM (env SGIPC_SORT=rocprim) as the cross-check
+    DBuf<char> sort_tmp;            // radix sort ping-pong buffers and pass state
+    DBuf<int> sort_err;             // sticky look-back failure word
     // contact classification workspaces
     DBuf<int> cls_buf, cls_flag, cls_scan, cls_pos;
 };
@@ -250,6 +253,20 @@ void prof_release(Ctx& c);
 void prof_drop_tail(Ctx& c, int slot, size_t keep);
 
 void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s);
+// Stable radix sort of (key, 32-bit value) pairs on the low `bits` bits of
+// the keys (radix.hip).  vin == nullptr: the values are id_base + i.  kin
+// and kout must differ; values of any 32-bit type move bit for bit.  A
+// failed look-back (never expected) sets *err, or the context's sticky
+// sort error word when err is null.
+enum { SORT_OWN = 0, SORT_ROCPRIM = 1 };
+void radix_sort_pairs(Ctx& c, const unsigned long long* kin, unsigned long long* kout, const int* vin, int id_base,
+                      int* vout, long long n, int bits, hipStream_t s, int* err = nullptr);
+void radix_sort_pairs(Ctx& c, const unsigned* kin, unsigned* kout, const int* vin, int id_base, int* vout,
+                      long long n, int bits, hipStream_t s, int* err = nullptr);
+int radix_sort_error(Ctx& c);  // the sticky word (synchronises)
+// one sort by the library's radix sort or by rocPRIM's (the cross-check); synchronises
+void sort_pairs_dispatch(Ctx& c, const void* kin, void* kout, const int* vin, int* vout, long long n, int key_bytes,
+                         int bits, int impl);
 
 // component entry points (implemented in the .hip files)
 void fem_set_mesh(Ctx& c, int n, int T, const uint32_t* h_tets);

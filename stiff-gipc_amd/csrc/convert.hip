@@ -287,18 +287,26 @@ static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* start
     int* beg = c.seg_beg.reserve((size_t)nbr + 1);
     int* fin = c.seg_fin.reserve((size_t)nbr + 1);
     int* cnt = c.row_count.reserve((size_t)nbr + 1);
+    const bool own = c.sort_impl == SORT_OWN;
     rocprim::counting_iterator<int> ids((int)t0);
-    size_t b1 = 0, b2 = 0;
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, b1, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
-    SGIPC_HIP(rocprim::exclusive_scan(nullptr, b2, head, hscan, 0, (size_t)n + 1, rocprim::plus<int>(), s));
-    size_t bytes = std::max(b1, b2);
-    char* tmp = c.seg_tmp.reserve(bytes + 1);
+    size_t bytes = 0;
+    if (!own) {  // the library's sort and scan, or rocPRIM's as the cross-check
+        size_t b1 = 0, b2 = 0;
+        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, b1, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
+        SGIPC_HIP(rocprim::exclusive_scan(nullptr, b2, head, hscan, 0, (size_t)n + 1, rocprim::plus<int>(), s));
+        bytes = std::max(b1, b2);
+    }
+    char* tmp = own ? nullptr : c.seg_tmp.reserve(bytes + 1);
     if (n > 0) {
         hipLaunchKernelGGL(k_seg_keys, dim3(div_up(n, 256)), dim3(256), 0, s, t0, n, c.t_rows, c.t_cols, nbr, bc, key,
                            err_dev);
         SGIPC_LAUNCH_CHECK();
-        size_t b = bytes;
-        SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, b, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
+        if (own) {
+            radix_sort_pairs(c, key, key2, nullptr, (int)t0, sorted, n, 2 * bc, s);
+        } else {
+            size_t b = bytes;
+            SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, b, key, key2, ids, sorted, (size_t)n, 0, 2 * bc, s));
+        }
     }
     fill<int>(beg, -1, nbr, s);
     if (n > 0) {
@@ -313,7 +321,9 @@ static void sort_segment(Ctx& c, long long t0, long long t1, int nbr, int* start
     hipLaunchKernelGGL(k_key_heads, dim3(div_up(n + 1, 256)), dim3(256), 0, s, n, (const unsigned long long*)key2, bc,
                        nbr, head);
     SGIPC_LAUNCH_CHECK();
-    {
+    if (own) {
+        exclusive_scan(c, head, hscan, (int)(n + 1), nullptr, s);
+    } else {
         size_t b = bytes;
         SGIPC_HIP(rocprim::exclusive_scan((void*)tmp, b, head, hscan, 0, (size_t)n + 1, rocprim::plus<int>(), s));
     }
@@ -355,9 +365,11 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
         int* start = c.row_start.reserve((size_t)nbr + 2);
         int* sorted = c.bcoo_row.reserve((size_t)nt + 1);
         sort_segment(c, 0, nt, nbr, start, sorted, ustart, nullptr, nullptr, dint + 2, dint);
-        int hk[4] = {0, 0, 0, 0};
+        int hk[4] = {0, 0, 0, 0}, serr = 0;
         SGIPC_HIP(hipMemcpyAsync(hk, dint, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+        if (c.sort_err.p) SGIPC_HIP(hipMemcpyAsync(&serr, c.sort_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
         SGIPC_HIP(hipStreamSynchronize(s));
+        if (serr) throw std::runtime_error("convert: radix sort look-back failed");
         if (hk[2]) throw arg_error("convert: triplet row/col index out of range [0, n_block_rows)");
         const int K = hk[0];
         int* orow = c.merge_row.reserve((size_t)K + 1);
@@ -390,9 +402,11 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
     int* ducol = c.dyn_ucol.reserve((size_t)s0 + 1);
     int* durun = c.dyn_urun.reserve((size_t)s0 + 1);
     sort_segment(c, 0, s0, nbr, dstart, dsorted, dus, ducol, durun, dint + 2, dint);
-    int hk[8] = {0};
+    int hk[8] = {0}, serr = 0;
     SGIPC_HIP(hipMemcpyAsync(hk, dint, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
+    if (c.sort_err.p) SGIPC_HIP(hipMemcpyAsync(&serr, c.sort_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
+    if (serr) throw std::runtime_error("convert: radix sort look-back failed");
     if (hk[2]) throw arg_error("convert: triplet row/col index out of range [0, n_block_rows)");
     if (!candidate || hk[5]) {
         // (re)build the plan of the static segment

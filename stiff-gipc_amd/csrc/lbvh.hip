@@ -298,10 +298,14 @@ static void lbvh_build(Ctx& c, LbvhTree& t, int m, const unsigned* prim, const d
     hipLaunchKernelGGL(k_morton, dim3(div_up(m, 256)), dim3(256), 0, s, m, (const Box*)leaf,
                        (const Box*)(part + gb), code, idx);
     SGIPC_LAUNCH_CHECK();
-    size_t bytes = 0;
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, code, code + m, idx, idx + m, (unsigned)m, 0, 30, s));
-    void* tmp = t.sort_tmp.reserve(bytes + 1);
-    SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, code, code + m, idx, idx + m, (unsigned)m, 0, 30, s));
+    if (c.sort_impl == SORT_OWN) {
+        radix_sort_pairs(c, code, code + m, (const int*)idx, 0, (int*)(idx + m), m, 30, s);
+    } else {  // rocPRIM as the cross-check
+        size_t bytes = 0;
+        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, code, code + m, idx, idx + m, (unsigned)m, 0, 30, s));
+        void* tmp = t.sort_tmp.reserve(bytes + 1);
+        SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, code, code + m, idx, idx + m, (unsigned)m, 0, 30, s));
+    }
     Node* nodes = (Node*)t.nodes.reserve((size_t)(2 * m - 1) * sizeof(Node));
     Box* nbox = (Box*)t.nbox.reserve((size_t)(2 * m - 1) * sizeof(Box));
     unsigned long long* key = t.key.reserve((size_t)m);

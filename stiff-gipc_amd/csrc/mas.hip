@@ -1416,10 +1416,16 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
                                (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, B, L,
                                (const int*)m.node_of.p, m.bank_upper.p, sentinel, key, rec);
             SGIPC_LAUNCH_CHECK();
-            size_t bytes = 0;
-            SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
-            void* tmp = m.proj_sort_tmp.reserve(bytes + 1);
-            SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0, bits, s));
+            if (c.sort_impl == SORT_OWN) {
+                radix_sort_pairs(c, key, key + mrec, (const int*)rec, 0, (int*)(rec + mrec), mrec, bits, s);
+            } else {  // rocPRIM as the cross-check
+                size_t bytes = 0;
+                SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0,
+                                                    bits, s));
+                void* tmp = m.proj_sort_tmp.reserve(bytes + 1);
+                SGIPC_HIP(rocprim::radix_sort_pairs(tmp, bytes, key, key + mrec, rec, rec + mrec, (size_t)mrec, 0,
+                                                    bits, s));
+            }
             double* contrib = m.proj_contrib.reserve(9 * (size_t)mrec);
             const unsigned* skey = key + mrec;
             hipLaunchKernelGGL(k_proj_contrib, dim3(div_up(mrec, 256)), dim3(256), 0, s, mrec, skey,

@@ -92,6 +92,23 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_apply(const int* __restri
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
 }
 
+// one level: tile sums of in, their exclusive scan (one block when they fit
+// one tile, else the same three passes over them, a level deeper), rescan
+static void scan_level(int* scratch, const int* in, int* out, int n, int* total_dev, hipStream_t s) {
+    const int ntiles = div_up(n, kScanTile);
+    int* tiles = scratch;
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles);
+    SGIPC_LAUNCH_CHECK();
+    if (ntiles <= kScanTile) {
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total_dev);
+        SGIPC_LAUNCH_CHECK();
+    } else {
+        scan_level(scratch + ntiles + 1, tiles, tiles, ntiles, total_dev, s);  // in place (aliasing allowed)
+    }
+    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles, out);
+    SGIPC_LAUNCH_CHECK();
+}
+
 // out[0..n] (n+1 entries), total_dev optional.  in and out may alias.
 void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipStream_t s) {
     if (n <= 0) {
@@ -99,15 +116,16 @@ void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipS
         if (total_dev) fill<int>(total_dev, 0, 1, s);
         return;
     }
-    int ntiles = div_up(n, kScanTile);
-    if (ntiles > kScanTile) throw arg_error("exclusive_scan: input too large");
-    int* tiles = c.scan_tmp.reserve((size_t)ntiles + 1);
-    hipLaunchKernelGGL(k_tile_sums, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total_dev);
-    SGIPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, s, in, n, tiles, out);
-    SGIPC_LAUNCH_CHECK();
+    // scratch: the tile sums of every level (each level's scan writes one
+    // extra entry, its total)
+    size_t need = 0;
+    for (long long m = n; m > 0;) {
+        const long long t = (m + kScanTile - 1) / kScanTile;
+        need += (size_t)t + 1;
+        if (t <= kScanTile) break;
+        m = t;
+    }
+    scan_level(c.scan_tmp.reserve(need + 1), in, out, n, total_dev, s);
 }
 
 static hipEvent_t pool_get(Ctx& c) {

@@ -155,13 +155,17 @@ void bsr_build(Ctx& c) {
         unsigned* key = S.lkey.reserve((size_t)K + 1);
         unsigned* key2 = S.lkey2.reserve((size_t)K + 1);
         int* kk = S.lidx.reserve((size_t)K + 1);
-        rocprim::counting_iterator<int> ids(0);
-        size_t bytes = 0;
-        SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
-        char* tmp = S.ltmp.reserve(bytes + 1);
         hipLaunchKernelGGL(k_low_keys, dim3(div_up(K, 256)), dim3(256), 0, s, K, c.t_rows, c.t_cols, nb, key);
         SGIPC_LAUNCH_CHECK();
-        SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
+        if (c.sort_impl == SORT_OWN) {
+            radix_sort_pairs(c, key, key2, nullptr, 0, kk, K, bc, s);
+        } else {  // rocPRIM as the cross-check
+            rocprim::counting_iterator<int> ids(0);
+            size_t bytes = 0;
+            SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
+            char* tmp = S.ltmp.reserve(bytes + 1);
+            SGIPC_HIP(rocprim::radix_sort_pairs((void*)tmp, bytes, key, key2, ids, kk, (size_t)K, 0, bc, s));
+        }
         hipLaunchKernelGGL(k_low_fill, dim3(div_up(K, 256)), dim3(256), 0, s, K, (const int*)(dint + 4),
                            (const int*)kk, c.t_rows, lref);
         SGIPC_LAUNCH_CHECK();

@@ -15,6 +15,7 @@ PRECOND_NONE, PRECOND_DIAG, PRECOND_MAS, PRECOND_MAS_FP32 = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 STOP_REF_RZ, STOP_TRUE_REL = 0, 1
 BANK_INVERSE_VALU64, BANK_INVERSE_MFMA64, BANK_INVERSE_MFMA32_REFINED = 0, 1, 2
+SORT_OWN, SORT_ROCPRIM = 0, 1
 PROF_SPMV, PROF_MAS_SOLVE, PROF_FEM, PROF_CONVERT, PROF_PRECOND_BUILD, PROF_SOLVE, PROF_BARRIER = range(7)
 
 
@@ -82,6 +83,8 @@ SIGNATURES = {
     "sgipc_autotune": ([P, P, I, P, I, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_set_schedule": ([P, I, I], I),
     "sgipc_set_batch_mode": ([P, I, I, I], I),
+    "sgipc_set_sort": ([P, I], I),
+    "sgipc_sort_pairs": ([P, P, P, P, P, LL, I, I, I], I),
     "sgipc_get_batch_mode": ([P, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_fem_param_adjoint": ([P, I, P, P, P, P, D, P, P, P, P], I),
     "sgipc_classify_contacts": ([P, I, P, I, P, P, P, D, P, P, I, C.POINTER(I), P], I),
