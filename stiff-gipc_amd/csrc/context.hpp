@@ -347,7 +347,8 @@ void mas_bank_solve(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip = SceneSk
 // instead of its own launch before it (the x/r update then restricts levels
 // 0..2 only: mas_update_restrict(..., top = false))
 bool mas_fused(const Ctx& c);
-void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl);
+void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip = SceneSkip{});
+
 // PCG x/r update fused with the restriction of the new residual (levels
 // 0..L-1); g1 + gx workgroups, each writing one r.r partial
 void mas_update_layout(const Ctx& c, int& g1, int& gx);

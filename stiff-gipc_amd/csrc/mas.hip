@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__(kCT) k_mas_solve_top(int L, const int* __restr
                                                      const int* __restrict__ going_next,
                                                      const unsigned* __restrict__ comp_mask, int top_bank0,
                                                      int nbanks, const T* __restrict__ inv, T* __restrict__ R,
-                                                     T* __restrict__ Z, const int* __restrict__ ctrl) {
+                                                     T* __restrict__ Z, const int* __restrict__ ctrl, SceneSkip skip) {
     if (ctrl[C_STOP]) return;
     constexpr int D = 48;
     using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
@@ -1184,13 +1184,14 @@ __global__ void __launch_bounds__(kCT) k_mas_solve_top(int L, const int* __restr
         __threadfence_block();
         __syncthreads();
         for (int b = top_bank0 + wv; b < nbanks; b += kCW) {
+            if (skip.stopped(b)) continue;
             const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
             if (lane < D) Z[(size_t)b * D + lane] = zb;
         }
         return;
     }
     const int b = (blockIdx.x - 1) * kCW + wv;  // levels 0 .. 2
-    if (b < top_bank0) {
+    if (b < top_bank0 && !skip.stopped(b)) {
         const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
         if (lane < D) Z[(size_t)b * D + lane] = zb;
     }
@@ -1593,24 +1594,24 @@ void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const floa
 }
 
 template <typename T>
-static void solve_top_t(Ctx& c, const T* inv, T* R, T* Z, const int* ctrl) {
+static void solve_top_t(Ctx& c, const T* inv, T* R, T* Z, const int* ctrl, SceneSkip skip) {
     MasState& m = c.mas;
     const int top_bank0 = m.L >= 4 ? m.level_size_h[7] / m.bank : m.nbanks;
     prof_begin(c, PROF_MAS_SOLVE);
     hipLaunchKernelGGL((k_mas_solve_top<T>), dim3(1 + div_up(top_bank0, kCW)), dim3(kCT), 0, c.stream, m.L,
                        (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
-                       top_bank0, m.nbanks, inv, R, Z, ctrl);
+                       top_bank0, m.nbanks, inv, R, Z, ctrl, skip);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_MAS_SOLVE);
 }
 
 bool mas_fused(const Ctx& c) { return c.mas_fused && c.mas.bank == 16 && c.mas.ready; }
 
-void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl) {
-    if (fp32) solve_top_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl);
+void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip) {
+    if (fp32) solve_top_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl, skip);
     else {
         if (!c.mas.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
-        solve_top_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl);
+        solve_top_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl, skip);
     }
 }
 

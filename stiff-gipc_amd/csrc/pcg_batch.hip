@@ -70,6 +70,48 @@ __global__ void __launch_bounds__(1024) k_bspmv(const int* __restrict__ chunk_ro
     write_partial(contrib, part, sh);
 }
 
+// The same over the row-contiguous FP32 copy of both triangles (mixed
+// solves): the G lanes of a row stream consecutive entries; same entry
+// order and products as k_bspmv, so the same sums.
+template <int G>
+__global__ void __launch_bounds__(1024) k_bspmv_full(const int* __restrict__ chunk_row,
+                                                     const int* __restrict__ chunk_scene, const int* __restrict__ eptr,
+                                                     const int* __restrict__ colf, const float* __restrict__ vf,
+                                                     const float* __restrict__ p, float* __restrict__ Ap,
+                                                     const int* __restrict__ sstop, double* __restrict__ part,
+                                                     const int* __restrict__ ctrl) {
+    if (ctrl[C_STOP]) return;
+    __shared__ double sh[1024 / 64];
+    const int ch = blockIdx.x;
+    if (sstop[chunk_scene[ch]]) {
+        if (threadIdx.x == 0) part[ch] = 0.0;
+        return;
+    }
+    const int r0 = chunk_row[ch], r1 = chunk_row[ch + 1];
+    const int row = r0 + threadIdx.x / G, sub = threadIdx.x % G;
+    float s0 = 0, s1 = 0, s2 = 0;
+    if (row < r1) {
+        const int end = eptr[row + 1];
+        int e = eptr[row] + sub;
+        for (; e + G < end; e += 2 * G) {  // two entries in flight
+            const FullEntry fa = load_full_entry(vf, e), fb = load_full_entry(vf, e + G);
+            const f3_a4 xa = load_z3(p, colf[e]), xb = load_z3(p, colf[e + G]);
+            full_madd(fa, xa, s0, s1, s2);
+            full_madd(fb, xb, s0, s1, s2);
+        }
+        if (e < end) full_madd(load_full_entry(vf, e), load_z3(p, colf[e]), s0, s1, s2);
+    }
+    group_sum3<G, float>(s0, s1, s2);
+    double contrib = 0.0;
+    if (sub < 3 && row < r1) {
+        const size_t o = 3 * (size_t)row + sub;
+        const float a = sub == 0 ? s0 : (sub == 1 ? s1 : s2);
+        Ap[o] = a;
+        contrib = (double)p[o] * (double)a;
+    }
+    write_partial(contrib, part, sh);
+}
+
 // x += alpha_s p ; r -= alpha_s Ap ; partial r.r per chunk
 template <typename V>
 __global__ void __launch_bounds__(kBT) k_bupdate(const int* __restrict__ chunk_row, const int* __restrict__ chunk_scene,
@@ -305,6 +347,8 @@ struct BEngine {
     const MV* val;
     bool fp32mas;
     int rule;
+    const float* vfull = nullptr;  // mixed: row-contiguous FP32 copy of both triangles (else the upper BCOO)
+    const int* colfull = nullptr;
 
     BScal scal() {
         return BScal{w.bb.p, w.rz.p, w.rz0.p, w.alpha.p, w.beta.p, w.tol.p, w.segout.p, w.stop.p, w.iters.p,
@@ -322,8 +366,10 @@ struct BEngine {
     void precond() {
         // z = M^-1 r over the union, then r.z partials per chunk
         const SceneSkip skip{w.bank_scene.p, w.stop.p};
-        mas_restrict(c, r, 0, fp32mas, w.ctrl.p, true, skip);
-        mas_bank_solve(c, fp32mas, w.ctrl.p, skip);
+        const bool fused = mas_fused(c);  // top-level restriction inside the bank-solve launch
+        mas_restrict(c, r, 0, fp32mas, w.ctrl.p, !fused, skip);
+        if (fused) mas_solve_with_top(c, fp32mas, w.ctrl.p, skip);
+        else mas_bank_solve(c, fp32mas, w.ctrl.p, skip);
         hipLaunchKernelGGL((k_bcollect<V, T>), dim3(nch), dim3(kBT), 0, c.stream, (const int*)w.chunk_row.p,
                            (const int*)w.chunk_scene.p, (const V*)r, z, c.mas.n, c.mas.L, (const int*)c.mas.node_of.p,
                            (const T*)Zm(), (const int*)w.stop.p, w.part_rz.p, (const int*)w.ctrl.p);
@@ -338,9 +384,24 @@ struct BEngine {
                                (const int*)w.chunk_scene.p, (const int*)B.eptr.p, (const int2*)B.ent.p, val,
                                (const V*)p, Ap, (const int*)w.stop.p, w.part_pap.p, (const int*)w.ctrl.p);
         };
-        if (G == 4) launch(k_bspmv<4, MV, V>);
-        else if (G == 16) launch(k_bspmv<16, MV, V>);
-        else launch(k_bspmv<8, MV, V>);
+        if constexpr (sizeof(MV) == 4 && sizeof(V) == 4) {
+            if (vfull) {
+                auto launchf = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, dim3(nch), dim3(1024), 0, s, (const int*)w.chunk_row.p,
+                                       (const int*)w.chunk_scene.p, (const int*)B.eptr.p, colfull, vfull,
+                                       (const float*)p, (float*)Ap, (const int*)w.stop.p, w.part_pap.p,
+                                       (const int*)w.ctrl.p);
+                };
+                if (G == 4) launchf(k_bspmv_full<4>);
+                else if (G == 16) launchf(k_bspmv_full<16>);
+                else launchf(k_bspmv_full<8>);
+            }
+        }
+        if (!vfull) {
+            if (G == 4) launch(k_bspmv<4, MV, V>);
+            else if (G == 16) launch(k_bspmv<16, MV, V>);
+            else launch(k_bspmv<8, MV, V>);
+        }
         SGIPC_LAUNCH_CHECK();
         scalars(B_ALPHA, w.part_pap.p, nullptr);
         hipLaunchKernelGGL((k_bupdate<V>), dim3(nch), dim3(kBT), 0, s, (const int*)w.chunk_row.p,
@@ -381,6 +442,7 @@ struct BEngine {
         mix((size_t)N); mix((size_t)S); mix((size_t)nch); mix((size_t)G); mix((size_t)per_graph);
         mix((size_t)rule); mix((size_t)r); mix((size_t)z); mix((size_t)p); mix((size_t)Ap); mix((size_t)x);
         mix((size_t)val); mix((size_t)fp32mas); mix((size_t)B.eptr.p); mix((size_t)B.ent.p); mix((size_t)B.nb);
+        mix((size_t)vfull); mix((size_t)colfull); mix((size_t)mas_fused(c));
         for (const void* q : {(const void*)w.chunk_row.p, (const void*)w.chunk_scene.p, (const void*)w.scene_chunk.p,
                               (const void*)w.part_pap.p, (const void*)w.part_rr.p, (const void*)w.part_rz.p,
                               (const void*)w.bb.p, (const void*)w.rz.p, (const void*)w.rz0.p, (const void*)w.alpha.p,
@@ -554,8 +616,14 @@ void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, d
         float* rf = pw.rf.reserve(N + 1);
         double* r64 = pw.r64b.reserve(N + 1);
         double* d64 = pw.d64.reserve(N + 1);
+        // the SpMV reads the row-contiguous FP32 copy of both triangles, or
+        // (SGIPC_SPMV_FULL32=0) the FP32 copy of the upper BCOO; only one is built
+        const int* colfull = nullptr;
+        const float* vfull = c.spmv_full32 ? bsr_full_f32(c, &colfull) : nullptr;
         BEngine<float, float, float> E{c, w, N, S, w.nch, G, rf, pw.zf.reserve(N + 1), pw.pf.reserve(N + 1),
-                                      pw.Apf.reserve(N + 1), d64, bsr_values_f32(c), true, 1};
+                                      pw.Apf.reserve(N + 1), d64, vfull ? nullptr : bsr_values_f32(c), true, 1};
+        E.vfull = vfull;
+        E.colfull = colfull;
         fill<double>(x, 0.0, N, s);
         hipLaunchKernelGGL((k_bcast<double, double>), dim3(flat_grid(N)), dim3(256), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();

@@ -397,22 +397,43 @@ __global__ void __launch_bounds__(NTH) k_pcg_persist(PArgs<V, MV, T> a) {
 
 // node range [lo, hi) of every scene at every MAS level, and the scene of
 // every node (a node's vertices are connected, so they share one scene)
-__global__ void k_scene_levels(int n, int L, int S, const int* __restrict__ srow, const int* __restrict__ node_of,
-                               int* __restrict__ lo, int* __restrict__ hi, int* __restrict__ node_scene) {
+__global__ void __launch_bounds__(256) k_scene_levels(int n, int L, int S, const int* __restrict__ srow,
+                                                       const int* __restrict__ node_of, int* __restrict__ lo,
+                                                       int* __restrict__ hi, int* __restrict__ node_scene) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    int a = 0, b = S;  // largest s with srow[s] <= v (srow[S] = n > v)
-    while (b - a > 1) {
-        const int m = (a + b) >> 1;
-        if (srow[m] <= v) a = m; else b = m;
+    const bool valid = v < n;  // no early exit: every lane takes part in the wavefront reductions
+    int a = -1;
+    if (valid) {
+        int a1 = 0, b = S;  // largest s with srow[s] <= v (srow[S] = n > v)
+        while (b - a1 > 1) {
+            const int m = (a1 + b) >> 1;
+            if (srow[m] <= v) a1 = m; else b = m;
+        }
+        a = a1;
     }
+    // a wavefront's vertices are consecutive, so they nearly always share one
+    // scene: then lane 0 (the lowest vertex) updates the range with the
+    // wavefront's min / max, else every lane updates it itself
+    const int a0 = __shfl(a, 0, 64);
+    const bool uniform = __all(!valid || a == a0);
     for (int l = 0; l < L; ++l) {
-        const int g = node_of[(size_t)l * n + v];
-        if (lo) {
+        const int g = valid ? node_of[(size_t)l * n + v] : 0;
+        if (valid) node_scene[g] = a;
+        if (!lo) continue;
+        if (uniform) {
+            int gmin = valid ? g : 0x7fffffff, gmax = valid ? g : -1;
+            for (int off = 32; off > 0; off >>= 1) {
+                gmin = min(gmin, __shfl_xor(gmin, off, 64));
+                gmax = max(gmax, __shfl_xor(gmax, off, 64));
+            }
+            if ((threadIdx.x & 63) == 0 && a0 >= 0) {
+                atomicMin(lo + (size_t)a0 * L + l, gmin);
+                atomicMax(hi + (size_t)a0 * L + l, gmax + 1);
+            }
+        } else if (valid) {
             atomicMin(lo + (size_t)a * L + l, g);
             atomicMax(hi + (size_t)a * L + l, g + 1);
         }
-        node_scene[g] = a;
     }
 }
 

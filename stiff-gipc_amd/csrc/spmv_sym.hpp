@@ -119,6 +119,42 @@ __device__ inline void sym_row_lane(int row, int sub, const int* __restrict__ ep
     }
 }
 
+// Row-contiguous FP32 copy of both triangles (mixed precision): entry e is
+// its block (transposed where it is a transposed reference), 10 floats, read
+// as two 16-byte loads at 8-byte alignment and one 8-byte load.
+typedef float f4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef float f3_a4 __attribute__((ext_vector_type(3), aligned(4)));
+
+struct FullEntry {
+    f4_a8 lo, hi;  // column-major elements 0-3, 4-7
+    float a8;
+};
+__device__ inline FullEntry load_full_entry(const float* __restrict__ vf, int e) {
+    const float* A = vf + kF32FullStride * (size_t)e;
+    FullEntry f;
+    f.lo = *reinterpret_cast<const f4_a8*>(A);
+    f.hi = *reinterpret_cast<const f4_a8*>(A + 4);
+    f.a8 = reinterpret_cast<const float2*>(A + 8)->x;
+    return f;
+}
+template <typename V>
+__device__ inline f3_a4 load_z3(const V* __restrict__ z, int c) {
+    if constexpr (sizeof(V) == 4) {
+        return *reinterpret_cast<const f3_a4*>(z + 3 * (size_t)c);
+    } else {
+        f3_a4 r;
+        r.x = (float)z[3 * (size_t)c];
+        r.y = (float)z[3 * (size_t)c + 1];
+        r.z = (float)z[3 * (size_t)c + 2];
+        return r;
+    }
+}
+__device__ inline void full_madd(const FullEntry& f, const f3_a4& x, float& s0, float& s1, float& s2) {
+    s0 += f.lo.x * x.x + f.lo.w * x.y + f.hi.z * x.z;
+    s1 += f.lo.y * x.x + f.hi.x * x.y + f.hi.w * x.z;
+    s2 += f.lo.z * x.x + f.hi.y * x.y + f.a8 * x.z;
+}
+
 template <int G, typename TV>
 __device__ inline void group_sum3(TV& s0, TV& s1, TV& s2) {
 #pragma unroll
