@@ -4,7 +4,7 @@ residual ||b - A dx|| / ||b|| recomputed here with the library's FP64 SpMV
 of the converted system (not the solver's own report), the converted BCOO
 sorted, unique and upper, the mixed and FP32-apply directions against the
 FP64 one, every batched scene to its own tolerance, and the cantilever
-adjoint against fourth-order central differences.
+adjoint against Ridders-extrapolated central differences.
 
 Workloads (the Newton-step path of StiffGIPC/GIPC.cu:10110-10344 ->
 linear_system/linear_system/global_linear_system.cu:124-133):
