@@ -27,7 +27,30 @@ __device__ inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a
 __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 __device__ inline void put(double* g, int slot, V3 v) { g[3 * slot] = v.x; g[3 * slot + 1] = v.y; g[3 * slot + 2] = v.z; }
-__device__ inline void add(double* g, int slot, V3 v) { g[3 * slot] += v.x; g[3 * slot + 1] += v.y; g[3 * slot + 2] += v.z; }
+// element s (0..3, known only at run time) of a four-element register array,
+// by selects, so the array stays in registers (an indexed access would put it
+// in scratch memory)
+__device__ inline V3 pick4(const V3 (&a)[4], int s) {
+    V3 r = a[0];
+    r = s == 1 ? a[1] : r;
+    r = s == 2 ? a[2] : r;
+    r = s == 3 ? a[3] : r;
+    return r;
+}
+__device__ inline int pick4(const int (&a)[4], int s) {
+    int r = a[0];
+    r = s == 1 ? a[1] : r;
+    r = s == 2 ? a[2] : r;
+    r = s == 3 ? a[3] : r;
+    return r;
+}
+// g[3 s .. 3 s + 2] += v for a run-time slot s of a register-resident g
+__device__ inline void add_at(double (&g)[12], int s, V3 v) {
+    g[0] += s == 0 ? v.x : 0.0; g[1] += s == 0 ? v.y : 0.0; g[2] += s == 0 ? v.z : 0.0;
+    g[3] += s == 1 ? v.x : 0.0; g[4] += s == 1 ? v.y : 0.0; g[5] += s == 1 ? v.z : 0.0;
+    g[6] += s == 2 ? v.x : 0.0; g[7] += s == 2 ? v.y : 0.0; g[8] += s == 2 ? v.z : 0.0;
+    g[9] += s == 3 ? v.x : 0.0; g[10] += s == 3 ? v.y : 0.0; g[11] += s == 3 ? v.z : 0.0;
+}
 
 // squared distances with gradients (slots in the decoded vertex order)
 __device__ inline double d_pp(V3 a, V3 b, double* g) {
@@ -111,17 +134,17 @@ __device__ inline Pair decode(int4 q) {
     return P;
 }
 
-// Hessian entry (i, j) of the per-pair form; evaluated on the fly so the
+// The per-pair Hessian form: entry (i, j) = gc_i (m00 gc_j + m01 gs_j) +
+// gs_i (m01 gc_j + m11 gs_j), evaluated where the blocks are written, so the
 // 12x12 matrix never lives in registers.
 struct HForm {
     double gs[12], gc[12];
     double m00, m01, m11;  // moll: 2x2 PSD; non-moll: m11 = lambda0, others 0
 };
-__device__ inline double hent(const HForm& F, int i, int j) {
-    return F.gc[i] * (F.m00 * F.gc[j] + F.m01 * F.gs[j]) + F.gs[i] * (F.m01 * F.gc[j] + F.m11 * F.gs[j]);
-}
 
-__global__ void __launch_bounds__(128) k_barrier(int np, const int4* __restrict__ pairs, const int* __restrict__ mat_index,
+constexpr int kBarT = 128;  // pairs (threads) per k_barrier workgroup
+
+__global__ void __launch_bounds__(kBarT) k_barrier(int np, const int4* __restrict__ pairs, const int* __restrict__ mat_index,
                                                  int c2, int c3, int c4, const double* __restrict__ x,
                                                  const double* __restrict__ xr, double dhat, double kappa,
                                                  const int* __restrict__ btype, int index_offset,
@@ -129,120 +152,160 @@ __global__ void __launch_bounds__(128) k_barrier(int np, const int4* __restrict_
                                                  int* __restrict__ tr, int* __restrict__ tc,
                                                  double* __restrict__ gscr, unsigned* __restrict__ gkey,
                                                  unsigned* __restrict__ grec, unsigned gsentinel) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    Pair P = decode(pairs[p]);
-    const int nd = 3 * P.nv;
-    V3 X[4];
+    // one thread per pair computes its barrier; the workgroup then writes the
+    // pairs' triplet blocks together (below)
+    __shared__ double sF[27][kBarT];  // per pair: gs[12], gc[12], m00, m01, m11
+    __shared__ int sV[4][kBarT];      // vertices
+    __shared__ int sInfo[kBarT];      // nv | pinned-vertex mask << 4 | active << 8; 0: no pair
+    __shared__ long long sOff[kBarT];  // first triplet of the pair
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < np;
+    if (valid) {
+        Pair P = decode(pairs[p]);
+        V3 X[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) X[a] = (a < P.nv) ? ld(x, P.v[a]) : V3{0, 0, 0};
-    double gd[12];
+        for (int a = 0; a < 4; ++a) X[a] = (a < P.nv) ? ld(x, P.v[a]) : V3{0, 0, 0};
+        double gd[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) gd[k] = 0.0;
-    double d;
-    if (P.dkind == 0) d = d_pp(X[0], X[1], gd);
-    else if (P.dkind == 1) d = d_pe(X[0], X[1], X[2], gd);
-    else if (P.dkind == 2) d = d_pt(X[0], X[1], X[2], X[3], gd);
-    else d = d_ee(X[0], X[1], X[2], X[3], gd);
-    const double k2 = kappa * dhat * dhat;
-    double I = d / dhat, s = sqrt(I);
-    double inv = (d > 0.0) ? 1.0 / (2.0 * sqrt(d * dhat)) : 0.0;
-    HForm F;
+        for (int k = 0; k < 12; ++k) gd[k] = 0.0;
+        double d;
+        if (P.dkind == 0) d = d_pp(X[0], X[1], gd);
+        else if (P.dkind == 1) d = d_pe(X[0], X[1], X[2], gd);
+        else if (P.dkind == 2) d = d_pt(X[0], X[1], X[2], X[3], gd);
+        else d = d_ee(X[0], X[1], X[2], X[3], gd);
+        const double k2 = kappa * dhat * dhat;
+        double I = d / dhat, s = sqrt(I);
+        double inv = (d > 0.0) ? 1.0 / (2.0 * sqrt(d * dhat)) : 0.0;
+        HForm F;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) { F.gs[k] = gd[k] * inv; F.gc[k] = 0.0; }
-    double grad[12];
-    bool active = true;
-    if (!P.moll) {
-        double pp = pk1(I, k2) * s;
+        for (int k = 0; k < 12; ++k) { F.gs[k] = gd[k] * inv; F.gc[k] = 0.0; }
+        double grad[12];
+        bool active = true;
+        if (!P.moll) {
+            double pp = pk1(I, k2) * s;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) grad[k] = F.gs[k] * pp;
-        F.m00 = 0.0; F.m01 = 0.0;
-        F.m11 = lam0((d < 1e-6 * dhat) ? 1e-6 : I, k2);
-    } else {
-        V3 eA = X[P.sl[1]] - X[P.sl[0]], eB = X[P.sl[3]] - X[P.sl[2]];
-        V3 v = cross(eA, eB);
-        double cc = sqrt(dot(v, v)), I1 = cc * cc;
-        if (I1 == 0.0) {
-            active = false;
-#pragma unroll
-            for (int k = 0; k < 12; ++k) grad[k] = 0.0;
-            F.m00 = F.m01 = F.m11 = 0.0;
+            for (int k = 0; k < 12; ++k) grad[k] = F.gs[k] * pp;
+            F.m00 = 0.0; F.m01 = 0.0;
+            F.m11 = lam0((d < 1e-6 * dhat) ? 1e-6 : I, k2);
         } else {
-            V3 vh = (1.0 / cc) * v;
-            V3 dA = cross(eB, vh), dB = cross(vh, eA);
-            add(F.gc, P.sl[1], dA); add(F.gc, P.sl[0], -1.0 * dA);
-            add(F.gc, P.sl[3], dB); add(F.gc, P.sl[2], -1.0 * dB);
-            V3 r0 = ld(xr, P.v[P.sl[0]]) - ld(xr, P.v[P.sl[1]]);
-            V3 r1 = ld(xr, P.v[P.sl[2]]) - ld(xr, P.v[P.sl[3]]);
-            double eps = 1e-3 * dot(r0, r0) * dot(r1, r1);
-            double l = log(I), e2 = eps * eps;
-            double p1 = -kappa * 2.0 * (2.0 * dhat * dhat * l * l * (I1 - eps) * (I - 1) * (I - 1)) / e2;
-            double p2 = -kappa * 2.0 * (2.0 * I1 * dhat * dhat * l * (I1 - 2 * eps) * (I - 1) * (I + I * l - 1)) / (I * e2);
+            V3 eA = pick4(X, P.sl[1]) - pick4(X, P.sl[0]), eB = pick4(X, P.sl[3]) - pick4(X, P.sl[2]);
+            V3 v = cross(eA, eB);
+            double cc = sqrt(dot(v, v)), I1 = cc * cc;
+            if (I1 == 0.0) {
+                active = false;
 #pragma unroll
-            for (int k = 0; k < 12; ++k) grad[k] = F.gc[k] * cc * p1 + F.gs[k] * s * p2;
-            double l10 = -kappa * (4.0 * dhat * dhat * l * l * (I - 1) * (I - 1) * (3 * I1 - eps)) / e2;
-            double l20 = kappa *
-                         (4.0 * I1 * dhat * dhat * (I1 - 2 * eps) *
-                          (4 * I + l - 3 * I * I * l * l + 6 * I * l - 2 * I * I + I * l * l - 7 * I * I * l - 2)) /
-                         (I * e2);
-            double lg = -kappa * 4.0 * cc * s * (4.0 * dhat * dhat * l * (I1 - eps) * (I - 1) * (I + I * l - 1)) / (I * e2);
-            double m00 = l10, m01 = lg, m11 = l20;
-            double trc = m00 + m11, det = m00 * m11 - m01 * m01;
-            double disc = sqrt(fmax(0.25 * trc * trc - det, 0.0));
-            double ev0 = 0.5 * trc - disc, ev1 = 0.5 * trc + disc;
-            if (ev0 < 0.0) {
-                double vx, vy;
-                if (fabs(m01) > 0.0) { vx = ev0 - m11; vy = m01; }
-                else if (m00 <= m11) { vx = 1.0; vy = 0.0; }
-                else { vx = 0.0; vy = 1.0; }
-                double nrm = sqrt(vx * vx + vy * vy);
-                vx /= nrm; vy /= nrm;
-                double ux = -vy, uy = vx, pe = ev1 > 0.0 ? ev1 : 0.0;
-                m00 = pe * ux * ux; m01 = pe * ux * uy; m11 = pe * uy * uy;
+                for (int k = 0; k < 12; ++k) grad[k] = 0.0;
+                F.m00 = F.m01 = F.m11 = 0.0;
+            } else {
+                V3 vh = (1.0 / cc) * v;
+                V3 dA = cross(eB, vh), dB = cross(vh, eA);
+                add_at(F.gc, P.sl[1], dA); add_at(F.gc, P.sl[0], -1.0 * dA);
+                add_at(F.gc, P.sl[3], dB); add_at(F.gc, P.sl[2], -1.0 * dB);
+                V3 r0 = ld(xr, pick4(P.v, P.sl[0])) - ld(xr, pick4(P.v, P.sl[1]));
+                V3 r1 = ld(xr, pick4(P.v, P.sl[2])) - ld(xr, pick4(P.v, P.sl[3]));
+                double eps = 1e-3 * dot(r0, r0) * dot(r1, r1);
+                double l = log(I), e2 = eps * eps;
+                double p1 = -kappa * 2.0 * (2.0 * dhat * dhat * l * l * (I1 - eps) * (I - 1) * (I - 1)) / e2;
+                double p2 = -kappa * 2.0 * (2.0 * I1 * dhat * dhat * l * (I1 - 2 * eps) * (I - 1) * (I + I * l - 1)) / (I * e2);
+#pragma unroll
+                for (int k = 0; k < 12; ++k) grad[k] = F.gc[k] * cc * p1 + F.gs[k] * s * p2;
+                double l10 = -kappa * (4.0 * dhat * dhat * l * l * (I - 1) * (I - 1) * (3 * I1 - eps)) / e2;
+                double l20 = kappa *
+                             (4.0 * I1 * dhat * dhat * (I1 - 2 * eps) *
+                              (4 * I + l - 3 * I * I * l * l + 6 * I * l - 2 * I * I + I * l * l - 7 * I * I * l - 2)) /
+                             (I * e2);
+                double lg = -kappa * 4.0 * cc * s * (4.0 * dhat * dhat * l * (I1 - eps) * (I - 1) * (I + I * l - 1)) / (I * e2);
+                double m00 = l10, m01 = lg, m11 = l20;
+                double trc = m00 + m11, det = m00 * m11 - m01 * m01;
+                double disc = sqrt(fmax(0.25 * trc * trc - det, 0.0));
+                double ev0 = 0.5 * trc - disc, ev1 = 0.5 * trc + disc;
+                if (ev0 < 0.0) {
+                    double vx, vy;
+                    if (fabs(m01) > 0.0) { vx = ev0 - m11; vy = m01; }
+                    else if (m00 <= m11) { vx = 1.0; vy = 0.0; }
+                    else { vx = 0.0; vy = 1.0; }
+                    double nrm = sqrt(vx * vx + vy * vy);
+                    vx /= nrm; vy /= nrm;
+                    double ux = -vy, uy = vx, pe = ev1 > 0.0 ? ev1 : 0.0;
+                    m00 = pe * ux * ux; m01 = pe * ux * uy; m11 = pe * uy * uy;
+                }
+                F.m00 = m00; F.m01 = m01; F.m11 = m11;
             }
-            F.m00 = m00; F.m01 = m01; F.m11 = m11;
         }
+        // gradient: FP64 atomics, as the reference; or, in the ordered mode, one
+        // record per pair vertex for a sorted per-vertex sum (k_grad_gather)
+        if (gscr) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const bool use = active && a < P.nv;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) gscr[12 * (size_t)p + 3 * a + k] = use ? grad[3 * a + k] : 0.0;
+                gkey[4 * (size_t)p + a] = use ? (unsigned)P.v[a] : gsentinel;
+                grec[4 * (size_t)p + a] = 4u * (unsigned)p + (unsigned)a;
+            }
+        } else if (active) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                if (a < P.nv)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) atomicAdd(fb + 3 * (size_t)P.v[a] + k, grad[3 * a + k]);
+        }
+        // triplet block layout: the pair's nv x nv blocks at off
+        long long off;
+        int mi = mat_index[p];
+        if (P.cls == 4) off = 16LL * mi;
+        else if (P.cls == 3) off = 16LL * c4 + 9LL * mi;
+        else off = 16LL * c4 + 9LL * c3 + 4LL * mi;
+        int pinmask = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            if (a < P.nv && btype && btype[P.v[a]] != 0) pinmask |= 1 << a;
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) { sF[k][t] = F.gs[k]; sF[12 + k][t] = F.gc[k]; }
+        sF[24][t] = F.m00; sF[25][t] = F.m01; sF[26][t] = F.m11;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) sV[a][t] = P.v[a];
+        sInfo[t] = P.nv | (pinmask << 4) | ((active ? 1 : 0) << 8);
+        sOff[t] = off;
+    } else {
+        sInfo[threadIdx.x] = 0;
     }
-    // gradient: FP64 atomics, as the reference; or, in the ordered mode, one
-    // record per pair vertex for a sorted per-vertex sum (k_grad_gather)
-    if (gscr) {
+    __syncthreads();
+    // Triplets, written by the workgroup together: work item w = (pair slot
+    // w / 16, block w % 16), so consecutive lanes write consecutive 72-byte
+    // blocks of one pair.  Block kk = ii nv + jj of the nv x nv grid holds
+    // (v_ii, v_jj); blocks below the diagonal, of a pinned vertex or of an
+    // inactive pair are written as zeros (then the upper/boundary remap of
+    // the indices).
+    for (int w = threadIdx.x; w < 16 * kBarT; w += kBarT) {
+        const int slot = w >> 4, kk = w & 15;
+        const int info = sInfo[slot];
+        const int nv = info & 15;
+        if (kk >= nv * nv) continue;
+        const int ii = kk / nv, jj = kk - ii * nv;
+        const int r = sV[ii][slot], c = sV[jj][slot];
+        const bool keep = r <= c;
+        const bool zero = !keep || ((info >> 4) & ((1 << ii) | (1 << jj))) || !((info >> 8) & 1);
+        const long long o = sOff[slot] + kk;
+        double* B = tv + 9 * o;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const bool use = active && a < P.nv;
+        for (int qq = 0; qq < 3; ++qq)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) gscr[12 * (size_t)p + 3 * a + k] = use ? grad[3 * a + k] : 0.0;
-            gkey[4 * (size_t)p + a] = use ? (unsigned)P.v[a] : gsentinel;
-            grec[4 * (size_t)p + a] = 4u * (unsigned)p + (unsigned)a;
-        }
-    } else if (active) {
-        for (int a = 0; a < P.nv; ++a)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) atomicAdd(fb + 3 * (size_t)P.v[a] + k, grad[3 * a + k]);
+            for (int pp = 0; pp < 3; ++pp) {
+                double h = 0.0;
+                if (!zero) {
+                    const int i = 3 * ii + pp, j = 3 * jj + qq;
+                    const double gci = sF[12 + i][slot], gsi = sF[i][slot];
+                    const double gcj = sF[12 + j][slot], gsj = sF[j][slot];
+                    const double m00 = sF[24][slot], m01 = sF[25][slot], m11 = sF[26][slot];
+                    h = gci * (m00 * gcj + m01 * gsj) + gsi * (m01 * gcj + m11 * gsj);
+                }
+                B[pp + 3 * qq] = h;
+            }
+        tr[o] = (keep ? r : c) + index_offset;
+        tc[o] = (keep ? c : r) + index_offset;
     }
-    // triplets: full nv x nv block grid, then the upper/boundary remap
-    long long off;
-    int mi = mat_index[p];
-    if (P.cls == 4) off = 16LL * mi;
-    else if (P.cls == 3) off = 16LL * c4 + 9LL * mi;
-    else off = 16LL * c4 + 9LL * c3 + 4LL * mi;
-    bool pin[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) pin[a] = (a < P.nv && btype) ? (btype[P.v[a]] != 0) : false;
-    int kk = 0;
-    for (int ii = 0; ii < P.nv; ++ii)
-        for (int jj = 0; jj < P.nv; ++jj, ++kk) {
-            int r = P.v[ii], c = P.v[jj];
-            bool keep = r <= c;
-            bool zero = !keep || pin[ii] || pin[jj] || !active;
-            double* B = tv + 9 * (off + kk);
-#pragma unroll
-            for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-                for (int qq = 0; qq < 3; ++qq) B[pp + 3 * qq] = zero ? 0.0 : hent(F, 3 * ii + pp, 3 * jj + qq);
-            tr[off + kk] = (keep ? r : c) + index_offset;
-            tc[off + kk] = (keep ? c : r) + index_offset;
-        }
-    (void)nd;
 }
 
 // ---- narrow phase: classification + encoding ---------------------------
@@ -262,11 +325,11 @@ __device__ double pair_energy(int4 q, const double* __restrict__ x, const double
     else d = d_ee(X[0], X[1], X[2], X[3], nullptr);
     const double I = d / dhat, b = bfun(I, kappa * dhat * dhat);
     if (!P.moll) return b;
-    V3 v = cross(X[P.sl[1]] - X[P.sl[0]], X[P.sl[3]] - X[P.sl[2]]);
+    V3 v = cross(pick4(X, P.sl[1]) - pick4(X, P.sl[0]), pick4(X, P.sl[3]) - pick4(X, P.sl[2]));
     double cc = sqrt(dot(v, v)), I1 = cc * cc;
     if (I1 == 0.0) return 0.0;  // no contribution, as in the reference
-    V3 r0 = ld(xr, P.v[P.sl[0]]) - ld(xr, P.v[P.sl[1]]);
-    V3 r1 = ld(xr, P.v[P.sl[2]]) - ld(xr, P.v[P.sl[3]]);
+    V3 r0 = ld(xr, pick4(P.v, P.sl[0])) - ld(xr, pick4(P.v, P.sl[1]));
+    V3 r1 = ld(xr, pick4(P.v, P.sl[2])) - ld(xr, pick4(P.v, P.sl[3]));
     double eps = 1e-3 * dot(r0, r0) * dot(r1, r1);
     return -I1 * (I1 - 2 * eps) / (eps * eps) * b;
 }
