@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kThreads = 1024;    // workgroup size of the PCG kernels (16 waves)
 constexpr int kRedBlocks = 1024;  // producers of dot partials use at most this many workgroups
+constexpr int kCollectThreads = 256;  // MAS collect: one block row per thread, workgroups for every CU
 
 // partial dot products of a.b, one per block (FP64 accumulation)
 template <typename VA, typename VB>
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(kThreads) k_update_xr(long long N, double* __r
 // block-Jacobi of the affine bodies on block rows < abd_rows (when abd_inv is
 // set), identity elsewhere, and partials of r.z; one thread per block row.
 template <typename V, typename T>
-__global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __restrict__ r, V* __restrict__ z,
+__global__ void __launch_bounds__(kCollectThreads) k_collect_dot(int nb, const V* __restrict__ r, V* __restrict__ z,
                                                           int sub_off, int n_mas, int L, const int* __restrict__ node_of,
                                                           const T* __restrict__ Z, const double* __restrict__ abd_inv,
                                                           int abd_rows, double* __restrict__ part_rz,
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(kThreads) k_collect_dot(int nb, const V* __res
         if (hp && blockIdx.x == 0 && threadIdx.x == 0) publish_progress(ctrl, sc, hp, hq);
         return;
     }
-    __shared__ double sh[kThreads / 64];
+    __shared__ double sh[kCollectThreads / 64];
     double v = 0.0;
     // the stop test's partials are loaded first and reduced after z (whose
     // loads do not depend on the test), so the two latencies overlap
@@ -386,13 +387,13 @@ struct Engine {
     volatile double* pub_q = nullptr;
     static constexpr bool kFp32Mas = sizeof(T) == 4;
 
-    // z = M^-1 r fused with the r.z partials (gN workgroups) and the stop test
+    // z = M^-1 r fused with the r.z partials (nRZ workgroups) and the stop test
     void precond_dot(double* rz_out, const double* rz_prev, int rule_now) {
         hipStream_t s = c.stream;
         if (mas) {
             if (fused) mas_solve_with_top(c, kFp32Mas, ctrl);
             else mas_bank_solve(c, kFp32Mas, ctrl);
-            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(gN), dim3(kThreads), 0, s, c.nb, r, z, c.sub_offset,
+            hipLaunchKernelGGL((k_collect_dot<V, T>), dim3(nRZ), dim3(kCollectThreads), 0, s, c.nb, r, z, c.sub_offset,
                                c.mas.n, c.mas.L, c.mas.node_of.p, (const T*)Z, abd_inv(c), abd_rows(c), rz_out,
                                rz_prev, nRZ, part_rr, gU, sc, ctrl, rule_now, pub_h, pub_q);
             SGIPC_LAUNCH_CHECK();
@@ -451,7 +452,10 @@ struct Engine {
             gU = gN;
         }
         fused = mas && mas_fused(c);
-        nRZ = gN;
+        // r.z partials: one per workgroup of the collect (256-thread
+        // workgroups, so a system of ~100 k block rows spreads over every CU)
+        // or of the simple preconditioner apply
+        nRZ = mas ? std::max(1, std::min(div_up(c.nb, kCollectThreads), kRedBlocks)) : gN;
         hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, max_iter);
         SGIPC_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL, tol);
