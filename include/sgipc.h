@@ -361,6 +361,15 @@ int sgipc_set_schedule(sgipc_ctx* ctx, int group, int iters_per_graph);
  * includes both forms among its candidates and leaves the faster set. */
 int sgipc_set_batch_mode(sgipc_ctx* ctx, int persistent, int threads, int max_workgroups);
 
+/* Mixed precision (SGIPC_PRECISION_MIXED): each FP32 inner solve runs to
+ * the relative tolerance max(0.9 tol ||b|| / ||r||, floor_tol) of the FP64
+ * residual it starts from, then x and the FP64 true residual are updated;
+ * this sets floor_tol (default 1e-6, env SGIPC_INNER_TOL).  A small floor
+ * reaches tol in one or two inner solves; a large one (e.g. 1e-4) restarts
+ * more often but wastes fewer FP32 iterations where FP32 rounding of a very
+ * stiff system stalls the true residual.  No reference counterpart. */
+int sgipc_set_inner_tol(sgipc_ctx* ctx, double floor_tol);
+
 /* ---- sorting ------------------------------------------------------------------
  * No reference counterpart (the reference sorts with thrust/CUB inside its
  * convert, linear_system/utils/converter.cu).  The library sorts with its own

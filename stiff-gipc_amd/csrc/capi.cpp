@@ -99,6 +99,10 @@ int sgipc_ctx_create(int device, sgipc_ctx** out) {
         if (const char* f = std::getenv("SGIPC_SPMV_FULL32")) c->spmv_full32 = std::atoi(f) != 0;
         if (const char* f = std::getenv("SGIPC_MAS_FUSED")) c->mas_fused = std::atoi(f) != 0;
         if (const char* f = std::getenv("SGIPC_SORT")) c->sort_impl = std::string(f) == "rocprim" ? 1 : 0;
+        if (const char* f = std::getenv("SGIPC_INNER_TOL")) {
+            const double v = std::atof(f);
+            if (v >= 0.0 && v < 1.0) c->inner_tol = v;
+        }
         if (const char* f = std::getenv("SGIPC_BANK_INVERSE")) {
             const int v = std::atoi(f);
             if (v >= SGIPC_BANK_INVERSE_VALU64 && v <= SGIPC_BANK_INVERSE_MFMA32_REFINED) c->bank_inverse = v;
@@ -509,8 +513,9 @@ int sgipc_precond_apply(sgipc_ctx* c, const double* r, double* z) {
     });
 }
 
-static sgipc::SolveOpts solve_opts_of(const sgipc_solve_opts* o) {
+static sgipc::SolveOpts solve_opts_of(const sgipc_ctx* c, const sgipc_solve_opts* o) {
     sgipc::SolveOpts so;
+    so.inner_tol = c->inner_tol;
     if (o) {
         so.precision = o->precision;
         so.stop_rule = o->stop_rule;
@@ -530,7 +535,7 @@ int sgipc_solve_batched(sgipc_ctx* c, int n_scenes, const int* row_start, const 
     CHECK_CTX(c);
     return guard(c, [&] {
         if (n_scenes <= 0 || !row_start || !b || !x || !st) throw sgipc::arg_error("solve_batched: invalid argument");
-        sgipc::SolveOpts so = solve_opts_of(o);
+        sgipc::SolveOpts so = solve_opts_of(c, o);
         set_device(c);
         std::vector<sgipc::SolveStats> s2(n_scenes);
         sgipc::prof_begin(*c, sgipc::PROF_SOLVE);
@@ -584,6 +589,14 @@ int sgipc_set_batch_mode(sgipc_ctx* c, int persistent, int threads, int max_work
     });
 }
 
+int sgipc_set_inner_tol(sgipc_ctx* c, double floor_tol) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (!(floor_tol >= 0.0 && floor_tol < 1.0)) throw sgipc::arg_error("set_inner_tol: must be in [0, 1)");
+        c->inner_tol = floor_tol;
+    });
+}
+
 int sgipc_set_sort(sgipc_ctx* c, int impl) {
     CHECK_CTX(c);
     return guard(c, [&] {
@@ -620,7 +633,7 @@ int sgipc_solve(sgipc_ctx* c, const double* b, double* x, const sgipc_solve_opts
     CHECK_CTX(c);
     return guard(c, [&] {
         if (!b || !x) throw sgipc::arg_error("solve: null vector");
-        const sgipc::SolveOpts so = solve_opts_of(o);
+        const sgipc::SolveOpts so = solve_opts_of(c, o);
         set_device(c);
         sgipc::SolveStats s2;
         sgipc::prof_begin(*c, sgipc::PROF_SOLVE);

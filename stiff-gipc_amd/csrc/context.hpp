@@ -240,6 +240,7 @@ struct Ctx {
     DBuf<double> tune_x;            // autotuner output scratch
     DBuf<int> scan_tmp;
     int sort_impl = 0;              // SORT_OWN, or SORT_ROCPRIM (env SGIPC_SORT=rocprim) as the cross-check
+    double inner_tol = 1e-6;        // mixed precision: floor of the inner solves' relative tolerance (env SGIPC_INNER_TOL)
     DBuf<char> sort_tmp;            // radix sort ping-pong buffers and pass state
     DBuf<int> sort_err;             // sticky look-back failure word
     // contact classification workspaces
@@ -373,7 +374,7 @@ struct SolveOpts {
     double max_iter_ratio = 0.3;
     int max_iter = 0;
     int check_every = 8;
-    double inner_tol = 1e-4;  // mixed precision: floor of the inner relative tolerance
+    double inner_tol = 1e-6;  // mixed precision: floor of the inner relative tolerance
 };
 struct SolveStats {
     int iterations = 0;

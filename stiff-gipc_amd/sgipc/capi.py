@@ -84,6 +84,7 @@ SIGNATURES = {
     "sgipc_set_schedule": ([P, I, I], I),
     "sgipc_set_batch_mode": ([P, I, I, I], I),
     "sgipc_set_sort": ([P, I], I),
+    "sgipc_set_inner_tol": ([P, D], I),
     "sgipc_sort_pairs": ([P, P, P, P, P, LL, I, I, I], I),
     "sgipc_get_batch_mode": ([P, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_fem_param_adjoint": ([P, I, P, P, P, P, D, P, P, P, P], I),
