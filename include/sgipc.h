@@ -170,6 +170,15 @@ int sgipc_ground_energy(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, const 
 int sgipc_ground_step_bound(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, const double* d_x,
                             const double* d_dir, const double normal[3], double offset, double slack,
                             double* h_alpha);
+/* Line-search step d_x = d_x_temp - alpha d_dir on the vertices with
+ * btype == 0, on all of them when move_boundary != 0 (d_btype may be NULL:
+ * every vertex free); _stepForward (StiffGIPC/GIPC.cu:7937-7954). */
+int sgipc_step_forward(sgipc_ctx* ctx, int n, double* d_x, const double* d_x_temp, const double* d_dir,
+                       const int* d_btype, double alpha, int move_boundary);
+/* Largest |d_dir[v]| over the listed (surface) vertices, the CFL speed of
+ * GIPC::cfl_largestSpeed (StiffGIPC/GIPC.cu:8916-8947); synchronises. */
+int sgipc_cfl_largest_speed(sgipc_ctx* ctx, int n_ids, const uint32_t* d_ids, const double* d_dir,
+                            double* h_speed);
 
 /* ---- affine bodies (ABD) ---------------------------------------------------
  * Stiff bodies with 12 DOF q = (p, a1, a2, a3) each; body vertex i sits at

@@ -295,6 +295,26 @@ int sgipc_ground_step_bound(sgipc_ctx* c, int m, const uint32_t* ids, const doub
     });
 }
 
+int sgipc_step_forward(sgipc_ctx* c, int n, double* x, const double* x_temp, const double* dir, const int* btype,
+                       double alpha, int move_boundary) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n < 0 || (n > 0 && (!x || !x_temp || !dir))) throw sgipc::arg_error("step_forward: invalid argument");
+        set_device(c);
+        sgipc::step_forward(*c, n, x, x_temp, dir, btype, alpha, move_boundary ? 1 : 0);
+    });
+}
+
+int sgipc_cfl_largest_speed(sgipc_ctx* c, int n_ids, const uint32_t* ids, const double* dir, double* speed) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (n_ids < 0 || !speed || (n_ids > 0 && (!ids || !dir)))
+            throw sgipc::arg_error("cfl_largest_speed: invalid argument");
+        set_device(c);
+        *speed = sgipc::max_speed(*c, n_ids, ids, dir);
+    });
+}
+
 int sgipc_broad_phase(sgipc_ctx* c, int n_surf_verts, const uint32_t* surf_verts, int n_faces, const uint32_t* faces,
                       int n_edges, const uint32_t* edges, const double* x, const int* btype, const int* body_id,
                       double dhat, int* pt_out, int pt_capacity, int* n_pt, int* ee_out, int ee_capacity, int* n_ee) {

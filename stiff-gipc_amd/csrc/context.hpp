@@ -310,6 +310,11 @@ void fem_energy_noalloc(Ctx& c, int T, const double* x, const uint32_t* tets, co
 // energy reductions: workgroups for n items (256 threads), partial buffer,
 // fixed-order sum of g partials to the host
 int energy_grid(long long n);
+// line-search step x = x_temp - alpha dir (free vertices, or all) and the
+// largest |dir| over listed vertices (the reference's CFL speed)
+void step_forward(Ctx& c, int n, double* x, const double* xt, const double* dir, const int* btype, double alpha,
+                  int move_boundary);
+double max_speed(Ctx& c, int m, const uint32_t* ids, const double* dir);
 double* energy_partials(Ctx& c);
 double energy_finish(Ctx& c, double* part, int g);
 void mas_set_topology(Ctx& c, int n, int node_offset, const unsigned* nstart, const unsigned* nnum,

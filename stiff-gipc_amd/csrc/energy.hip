@@ -127,6 +127,57 @@ void fem_energy_noalloc(Ctx& c, int T, const double* x, const uint32_t* tets, co
     SGIPC_LAUNCH_CHECK();
 }
 
+// ---- line-search step (no energies) ----------------------------------------
+// x = x_temp - alpha dir on the free vertices (|btype| == 0), on every vertex
+// when move_boundary (_stepForward, S/GIPC.cu:7937-7954)
+__global__ void k_step_forward(int n, double* __restrict__ x, const double* __restrict__ xt,
+                               const double* __restrict__ dir, const int* __restrict__ btype, double alpha,
+                               int move_boundary) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    if (!move_boundary && btype && btype[v] != 0) return;
+    const size_t o = 3 * (size_t)v;
+    x[o] = xt[o] - alpha * dir[o];
+    x[o + 1] = xt[o + 1] - alpha * dir[o + 1];
+    x[o + 2] = xt[o + 2] - alpha * dir[o + 2];
+}
+
+// per-workgroup max of |dir_v| over the listed vertices (the CFL speed of
+// cfl_largestSpeed, S/GIPC.cu:8916-8947; _reduct_max_cfl_to_double :7157)
+__global__ void __launch_bounds__(kEThreads) k_max_speed(int m, const uint32_t* __restrict__ ids,
+                                                          const double* __restrict__ dir, double* __restrict__ part) {
+    __shared__ double sh[kEThreads / 64];
+    double v = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const size_t o = 3 * (size_t)ids[i];
+        const double a = dir[o], b = dir[o + 1], c = dir[o + 2];
+        v = fmax(v, sqrt(a * a + b * b + c * c));
+    }
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sh[0];
+        for (int i = 1; i < kEThreads / 64; ++i) w = fmax(w, sh[i]);
+        part[blockIdx.x] = w;
+    }
+}
+
+__global__ void __launch_bounds__(kEThreads) k_max_total(const double* __restrict__ part, int g,
+                                                          double* __restrict__ out) {
+    __shared__ double sh[kEThreads / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < g; i += blockDim.x) v = fmax(v, part[i]);
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sh[0];
+        for (int i = 1; i < kEThreads / 64; ++i) w = fmax(w, sh[i]);
+        *out = w;
+    }
+}
+
 int energy_grid(long long n) {
     long long g = div_up(n, kEThreads);
     return (int)(g < 1 ? 1 : (g > kEParts ? kEParts : g));
@@ -161,6 +212,29 @@ double fem_energy(Ctx& c, int T, const double* x, const uint32_t* tets, const do
                        lam, part);
     SGIPC_LAUNCH_CHECK();
     return dt * dt * energy_finish(c, part, g);
+}
+
+void step_forward(Ctx& c, int n, double* x, const double* xt, const double* dir, const int* btype, double alpha,
+                  int move_boundary) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_step_forward, dim3(div_up(n, 256)), dim3(256), 0, c.stream, n, x, xt, dir, btype, alpha,
+                       move_boundary);
+    SGIPC_LAUNCH_CHECK();
+}
+
+double max_speed(Ctx& c, int m, const uint32_t* ids, const double* dir) {
+    if (m <= 0) return 0.0;
+    double* part = energy_partials(c);
+    const int g = energy_grid(m);
+    hipLaunchKernelGGL(k_max_speed, dim3(g), dim3(kEThreads), 0, c.stream, m, ids, dir, part);
+    SGIPC_LAUNCH_CHECK();
+    double* out = part + kEParts;
+    hipLaunchKernelGGL(k_max_total, dim3(1), dim3(kEThreads), 0, c.stream, (const double*)part, g, out);
+    SGIPC_LAUNCH_CHECK();
+    double h = 0.0;
+    SGIPC_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    return h;
 }
 
 }  // namespace sgipc

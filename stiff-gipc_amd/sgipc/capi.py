@@ -85,6 +85,8 @@ SIGNATURES = {
     "sgipc_set_batch_mode": ([P, I, I, I], I),
     "sgipc_set_sort": ([P, I], I),
     "sgipc_set_inner_tol": ([P, D], I),
+    "sgipc_step_forward": ([P, I, P, P, P, P, D, I], I),
+    "sgipc_cfl_largest_speed": ([P, I, P, P, C.POINTER(D)], I),
     "sgipc_sort_pairs": ([P, P, P, P, P, LL, I, I, I], I),
     "sgipc_get_batch_mode": ([P, C.POINTER(I), C.POINTER(I)], I),
     "sgipc_fem_param_adjoint": ([P, I, P, P, P, P, D, P, P, P, P], I),
