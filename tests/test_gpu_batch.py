@@ -209,6 +209,25 @@ def test_persistent_budget_cut(torch_cuda, five_scenes):
         assert not st0[s].converged and not st1[s].converged
 
 
+@pytest.mark.parametrize("form", FORMS)
+def test_batched_empty_scene(torch_cuda, five_scenes, form):
+    """A scene with no block rows (row_start repeats a value): zero
+    iterations, converged, and the other scenes unchanged."""
+    from sgipc import capi
+    scs, bn = five_scenes
+    print("case: batched with an empty scene (%s)" % form, flush=True)
+    rs = [int(v) for v in bn.row_start]
+    rs_e = rs[:2] + [rs[1]] + rs[2:]                               # empty scene at index 1
+    with _mode(bn.sys, form):
+        st0, x0 = _solve_batched(bn.sys, rs, capi.PRECISION_FP64, 1e-5, 0)
+        st1, x1 = _solve_batched(bn.sys, rs_e, capi.PRECISION_FP64, 1e-5, 0)
+    assert st1[1].iterations == 0 and st1[1].converged and st1[1].rel_residual == 0.0
+    others = [0] + list(range(2, bn.S + 1))
+    for s0, s1 in zip(range(bn.S), others):
+        assert st1[s1].iterations == st0[s0].iterations and st1[s1].converged
+    assert np.array_equal(x0, x1)
+
+
 def test_batched_rejects_coupled_split(torch_cuda, five_scenes):
     from sgipc import capi
     from sgipc.capi import SgipcError
