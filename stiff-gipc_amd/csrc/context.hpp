@@ -121,6 +121,7 @@ struct BatchWork {
     DBuf<int> order, queue, refine;                 // scenes largest first; scene queue; refinements
     DBuf<int> mlo, mhi, node_scene, bank_scene;     // per scene and MAS level node ranges; scene of each node / bank
     DBuf<double> relres;                            // per-scene true relative residuals
+    PcgGraph graph_persist;                         // the fill + persistent launch, captured
 };
 
 // Affine bodies coupled to the FEM system (abd.hip)

@@ -1,4 +1,5 @@
-// Batched PCG + MAS as one persistent launch: every workgroup takes scenes
+// Batched PCG + MAS as one persistent launch (replayed from a captured
+// graph): every workgroup takes scenes
 // from a queue (largest first) and runs a whole scene's solve -- all PCG
 // iterations, the MAS apply of every level, and for mixed precision the
 // outer FP64 refinement -- with workgroup barriers only.  Scenes share no
@@ -459,29 +460,80 @@ int occupancy_of() {
 }
 
 template <int NTH, int G, typename V, typename MV, typename T>
-void launch_persist(Ctx& c, const PArgs<V, MV, T>& a, int S) {
+int persist_grid(Ctx& c, int S) {
     int ncu = 0;
     SGIPC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
     const int per_cu = std::max(1, occupancy_of<NTH, G, V, MV, T>());
     int grid = std::max(1, std::min(S, per_cu * std::max(ncu, 1)));
     if (c.batch_max_wg > 0) grid = std::min(grid, c.batch_max_wg);
-    hipLaunchKernelGGL((k_pcg_persist<NTH, G, V, MV, T>), dim3(grid), dim3(NTH), 0, c.stream, a);
-    SGIPC_LAUNCH_CHECK();
+    return grid;
 }
 
 template <typename V, typename MV, typename T>
+size_t persist_key(const PArgs<V, MV, T>& a, int G, int NTH, int grid, hipStream_t s) {
+    size_t h = 1469598103934665603ull;
+    auto mix = [&](size_t v) { h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); };
+    for (const void* q : {(const void*)a.order, (const void*)a.srow, (const void*)a.mlo, (const void*)a.mhi,
+                          (const void*)a.node_scene, (const void*)a.queue, (const void*)a.eptr, (const void*)a.ent,
+                          (const void*)a.val, (const void*)a.val64, (const void*)a.part_map,
+                          (const void*)a.going_next, (const void*)a.fine_mask, (const void*)a.comp_mask,
+                          (const void*)a.node_of, (const void*)a.inv, (const void*)a.R, (const void*)a.Z,
+                          (const void*)a.b, (const void*)a.x, (const void*)a.r, (const void*)a.z, (const void*)a.p,
+                          (const void*)a.Ap, (const void*)a.r64, (const void*)a.d, (const void*)a.iters,
+                          (const void*)a.noconv, (const void*)a.refine, (const void*)a.relres, (const void*)s})
+        mix((size_t)q);
+    for (int v : {a.S, a.L, a.n, a.rule, a.maxit, a.mixed, G, NTH, grid, (int)sizeof(V), (int)sizeof(MV),
+                  (int)sizeof(T)})
+        mix((size_t)(unsigned)v);
+    mix(std::hash<double>{}(a.tol));
+    mix(std::hash<double>{}(a.inner_tol));
+    return h;
+}
+
+// The queue reset (a fill kernel, no memset) and the persistent launch,
+// captured once into a graph per argument set and replayed.
+template <typename V, typename MV, typename T>
 void run_persist(Ctx& c, const PArgs<V, MV, T>& a, int S) {
     const int G = c.spmv_group, NTH = c.batch_threads;
-    // the queue counter starts at zero (a fill kernel, no memset)
-    fill<int>(a.queue, 0, 1, c.stream);
-    auto go = [&](auto nth) {
+    hipStream_t s = c.stream;
+    int grid = 0;
+    auto pick = [&](auto nth, auto body) {
         constexpr int N = decltype(nth)::value;
-        if (G == 4) launch_persist<N, 4>(c, a, S);
-        else if (G == 16) launch_persist<N, 16>(c, a, S);
-        else launch_persist<N, 8>(c, a, S);
+        if (G == 4) body(std::integral_constant<int, N>{}, std::integral_constant<int, 4>{});
+        else if (G == 16) body(std::integral_constant<int, N>{}, std::integral_constant<int, 16>{});
+        else body(std::integral_constant<int, N>{}, std::integral_constant<int, 8>{});
     };
-    if (NTH == 512) go(std::integral_constant<int, 512>{});
-    else go(std::integral_constant<int, 1024>{});
+    auto each = [&](auto body) {
+        if (NTH == 512) pick(std::integral_constant<int, 512>{}, body);
+        else pick(std::integral_constant<int, 1024>{}, body);
+    };
+    each([&](auto nth, auto g) { grid = persist_grid<decltype(nth)::value, decltype(g)::value, V, MV, T>(c, S); });
+    const size_t key = persist_key(a, G, NTH, grid, s);
+    PcgGraph& gc = c.batch.graph_persist;
+    if (!gc.exec || gc.key != key) {
+        if (gc.exec) { (void)hipGraphExecDestroy(gc.exec); gc.exec = nullptr; }
+        hipGraph_t gr = nullptr;
+        SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            fill<int>(a.queue, 0, 1, s);
+            each([&](auto nth, auto g) {
+                hipLaunchKernelGGL((k_pcg_persist<decltype(nth)::value, decltype(g)::value, V, MV, T>), dim3(grid),
+                                   dim3(decltype(nth)::value), 0, s, a);
+            });
+            SGIPC_LAUNCH_CHECK();
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(s, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        SGIPC_HIP(hipStreamEndCapture(s, &gr));
+        hipError_t e = hipGraphInstantiate(&gc.exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        SGIPC_HIP(e);
+        gc.key = key;
+    }
+    SGIPC_HIP(hipGraphLaunch(gc.exec, s));
 }
 
 }  // namespace

@@ -197,6 +197,23 @@ def test_persistent_matches_launch_sequence(torch_cuda, five_scenes, precond, pr
     bn.sys.build_precond(capi.PRECOND_MAS)
 
 
+def test_persistent_graph_replays(torch_cuda, five_scenes):
+    """The persistent form is captured once per argument set and replayed:
+    three replays, with eager launch-sequence solves between them, give the
+    same bits each time."""
+    from sgipc import capi
+    scs, bn = five_scenes
+    print("case: persistent graph replays", flush=True)
+    outs = []
+    for rep in range(3):
+        with _mode(bn.sys, "persistent"):
+            st, x = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_FP64, 1e-5, 0)
+        outs.append((x, [st[s].iterations for s in range(bn.S)]))
+        _solve_batched(bn.sys, bn.row_start, capi.PRECISION_FP64, 1e-3, 1)   # eager work in between
+    for x, its in outs[1:]:
+        assert np.array_equal(x, outs[0][0]) and its == outs[0][1]
+
+
 def test_persistent_budget_cut(torch_cuda, five_scenes):
     from sgipc import capi
     scs, bn = five_scenes
