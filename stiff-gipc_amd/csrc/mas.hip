@@ -307,74 +307,6 @@ __global__ void k_aggregate(int n, int L, const int* __restrict__ real_map, cons
 // ---- Galerkin projection ----------------------------------------------------
 __device__ inline int pack_index(int B, int r, int c) { return B * r - r * (r + 1) / 2 + c; }  // r <= c
 
-// Galerkin projection of every BCOO block into the bank matrices of all
-// levels.  Level 0 targets are unique (plain stores); coarse levels whose
-// banks all fit in LDS (the top levels, where many fine blocks land on few
-// entries) are pre-summed per workgroup with LDS atomics and flushed once;
-// the remaining coarse levels use global FP64 atomics.
-constexpr int kProjLdsDoubles = 6 * 1224;  // 6 banks of 16 (136 blocks x 9) or 1 bank of 32 + margin
-__global__ void __launch_bounds__(256) k_project(int K, const int* __restrict__ rows, const int* __restrict__ cols,
-                                                 const double* __restrict__ vals, int off, int n, int B, int L,
-                                                 const int* __restrict__ node_of, double* __restrict__ bank_upper,
-                                                 int lds_bank_lo, int lds_nbanks) {
-    __shared__ double acc[kProjLdsDoubles];
-    const int npack = B * (B + 1) / 2;
-    const int lds_len = lds_nbanks * npack * 9;
-    for (int i = threadIdx.x; i < lds_len; i += blockDim.x) acc[i] = 0.0;
-    __syncthreads();
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
-        int r = rows[k] - off, c = cols[k] - off;
-        if (r < 0 || c < 0 || r >= n || c >= n) continue;
-        double H[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];  // column-major
-        for (int l = 0; l < L; ++l) {
-            int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
-            if (I / B != J / B) continue;
-            int li = I % B, lj = J % B, bank = I / B;
-            int pidx;
-            double blk[9];
-            if (li == lj) {
-                pidx = pack_index(B, li, li);
-#pragma unroll
-                for (int p = 0; p < 3; ++p)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q)
-                        blk[p + 3 * q] = (r == c) ? H[p + 3 * q] : H[p + 3 * q] + H[q + 3 * p];
-            } else if (li < lj) {
-                pidx = pack_index(B, li, lj);
-#pragma unroll
-                for (int e = 0; e < 9; ++e) blk[e] = H[e];
-            } else {
-                pidx = pack_index(B, lj, li);
-#pragma unroll
-                for (int p = 0; p < 3; ++p)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) blk[p + 3 * q] = H[q + 3 * p];
-            }
-            if (l == 0) {
-                double* dst = bank_upper + ((size_t)bank * npack + pidx) * 9;  // unique per BCOO block
-#pragma unroll
-                for (int e = 0; e < 9; ++e) dst[e] += blk[e];
-            } else if (bank >= lds_bank_lo && bank < lds_bank_lo + lds_nbanks) {
-                double* dst = acc + ((size_t)(bank - lds_bank_lo) * npack + pidx) * 9;
-#pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
-            } else {
-                double* dst = bank_upper + ((size_t)bank * npack + pidx) * 9;
-#pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
-            }
-        }
-    }
-    __syncthreads();
-    double* g = bank_upper + (size_t)lds_bank_lo * npack * 9;
-    for (int i = threadIdx.x; i < lds_len; i += blockDim.x)
-        if (acc[i] != 0.0) atomicAdd(g + i, acc[i]);
-}
-
 // Projection of one BCOO block H = A(r, c) (r <= c, column-major) onto the
 // level-l bank entry of its node pair: the packed index and the 3x3 block
 // (H + H^T on a coarse diagonal, H^T below the bank diagonal).
@@ -1403,8 +1335,14 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         }
     }
     fill<double>(m.bank_upper.p, 0.0, (long long)m.nbanks * npack * 9, s);
+    // The banked projection (16-vertex banks, vertex ids in slot order) sums
+    // coarse levels in LDS with few atomics; every other case -- and the
+    // deterministic mode -- uses the ordered projection, which sorts
+    // (entry, block) records and sums each entry in a fixed order: no atomic
+    // contention where many fine blocks land on few coarse entries
+    // (32-vertex banks: 23 ms with global FP64 atomics at 137 k vertices).
     const bool banked = (B == 16) && m.slot_ordered && m.nb1 > 0 && L >= 2;
-    if (c.K > 0 && c.deterministic) {
+    if (c.K > 0 && (c.deterministic || !banked)) {
         if ((long long)c.K >= (1LL << 29)) throw capacity_error("MAS build: too many blocks for the ordered projection");
         const unsigned sentinel = (unsigned)((long long)m.nbanks * npack);
         int bits = 1;
@@ -1477,21 +1415,6 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
                                (const double*)scratch, m.nb1, top_len, m.bank_upper.p + (size_t)top_lo * kProjBank16);
             SGIPC_LAUNCH_CHECK();
         }
-    } else if (c.K > 0) {
-        // coarse levels whose banks fit the LDS accumulator (top levels first)
-        int lds_lo = m.nbanks, lds_nb = 0;
-        for (int l = L - 1; l >= 1; --l) {
-            int beg = m.level_size_h[2 * l + 1] / B;
-            int end = (l + 1 < L + 1) ? m.level_size_h[2 * (l + 1) + 1] / B : m.nbanks;
-            int nb_l = end - beg;
-            if ((lds_nb + nb_l) * npack * 9 > kProjLdsDoubles) break;
-            lds_nb += nb_l;
-            lds_lo = beg;
-        }
-        int grid = std::min(div_up(c.K, 256), 1024);
-        hipLaunchKernelGGL(k_project, dim3(grid), dim3(256), 0, s, c.K, c.t_rows, c.t_cols, c.t_vals, sub_offset, n, B,
-                           L, m.node_of.p, m.bank_upper.p, lds_lo, lds_nb);
-        SGIPC_LAUNCH_CHECK();
     }
     // the FP32-with-refinement inverses serve FP32 applies only
     int inv_mode = c.bank_inverse;
