@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kCollectThreads) k_collect_dot(int nb, const V
             const double* P = abd_inv + 144 * (size_t)b + 36 * k;  // rows 3k .. 3k+2 of the body inverse
             const V* rb = r + 12 * (size_t)b;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
+#pragma unroll 2  // a full unroll's 36 loads in flight would set the whole kernel's register budget
             for (int j = 0; j < 12; ++j) {
                 const double rj = (double)rb[j];
                 s0 += P[j] * rj; s1 += P[12 + j] * rj; s2 += P[24 + j] * rj;
