@@ -27,9 +27,8 @@ struct SymBcoo {
     bool valf_ok = false;
     // row-contiguous FP32 copy of both triangles in entry order (mixed
     // precision): entry e of ent as its block, transposed where it is a
-    // transposed reference, 10 floats (9 column-major + pad); its column
+    // transposed reference, 10 words (9 column-major floats + its column)
     DBuf<float> valfull;
-    DBuf<int> colfull;
     bool full_ok = false;
 };
 
@@ -295,7 +294,7 @@ double ground_step_bound(Ctx& c, int m, const unsigned* ids, const double* x, co
 int bcoo_convert(Ctx& c, long long nt, int nblockrows);
 void bsr_build(Ctx& c);
 const float* bsr_values_f32(Ctx& c);
-const float* bsr_full_f32(Ctx& c, const int** cols);
+const float* bsr_full_f32(Ctx& c);
 void spmv64(Ctx& c, double a, const double* x, double b, double* y);
 
 double kinetic_energy(Ctx& c, int n, const double* x, const double* xt, const double* m);

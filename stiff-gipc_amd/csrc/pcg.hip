@@ -227,14 +227,15 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
 
 // The same with the row-contiguous FP32 copy of both triangles: the G lanes
 // of a row read consecutive entries (40 bytes each: two 16-byte loads at
-// 8-byte alignment and one 8-byte load), no block index, no transposition in
-// registers; the column's three z values come in one 12-byte load; same
+// 8-byte alignment and one 8-byte load that also brings the column), no
+// block index, no transposition in registers, no column stream (measured
+// 44 -> 40 bytes per entry: 19.7 -> 19.2 us per launch at 137 k block
+// rows); the column's three z values come in one 12-byte load; same
 // entry order and the same products, so the same sums.  (Measured and kept
 // out: 9-float entries read at 4-byte alignment, 30 % slower; the row's own
 // z, p, Ap loaded ahead of the row sums, 35 % slower.)
 template <int G, typename V>
 __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* __restrict__ eptr,
-                                                            const int* __restrict__ colf,
                                                             const float* __restrict__ vf, const V* __restrict__ z,
                                                             V* __restrict__ p, V* __restrict__ Ap,
                                                             const double* __restrict__ rz_last,
@@ -267,11 +268,14 @@ __global__ void __launch_bounds__(kThreads) k_spmv_pap_full(int nb, const int* _
             int e = eptr[row] + sub;
             for (; e + G < end; e += 2 * G) {  // two entries in flight
                 const FullEntry fa = load_full_entry(vf, e), fb = load_full_entry(vf, e + G);
-                const f3_a4 xa = load_z3(z, colf[e]), xb = load_z3(z, colf[e + G]);
+                const f3_a4 xa = load_z3(z, fa.col), xb = load_z3(z, fb.col);
                 full_madd(fa, xa, s0, s1, s2);
                 full_madd(fb, xb, s0, s1, s2);
             }
-            if (e < end) full_madd(load_full_entry(vf, e), load_z3(z, colf[e]), s0, s1, s2);
+            if (e < end) {
+                const FullEntry f = load_full_entry(vf, e);
+                full_madd(f, load_z3(z, f.col), s0, s1, s2);
+            }
         }
         group_sum3<G, float>(s0, s1, s2);
         if (!have_be) {
@@ -380,7 +384,6 @@ struct Engine {
     int G = 8;  // lanes per block row in the SpMV (4, 8 or 16)
     // mixed precision: the row-contiguous FP32 copy (null: the upper BCOO)
     const float* vfull = nullptr;
-    const int* cfull = nullptr;
     // pinned progress words published by the MAS collect kernel (set while
     // the last iteration of a captured group is enqueued)
     volatile int* pub_h = nullptr;
@@ -417,7 +420,7 @@ struct Engine {
         };
         if (vfull) {
             auto launchf = [&](auto kern) {
-                hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, cfull, vfull, (const V*)z, p,
+                hipLaunchKernelGGL(kern, dim3(gS), dim3(kThreads), 0, s, S.nb, S.eptr.p, vfull, (const V*)z, p,
                                    Ap, (const double*)rz_last, (const double*)rz_this, nRZ, part_pap, ctrl);
             };
             if (G == 4) launchf(k_spmv_pap_full<4, V>);
@@ -484,7 +487,7 @@ struct Engine {
         mix((size_t)Ap); mix((size_t)x); mix((size_t)part_rz[0]); mix((size_t)sc);
         mix((size_t)ctrl); mix((size_t)mas); mix((size_t)R); mix((size_t)Z); mix((size_t)Dinv); mix((size_t)rule);
         mix((size_t)c.sub_offset); mix((size_t)c.nb); mix((size_t)c.stream);
-        mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c)); mix((size_t)vfull); mix((size_t)cfull);
+        mix((size_t)abd_inv(c)); mix((size_t)abd_rows(c)); mix((size_t)vfull);
         mix((size_t)fused); mix((size_t)nRZ);
         if (mas) {
             const MasState& m = c.mas;
@@ -766,7 +769,7 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
                                       mas ? c.mas.R32.p : nullptr, mas ? c.mas.Z32.p : nullptr,
                                       c.precond_kind == 1 ? w.Dinv.p : nullptr, 1};
         E.G = G;
-        if (c.spmv_full32) E.vfull = bsr_full_f32(c, &E.cfull);
+        if (c.spmv_full32) E.vfull = bsr_full_f32(c);
         fill<double>(x, 0.0, N, s);
         hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();

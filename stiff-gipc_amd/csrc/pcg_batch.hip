@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(1024) k_bspmv(const int* __restrict__ chunk_ro
 template <int G>
 __global__ void __launch_bounds__(1024) k_bspmv_full(const int* __restrict__ chunk_row,
                                                      const int* __restrict__ chunk_scene, const int* __restrict__ eptr,
-                                                     const int* __restrict__ colf, const float* __restrict__ vf,
+                                                     const float* __restrict__ vf,
                                                      const float* __restrict__ p, float* __restrict__ Ap,
                                                      const int* __restrict__ sstop, double* __restrict__ part,
                                                      const int* __restrict__ ctrl) {
@@ -95,11 +95,14 @@ __global__ void __launch_bounds__(1024) k_bspmv_full(const int* __restrict__ chu
         int e = eptr[row] + sub;
         for (; e + G < end; e += 2 * G) {  // two entries in flight
             const FullEntry fa = load_full_entry(vf, e), fb = load_full_entry(vf, e + G);
-            const f3_a4 xa = load_z3(p, colf[e]), xb = load_z3(p, colf[e + G]);
+            const f3_a4 xa = load_z3(p, fa.col), xb = load_z3(p, fb.col);
             full_madd(fa, xa, s0, s1, s2);
             full_madd(fb, xb, s0, s1, s2);
         }
-        if (e < end) full_madd(load_full_entry(vf, e), load_z3(p, colf[e]), s0, s1, s2);
+        if (e < end) {
+            const FullEntry f = load_full_entry(vf, e);
+            full_madd(f, load_z3(p, f.col), s0, s1, s2);
+        }
     }
     group_sum3<G, float>(s0, s1, s2);
     double contrib = 0.0;
@@ -348,7 +351,6 @@ struct BEngine {
     bool fp32mas;
     int rule;
     const float* vfull = nullptr;  // mixed: row-contiguous FP32 copy of both triangles (else the upper BCOO)
-    const int* colfull = nullptr;
 
     BScal scal() {
         return BScal{w.bb.p, w.rz.p, w.rz0.p, w.alpha.p, w.beta.p, w.tol.p, w.segout.p, w.stop.p, w.iters.p,
@@ -388,7 +390,7 @@ struct BEngine {
             if (vfull) {
                 auto launchf = [&](auto kern) {
                     hipLaunchKernelGGL(kern, dim3(nch), dim3(1024), 0, s, (const int*)w.chunk_row.p,
-                                       (const int*)w.chunk_scene.p, (const int*)B.eptr.p, colfull, vfull,
+                                       (const int*)w.chunk_scene.p, (const int*)B.eptr.p, vfull,
                                        (const float*)p, (float*)Ap, (const int*)w.stop.p, w.part_pap.p,
                                        (const int*)w.ctrl.p);
                 };
@@ -442,7 +444,7 @@ struct BEngine {
         mix((size_t)N); mix((size_t)S); mix((size_t)nch); mix((size_t)G); mix((size_t)per_graph);
         mix((size_t)rule); mix((size_t)r); mix((size_t)z); mix((size_t)p); mix((size_t)Ap); mix((size_t)x);
         mix((size_t)val); mix((size_t)fp32mas); mix((size_t)B.eptr.p); mix((size_t)B.ent.p); mix((size_t)B.nb);
-        mix((size_t)vfull); mix((size_t)colfull); mix((size_t)mas_fused(c));
+        mix((size_t)vfull); mix((size_t)mas_fused(c));
         for (const void* q : {(const void*)w.chunk_row.p, (const void*)w.chunk_scene.p, (const void*)w.scene_chunk.p,
                               (const void*)w.part_pap.p, (const void*)w.part_rr.p, (const void*)w.part_rz.p,
                               (const void*)w.bb.p, (const void*)w.rz.p, (const void*)w.rz0.p, (const void*)w.alpha.p,
@@ -618,12 +620,10 @@ void pcg_solve_batched(Ctx& c, int S, const int* h_row_start, const double* b, d
         double* d64 = pw.d64.reserve(N + 1);
         // the SpMV reads the row-contiguous FP32 copy of both triangles, or
         // (SGIPC_SPMV_FULL32=0) the FP32 copy of the upper BCOO; only one is built
-        const int* colfull = nullptr;
-        const float* vfull = c.spmv_full32 ? bsr_full_f32(c, &colfull) : nullptr;
+        const float* vfull = c.spmv_full32 ? bsr_full_f32(c) : nullptr;
         BEngine<float, float, float> E{c, w, N, S, w.nch, G, rf, pw.zf.reserve(N + 1), pw.pf.reserve(N + 1),
                                       pw.Apf.reserve(N + 1), d64, vfull ? nullptr : bsr_values_f32(c), true, 1};
         E.vfull = vfull;
-        E.colfull = colfull;
         fill<double>(x, 0.0, N, s);
         hipLaunchKernelGGL((k_bcast<double, double>), dim3(flat_grid(N)), dim3(256), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();

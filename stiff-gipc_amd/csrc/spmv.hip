@@ -91,12 +91,12 @@ __global__ void k_to_f32(long long K, const double* __restrict__ s, float* __res
 
 // entry-ordered FP32 copy: eight lanes per block row, lane j taking entries
 // j, j + 8, ... of the row, so a row's lanes write adjacent 40-byte entries
-// (five aligned float2 stores each); rows are walked in per-XCD ranges, as
-// in the SpMV, so the transposed blocks a row gathers are shared in L2.
+// (nine floats and the column: five aligned 8-byte stores each); rows are
+// walked in per-XCD ranges, as in the SpMV, so the transposed blocks a row
+// gathers are shared in L2.
 constexpr int kFullG = 8;
 __global__ void __launch_bounds__(256) k_full_f32(int nb, const int* __restrict__ eptr, const int2* __restrict__ ent,
-                                                  const double* __restrict__ vals, int* __restrict__ colf,
-                                                  float* __restrict__ vf) {
+                                                  const double* __restrict__ vals, float* __restrict__ vf) {
     constexpr int RPB = 256 / kFullG;
     const ChunkWalk w = xcd_chunks((nb + RPB - 1) / RPB);
     for (int ch = w.first; ch < w.end; ch += w.step) {
@@ -115,11 +115,10 @@ __global__ void __launch_bounds__(256) k_full_f32(int nb, const int* __restrict_
             for (int pp = 0; pp < 3; ++pp)
 #pragma unroll
                 for (int qq = 0; qq < 3; ++qq) d[pp + 3 * qq] = (float)(tr ? a[qq + 3 * pp] : a[pp + 3 * qq]);
-            d[9] = 0.0f;
+            d[9] = __int_as_float(q.y);
             float2* o = reinterpret_cast<float2*>(vf + kF32FullStride * (size_t)e);
 #pragma unroll
             for (int k = 0; k < 5; ++k) o[k] = make_float2(d[2 * k], d[2 * k + 1]);
-            colf[e] = q.y;
         }
     }
 }
@@ -181,20 +180,18 @@ void bsr_build(Ctx& c) {
 }
 
 // Row-contiguous FP32 copy of both triangles (mixed-precision SpMV)
-const float* bsr_full_f32(Ctx& c, const int** cols) {
+const float* bsr_full_f32(Ctx& c) {
     if (!c.have_bsr) bsr_build(c);
     SymBcoo& S = c.bsr;
     const size_t ne = 2 * (size_t)S.K;  // bound on the entries (lower references + upper blocks)
     float* vf = S.valfull.reserve(kF32FullStride * ne + 4);
-    int* cf = S.colfull.reserve(ne + 1);
     if (!S.full_ok && S.nb > 0) {
         const int blocks = 8 * std::min(div_up(div_up(S.nb, 256 / kFullG), 8), 2048);
         hipLaunchKernelGGL(k_full_f32, dim3(blocks), dim3(256), 0, c.stream, S.nb, (const int*)S.eptr.p,
-                           (const int2*)S.ent.p, (const double*)c.t_vals, cf, vf);
+                           (const int2*)S.ent.p, (const double*)c.t_vals, vf);
         SGIPC_LAUNCH_CHECK();
     }
     S.full_ok = true;
-    *cols = cf;
     return vf;
 }
 

@@ -40,7 +40,7 @@ __device__ inline ChunkWalk xcd_chunks(int nchunks) {
 // (48 B per block, columns in float4 lanes 0-2), three aligned 16-byte loads.
 typedef double double2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 constexpr int kF32BlockStride = 12;  // floats per block in the FP32 copy
-constexpr int kF32FullStride = 10;   // floats per entry in the row-contiguous FP32 copy (five aligned float2)
+constexpr int kF32FullStride = 10;   // words per entry in the row-contiguous FP32 copy: 9 floats + the column (five aligned 8-byte pairs)
 
 template <typename TV>
 __device__ inline void load_block(const TV* __restrict__ vals, int k, bool tr, TV a[9]);
@@ -128,13 +128,16 @@ typedef float f3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 struct FullEntry {
     f4_a8 lo, hi;  // column-major elements 0-3, 4-7
     float a8;
+    int col;       // block column (the entry's tenth word)
 };
 __device__ inline FullEntry load_full_entry(const float* __restrict__ vf, int e) {
     const float* A = vf + kF32FullStride * (size_t)e;
     FullEntry f;
     f.lo = *reinterpret_cast<const f4_a8*>(A);
     f.hi = *reinterpret_cast<const f4_a8*>(A + 4);
-    f.a8 = reinterpret_cast<const float2*>(A + 8)->x;
+    const int2 t = *reinterpret_cast<const int2*>(A + 8);
+    f.a8 = __int_as_float(t.x);
+    f.col = t.y;
     return f;
 }
 template <typename V>
