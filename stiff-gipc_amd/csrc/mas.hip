@@ -331,32 +331,54 @@ __device__ inline int project_block(int B, int li, int lj, bool same_vertex, con
 }
 
 // Galerkin projection for 16-vertex banks when vertex ids follow slot order:
-// workgroup b owns the vertices of the level-0 banks whose first child lands
-// in level-1 bank b, i.e. the contiguous BCOO rows [vlo_b, vhi_b).  Level 0
-// targets are unique (plain stores).  Levels 1..win_levels accumulate in an
-// LDS window of two banks around this workgroup's nodes (targets outside
-// fall back to global atomics) and are flushed once; the top levels
-// (>= top_lvl) accumulate in LDS and are written as per-workgroup partials,
-// summed afterwards in a fixed order (k_project_top_reduce).  Other levels
-// use global FP64 atomics.
-constexpr int kProjWin = 2;
+// workgroup b owns the vertices whose level-1 node lies in level-1 bank b,
+// i.e. the contiguous BCOO rows [vlo_b, vhi_b).  Level 0 targets are unique
+// (plain stores).  Level-1 and level-2 targets accumulate in one-bank LDS
+// windows (the first owned vertex's banks, which hold nearly every target)
+// flushed once; the top levels (>= top_lvl, at most kProjTopBanks banks)
+// accumulate in LDS and are written as per-workgroup partials, summed
+// afterwards in a fixed order (k_project_top_reduce).  Other levels and
+// window misses use global FP64 atomics.
+// A block whose two vertices share the first owned vertex's level-2 node
+// (most blocks: that node usually covers every owned vertex) adds the same
+// 3x3 block, H + H^T (H on the diagonal), to that node's diagonal block on
+// every level from 2 up, since nodes nest; such blocks are summed in
+// registers and once over the workgroup instead of with LDS atomics that
+// all lanes of a wavefront would aim at the same nine words.
+constexpr int kProjWin1 = 1, kProjWin2 = 1;  // LDS window banks of levels 1 and 2
+constexpr int kProjWinBanks = kProjWin1 + kProjWin2;
 constexpr int kProjWinLevels = 2;
 constexpr int kProjBank16 = 136 * 9;  // doubles per packed 16-vertex bank
-constexpr int kProjTopBanks = 8;
+constexpr int kProjTopBanks = 6;      // with the windows: 78 KB of LDS, two workgroups per CU
 __global__ void __launch_bounds__(512) k_project_banked(
     const int* __restrict__ ustart, const int* __restrict__ rows, const int* __restrict__ cols,
     const double* __restrict__ vals, int off, int n, int L, const int* __restrict__ node_of,
     const int* __restrict__ vlo, const int* __restrict__ vhi, int nbanks, double* __restrict__ bank_upper,
     int win_levels, int top_lvl, int top_lo, int top_nb, double* __restrict__ top_scratch) {
     constexpr int B = 16, NB = kProjBank16;
-    __shared__ double win[kProjWinLevels * kProjWin * NB];
+    __shared__ double win[kProjWinBanks * NB];  // level 1, then level 2
     __shared__ double top[kProjTopBanks * NB];
+    __shared__ double red[9][512 / 64];
     __shared__ int wbase[kProjWinLevels];
     const int b = blockIdx.x, t = threadIdx.x;
-    for (int i = t; i < kProjWinLevels * kProjWin * NB; i += blockDim.x) win[i] = 0.0;
+    for (int i = t; i < kProjWinBanks * NB; i += blockDim.x) win[i] = 0.0;
     for (int i = t; i < top_nb * NB; i += blockDim.x) top[i] = 0.0;
     const int v0 = vlo[b], v1 = vhi[b];
     if (t < kProjWinLevels) wbase[t] = (v0 < v1 && t + 1 < L) ? node_of[(size_t)(t + 1) * n + v0] / B : -1;
+    const int dom2 = (v0 < v1 && L > 2) ? node_of[2 * (size_t)n + v0] : -1;
+    // LDS (or global) address of packed entry pidx of bank `bank` at level l >= 1
+    auto target = [&](int l, int bank, int pidx) -> double* {
+        if (l >= top_lvl) return top + (size_t)(bank - top_lo) * NB + 9 * pidx;
+        if (l <= win_levels) {
+            const int w = bank - wbase[l - 1];
+            if (l == 1 && w == 0) return win + 9 * pidx;
+            if (l == 2 && w >= 0 && w < kProjWin2) return win + (size_t)(kProjWin1 + w) * NB + 9 * pidx;
+        }
+        return bank_upper + (size_t)bank * NB + 9 * pidx;
+    };
+    double acc[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) acc[e] = 0.0;
     __syncthreads();
     if (v0 < v1) {
         const int k0 = ustart[v0 + off], k1 = ustart[v1 + off];
@@ -366,7 +388,9 @@ __global__ void __launch_bounds__(512) k_project_banked(
             double H[9];
 #pragma unroll
             for (int e = 0; e < 9; ++e) H[e] = vals[9 * (size_t)k + e];
-            for (int l = 0; l < L; ++l) {
+            const bool dom = dom2 >= 0 && node_of[2 * (size_t)n + r] == dom2 && node_of[2 * (size_t)n + c] == dom2;
+            const int lend = dom ? 2 : L;  // levels >= 2 of a dominant block: summed below
+            for (int l = 0; l < lend; ++l) {
                 const int I = node_of[(size_t)l * n + r], J = node_of[(size_t)l * n + c];
                 if (I / B != J / B) continue;
                 const int bank = I / B;
@@ -378,25 +402,49 @@ __global__ void __launch_bounds__(512) k_project_banked(
                     for (int e = 0; e < 9; ++e) dst[e] += blk[e];
                     continue;
                 }
-                double* dst;
-                if (l >= top_lvl) {
-                    dst = top + (size_t)(bank - top_lo) * NB + 9 * pidx;
-                } else if (l <= win_levels && bank - wbase[l - 1] >= 0 && bank - wbase[l - 1] < kProjWin) {
-                    dst = win + ((size_t)(l - 1) * kProjWin + (bank - wbase[l - 1])) * NB + 9 * pidx;
-                } else {
-                    dst = bank_upper + (size_t)bank * NB + 9 * pidx;
-                }
+                double* dst = target(l, bank, pidx);
 #pragma unroll
                 for (int e = 0; e < 9; ++e)
                     if (blk[e] != 0.0) atomicAdd(dst + e, blk[e]);
             }
+            if (dom) {
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) acc[p + 3 * q] += (r == c) ? H[p + 3 * q] : H[p + 3 * q] + H[q + 3 * p];
+            }
+        }
+    }
+    // the dominant blocks' sum, over the workgroup, onto the diagonal block
+    // of the first owned vertex's node on every level from 2 up
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        const double s = wave_sum(acc[e]);
+        if ((t & 63) == 0) red[e][t >> 6] = s;
+    }
+    __syncthreads();
+    if (t == 0 && dom2 >= 0) {
+        double s[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            s[e] = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s[e] += red[e][w];
+        }
+        for (int l = 2; l < L; ++l) {
+            const int I = node_of[(size_t)l * n + v0];
+            double* dst = target(l, I / B, pack_index(B, I % B, I % B));
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                if (s[e] != 0.0) atomicAdd(dst + e, s[e]);
         }
     }
     __syncthreads();
-    for (int q = t; q < win_levels * kProjWin * NB; q += blockDim.x) {
+    for (int q = t; q < kProjWinBanks * NB; q += blockDim.x) {
         const double v = win[q];
-        const int lw = q / (kProjWin * NB), w = (q / NB) % kProjWin, e = q % NB;
-        const int bank = wbase[lw] + w;
+        const int w = q / NB, e = q % NB;
+        const int lw = w < kProjWin1 ? 0 : 1;
+        if (lw >= win_levels) continue;
+        const int bank = wbase[lw] + (w < kProjWin1 ? w : w - kProjWin1);
         if (v != 0.0 && bank >= 0 && bank < nbanks) atomicAdd(bank_upper + (size_t)bank * NB + e, v);
     }
     double* out = top_scratch + (size_t)b * top_nb * NB;
