@@ -335,7 +335,7 @@ __device__ inline int project_block(int B, int li, int lj, bool same_vertex, con
 // i.e. the contiguous BCOO rows [vlo_b, vhi_b).  Level 0 targets are unique
 // (plain stores).  Level-1 and level-2 targets accumulate in one-bank LDS
 // windows (the first owned vertex's banks, which hold nearly every target)
-// flushed once; the top levels (>= top_lvl, at most kProjTopBanks banks)
+// flushed once; the top levels (>= top_lvl, at most kProjTopBanksMax banks)
 // accumulate in LDS and are written as per-workgroup partials, summed
 // afterwards in a fixed order (k_project_top_reduce).  Other levels and
 // window misses use global FP64 atomics.
@@ -350,6 +350,8 @@ constexpr int kProjWinBanks = kProjWin1 + kProjWin2;
 constexpr int kProjWinLevels = 2;
 constexpr int kProjBank16 = 136 * 9;  // doubles per packed 16-vertex bank
 constexpr int kProjTopBanks = 6;      // with the windows: 78 KB of LDS, two workgroups per CU
+constexpr int kProjTopBanksMax = 12;  // larger top levels: 137 KB, one workgroup per CU
+template <int TOPB>
 __global__ void __launch_bounds__(512) k_project_banked(
     const int* __restrict__ ustart, const int* __restrict__ rows, const int* __restrict__ cols,
     const double* __restrict__ vals, int off, int n, int L, const int* __restrict__ node_of,
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__(512) k_project_banked(
     int win_levels, int top_lvl, int top_lo, int top_nb, double* __restrict__ top_scratch) {
     constexpr int B = 16, NB = kProjBank16;
     __shared__ double win[kProjWinBanks * NB];  // level 1, then level 2
-    __shared__ double top[kProjTopBanks * NB];
+    __shared__ double top[TOPB * NB];
     __shared__ double red[9][512 / 64];
     __shared__ int wbase[kProjWinLevels];
     const int b = blockIdx.x, t = threadIdx.x;
@@ -1432,12 +1434,14 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
             SGIPC_LAUNCH_CHECK();
         }
     } else if (c.K > 0 && banked) {
-        // top levels: the highest levels whose banks fit kProjTopBanks
+        // top levels: the highest levels whose banks fit kProjTopBanksMax
+        // (the kernel with kProjTopBanks of them, when they fit, runs two
+        // workgroups per CU; a level left out goes to global atomics)
         int top_lvl = L, top_lo = m.nbanks, top_nb = 0;
         for (int l = L - 1; l >= 1; --l) {
             int beg = m.level_size_h[2 * l + 1] / B;
             int end = m.level_size_h[2 * (l + 1) + 1] / B;
-            if (top_nb + (end - beg) > kProjTopBanks) break;
+            if (top_nb + (end - beg) > kProjTopBanksMax) break;
             top_nb += end - beg;
             top_lo = beg;
             top_lvl = l;
@@ -1453,10 +1457,14 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
         SGIPC_LAUNCH_CHECK();
         const int top_len = top_nb * kProjBank16;
         double* scratch = m.proj_scratch.reserve((size_t)m.nb1 * (top_len > 0 ? top_len : 1));
-        hipLaunchKernelGGL(k_project_banked, dim3(m.nb1), dim3(512), 0, s, (const int*)c.uniq_start.p,
-                           (const int*)c.t_rows, (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, L,
-                           (const int*)m.node_of.p, (const int*)vlo, (const int*)vhi, m.nbanks, m.bank_upper.p,
-                           win_levels, top_lvl, top_lo, top_nb, scratch);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(m.nb1), dim3(512), 0, s, (const int*)c.uniq_start.p, (const int*)c.t_rows,
+                               (const int*)c.t_cols, (const double*)c.t_vals, sub_offset, n, L,
+                               (const int*)m.node_of.p, (const int*)vlo, (const int*)vhi, m.nbanks, m.bank_upper.p,
+                               win_levels, top_lvl, top_lo, top_nb, scratch);
+        };
+        if (top_nb <= kProjTopBanks) launch(k_project_banked<kProjTopBanks>);
+        else launch(k_project_banked<kProjTopBanksMax>);
         SGIPC_LAUNCH_CHECK();
         if (top_len > 0) {
             hipLaunchKernelGGL(k_project_top_reduce, dim3(div_up(top_len, 64)), dim3(256), 0, s,

@@ -68,3 +68,34 @@ def test_ordered_projection_contact_solves_repeat(torch_cuda):
     st = s2.solve(tol=1e-6, stop_rule=capi.STOP_TRUE_REL)
     assert abs(st.iterations - xs[0][0]) <= 1
     assert np.linalg.norm(s2.dx.cpu().numpy() - xs[0][1]) <= 1e-5 * np.linalg.norm(xs[0][1])
+
+
+def _top_banks(ls, bank=16):
+    """Banks of levels >= 3, as the banked projection's top levels hold them
+    (level_size rows: count, first node)."""
+    L = len(ls) - 1
+    return sum((ls[l + 1][1] - ls[l][1]) // bank for l in range(3, L))
+
+
+@pytest.mark.parametrize("cells,contacts", [(40, True), (50, False)])
+def test_banked_projection_matches_ordered(torch_cuda, cells, contacts):
+    """The default (banked) Galerkin projection -- register sums for the
+    blocks inside the workgroup's level-2 node, LDS windows, LDS top levels
+    in the two-workgroups-per-CU or the twelve-bank form -- against the
+    ordered projection, through the MAS apply, to round-off: two boxes of
+    40^3 cells with contacts (top levels in six banks) and of 50^3 cells
+    (more than six: the larger form)."""
+    from sgipc import scene
+    print("case: banked vs ordered projection two_box %d contacts=%s" % (cells, contacts), flush=True)
+    sc = scene.two_box_scene(cells, seed=2)
+    pt, ee = scene.facing_contact_candidates(sc) if contacts else (None, None)
+    s = _det_system(sc, pt, ee)
+    s.build_precond()
+    tops = _top_banks(s.mas_tables()["level_size"])
+    assert (tops <= 6) if cells == 40 else (6 < tops <= 12), tops
+    r = np.random.default_rng(5).normal(size=3 * sc.n)
+    z_ord = s.precond_apply(r)
+    s.ctx.call("sgipc_set_deterministic", 0)
+    s.build_precond()
+    z_bank = s.precond_apply(r)
+    assert np.abs(z_bank - z_ord).max() <= 1e-10 * np.abs(z_ord).max()
