@@ -347,7 +347,10 @@ int sgipc_solve_batched(sgipc_ctx* ctx, int n_scenes, const int* h_row_start, co
 /* Schedule autotuner: times 48 PCG iterations of each candidate schedule on
  * the converted system (n_scenes > 0: the batched solver, else sgipc_solve)
  * -- SpMV lanes per block row 4/8/16 and, batched, 2/4/8 iterations per
- * graph launch -- and keeps the fastest for later solves; d_b is only read. */
+ * graph launch of the launch sequence and 512/1024 threads per workgroup of
+ * the persistent form -- and keeps the fastest for later solves; batched,
+ * the fastest sequence and persistent candidates are then timed on whole
+ * solves to a 1e-6 true relative residual, which decide.  d_b is only read. */
 int sgipc_autotune(sgipc_ctx* ctx, const double* d_b, int n_scenes, const int* h_row_start, int precision,
                    int* h_group, int* h_iters_per_graph);
 /* on != 0: the MAS build sums every coarse-level bank entry in a fixed
