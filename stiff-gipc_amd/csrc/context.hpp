@@ -85,13 +85,15 @@ struct MasState {
 };
 
 struct PcgGraph {
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;      // a group of two iterations
+    hipGraphExec_t exec_big = nullptr;  // a longer group (single-system PCG), same key
     size_t key = 0;
     PcgGraph() = default;
     PcgGraph(const PcgGraph&) = delete;
     PcgGraph& operator=(const PcgGraph&) = delete;
     ~PcgGraph() {
         if (exec) (void)hipGraphExecDestroy(exec);
+        if (exec_big) (void)hipGraphExecDestroy(exec_big);
     }
 };
 

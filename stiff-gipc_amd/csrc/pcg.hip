@@ -501,6 +501,45 @@ struct Engine {
         return h;
     }
 
+    static constexpr int kBigGroup = 8;  // iterations in the long captured group (even: parities repeat)
+
+    // Capture n (even) iterations into a graph; with pinned polling the
+    // group's last collect kernel (or k_publish) publishes the progress.
+    hipGraphExec_t capture_group(int n, bool pinned, PinnedPoll& pp) {
+        hipStream_t s = c.stream;
+        hipGraph_t g = nullptr;
+        SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < n; ++i) {
+                if (i == n - 1 && pinned && mas) {  // the group's last collect kernel publishes
+                    pub_h = pp.d;
+                    pub_q = reinterpret_cast<volatile double*>(pp.d + 8);
+                }
+                enqueue_iteration((i + 1) & 1);
+            }
+            pub_h = nullptr;
+            pub_q = nullptr;
+            if (pinned && !mas) {
+                hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, ctrl, (const double*)sc, pp.d,
+                                   reinterpret_cast<double*>(pp.d + 8));
+                SGIPC_LAUNCH_CHECK();
+            }
+        } catch (...) {
+            pub_h = nullptr;
+            pub_q = nullptr;
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(s, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        SGIPC_HIP(hipStreamEndCapture(s, &g));
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SGIPC_HIP(e);
+        return exec;
+    }
+
     // Run iterations until the device stop flag is set; returns the
     // reference iteration count and whether it converged.
     int run(PcgGraph& gcache, int check_every, bool& converged) {
@@ -534,34 +573,9 @@ struct Engine {
             size_t key = graph_key();
             if (!gcache.exec || gcache.key != key) {
                 if (gcache.exec) { (void)hipGraphExecDestroy(gcache.exec); gcache.exec = nullptr; }
-                hipGraph_t g = nullptr;
-                SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_iteration(1);
-                    if (pinned && mas) {  // the group's last collect kernel publishes
-                        pub_h = pp.d;
-                        pub_q = reinterpret_cast<volatile double*>(pp.d + 8);
-                    }
-                    enqueue_iteration(0);
-                    pub_h = nullptr;
-                    pub_q = nullptr;
-                    if (pinned && !mas) {
-                        hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, ctrl, (const double*)sc, pp.d,
-                                           reinterpret_cast<double*>(pp.d + 8));
-                        SGIPC_LAUNCH_CHECK();
-                    }
-                } catch (...) {
-                    pub_h = nullptr;
-                    pub_q = nullptr;
-                    hipGraph_t dead = nullptr;
-                    (void)hipStreamEndCapture(s, &dead);
-                    if (dead) (void)hipGraphDestroy(dead);
-                    throw;
-                }
-                SGIPC_HIP(hipStreamEndCapture(s, &g));
-                hipError_t e = hipGraphInstantiate(&gcache.exec, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                SGIPC_HIP(e);
+                if (gcache.exec_big) { (void)hipGraphExecDestroy(gcache.exec_big); gcache.exec_big = nullptr; }
+                gcache.exec = capture_group(2, pinned, pp);
+                if (pinned) gcache.exec_big = capture_group(kBigGroup, pinned, pp);
                 gcache.key = key;
             }
         }
@@ -570,21 +584,45 @@ struct Engine {
             // Keep two groups of iterations queued ahead of the device and
             // follow its published progress in pinned memory: the device never
             // waits for the host, and at most two groups run past the stop
-            // (their kernels return at once).
+            // (their kernels return at once).  Groups are kBigGroup
+            // iterations long (one graph launch and one publication per
+            // group) while the convergence rate seen in the publications
+            // predicts more than that many iterations to go, two iterations
+            // near the stop, so few iterations run past it.
             volatile int* hp = pp.h;
+            volatile double* hq = reinterpret_cast<volatile double*>(pp.h + 8);
             hp[0] = 0; hp[1] = 0; hp[2] = 0;
             std::atomic_thread_fence(std::memory_order_seq_cst);
-            const int max_groups = maxit / 2 + 2;
             constexpr int kAhead = 2;
             int launched = 0, last_done = -1;
+            long long launched_iters = 0;
+            int k_prev = -1;
+            double q_prev = 0.0, stop_pred = -1.0;  // predicted stopping iteration (< 0: none yet)
             auto last_change = std::chrono::steady_clock::now();
             for (;;) {
                 const int done = hp[2];
                 std::atomic_thread_fence(std::memory_order_acquire);
                 if (done > 0 && hp[0] != 0) break;  // stopped (converged or budget)
-                if (launched < max_groups && launched - done < kAhead) {
-                    SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
+                if (done != last_done && done > 0) {  // a new publication: update the prediction
+                    const int k = hp[1] - 1;
+                    const double q = hq[0];
+                    if (k_prev >= 0 && k > k_prev && q > 0.0 && q_prev > q && tol_now > 0.0) {
+                        const double rate = std::log(q / q_prev) / (double)(k - k_prev);  // < 0
+                        stop_pred = (double)k + std::log(tol_now / q) / rate;
+                    }
+                    k_prev = k;
+                    q_prev = q;
+                }
+                if (launched_iters < (long long)maxit + 2 && launched - done < kAhead) {
+                    const bool big = gcache.exec_big &&
+                                     (stop_pred < 0.0 || stop_pred - (double)launched_iters > kBigGroup + 2);
+                    SGIPC_HIP(hipGraphLaunch(big ? gcache.exec_big : gcache.exec, s));
+                    launched_iters += big ? kBigGroup : 2;
                     ++launched;
+                    if (done != last_done) {
+                        last_done = done;
+                        last_change = std::chrono::steady_clock::now();
+                    }
                     continue;
                 }
                 if (done >= launched) break;        // all launched groups ran, none stopped: budget used up
