@@ -333,6 +333,19 @@ __global__ void k_publish(int* __restrict__ ctrl, const double* __restrict__ sc,
 
 __global__ void k_set_scalar(double* sc, int idx, double v) { sc[idx] = v; }
 
+// rz0 = sum of the r.z partials; a zero rz0 (zero right-hand side) stops the
+// loop before its first iteration, so x stays 0 (the reference would divide
+// 0/0), with no host round trip: the caller reads rz0 with the final words.
+__global__ void __launch_bounds__(kThreads) k_store_rz0(const double* __restrict__ part, int np,
+                                                        double* __restrict__ sc, int* __restrict__ ctrl) {
+    __shared__ double sh[kThreads / 64];
+    const double s = sum_partials(part, np, sh);
+    if (threadIdx.x == 0) {
+        sc[S_RZ0] = s;
+        if (s == 0.0) ctrl[C_STOP] = 1;
+    }
+}
+
 __global__ void k_diag_inv(int K, const int* __restrict__ rows, const int* __restrict__ cols,
                            const double* __restrict__ vals, double* __restrict__ Dinv) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,6 +395,7 @@ struct Engine {
     bool fused = false;  // MAS apply as one launch
 
     int G = 8;  // lanes per block row in the SpMV (4, 8 or 16)
+    double rz0_h = 0.0;  // rz0 of the last run, read with its final words
     // mixed precision: the row-contiguous FP32 copy (null: the upper BCOO)
     const float* vfull = nullptr;
     // pinned progress words published by the MAS collect kernel (set while
@@ -441,9 +455,10 @@ struct Engine {
         precond_dot(rz_this, rz_last, rule);
     }
 
-    // r holds the right-hand side, x the initial guess is zero.  Returns the
-    // host copy of rz0 (0 means a zero right-hand side).
-    double init(double tol, int max_iter) {
+    // r holds the right-hand side, x the initial guess is zero.  rz0 stays on
+    // the device (run() reads it with its final words into rz0_h; 0 means a
+    // zero right-hand side, for which the loop stops at once).
+    void init(double tol, int max_iter) {
         hipStream_t s = c.stream;
         maxit = max_iter;
         tol_now = tol;
@@ -470,12 +485,8 @@ struct Engine {
         SGIPC_LAUNCH_CHECK();
         if (mas) mas_restrict(c, r, 3LL * c.sub_offset, kFp32Mas, ctrl, !fused);
         precond_dot(part_rz[0], part_rz[1], -1);
-        hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part_rz[0], nRZ, sc, (int)S_RZ0);
+        hipLaunchKernelGGL(k_store_rz0, dim3(1), dim3(kThreads), 0, s, part_rz[0], nRZ, sc, ctrl);
         SGIPC_LAUNCH_CHECK();
-        double rz0 = 0.0;
-        SGIPC_HIP(hipMemcpyAsync(&rz0, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
-        SGIPC_HIP(hipStreamSynchronize(s));
-        return rz0;
     }
 
     size_t graph_key() const {
@@ -545,6 +556,8 @@ struct Engine {
     int run(PcgGraph& gcache, int check_every, bool& converged) {
         hipStream_t s = c.stream;
         if (maxit <= 1) {  // no iteration fits the budget
+            SGIPC_HIP(hipMemcpyAsync(&rz0_h, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
+            SGIPC_HIP(hipStreamSynchronize(s));
             converged = false;
             return maxit;
         }
@@ -643,6 +656,7 @@ struct Engine {
 #endif
             }
             SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+            SGIPC_HIP(hipMemcpyAsync(&rz0_h, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
             SGIPC_HIP(hipStreamSynchronize(s));
         } else if (use_graph) {
             // Poll the stop word; after two polls the next one is placed where
@@ -655,6 +669,7 @@ struct Engine {
                 for (int j = 0; j < npairs; ++j) SGIPC_HIP(hipGraphLaunch(gcache.exec, s));
                 SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
                 SGIPC_HIP(hipMemcpyAsync(&q, sc + S_PROG, sizeof(double), hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipMemcpyAsync(&rz0_h, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
                 SGIPC_HIP(hipStreamSynchronize(s));
                 if (hc[C_STOP]) break;
                 const int k = hc[C_ITER] - 1;  // iterations done; q is the ratio after iteration k
@@ -678,6 +693,7 @@ struct Engine {
                     enqueue_iteration(0);
                 }
                 SGIPC_HIP(hipMemcpyAsync(hc, ctrl, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, s));
+                SGIPC_HIP(hipMemcpyAsync(&rz0_h, sc + S_RZ0, sizeof(double), hipMemcpyDeviceToHost, s));
                 SGIPC_HIP(hipStreamSynchronize(s));
                 if (hc[C_STOP]) break;
             }
@@ -726,17 +742,23 @@ void precond_apply(Ctx& c, const double* r, double* z) {
     SGIPC_LAUNCH_CHECK();
 }
 
-static double device_norm2(Ctx& c, const double* v, long long N, double* part, double* sc, int idx) {
+// ||v||^2 into the device scalar sc[idx] (no host read)
+static void device_norm2_to(Ctx& c, const double* v, long long N, double* part, double* sc, int idx) {
     hipStream_t s = c.stream;
     int g = grid_for(N);
     hipLaunchKernelGGL((k_dot_part<double, double>), dim3(g), dim3(kThreads), 0, s, N, v, v, part, (const int*)nullptr);
     SGIPC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, s, part, g, sc, idx);
     SGIPC_LAUNCH_CHECK();
-    double h = 0.0;
-    SGIPC_HIP(hipMemcpyAsync(&h, sc + idx, sizeof(double), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    return h;
+}
+// device scalars sc[i0] and sc[i1] to the host in one synchronisation
+static void read_scalars(Ctx& c, const double* sc, int i0, int i1, double& a, double& b) {
+    const int lo = std::min(i0, i1), hi = std::max(i0, i1);
+    double h[16];
+    SGIPC_HIP(hipMemcpyAsync(h, sc + lo, sizeof(double) * (hi - lo + 1), hipMemcpyDeviceToHost, c.stream));
+    SGIPC_HIP(hipStreamSynchronize(c.stream));
+    a = h[i0 - lo];
+    b = h[i1 - lo];
 }
 
 void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStats& st) {
@@ -776,15 +798,17 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
             E.G = G;
             hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, E.r, b);
             SGIPC_LAUNCH_CHECK();
-            double rz0 = E.init(o.tol, max_iter);
-            st.rz0 = rz0;
-            if (rz0 == 0.0) {
-                // zero right-hand side: x = 0 is exact (the reference would divide 0/0)
+            E.init(o.tol, max_iter);
+            bool conv = false;
+            const int it = E.run(fp32mas ? w.graph32 : w.graph64, o.check_every, conv);
+            st.rz0 = E.rz0_h;
+            if (st.rz0 == 0.0) {
+                // zero right-hand side: the loop stopped at once and x = 0 is
+                // exact (the reference would divide 0/0)
                 st.iterations = 0;
                 st.converged = 1;
             } else {
-                bool conv = false;
-                st.iterations = E.run(fp32mas ? w.graph32 : w.graph64, o.check_every, conv);
+                st.iterations = it;
                 st.converged = conv ? 1 : 0;
             }
         };
@@ -812,23 +836,24 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
         hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
         SGIPC_LAUNCH_CHECK();
         double* part_o = parts + 5 * kPart;
-        const double bb = device_norm2(c, b, N, part_o, sc, S_NUM);
-        const double bn = std::sqrt(bb);
+        // ||b||^2 stays on the device until the first true residual is read
+        // with it (the first inner tolerance needs only ||r|| = ||b||)
+        constexpr int S_B2 = S_NUM + 1;
+        device_norm2_to(c, b, N, part_o, sc, S_B2);
         int total = 0;
-        bool conv = false;
-        double rn = bn;
-        double rr = bb;  // ||b - A x||^2 of the current x (x = 0)
+        bool conv = false, have = false;  // have: rr, bb read from the device
+        double bb = 0.0, rr = 0.0, bn = 0.0, rn = 0.0;  // rr: ||b - A x||^2 of the current x
         st.rz0 = 0.0;
         for (int outer = 0; outer < 50 && total < max_iter; ++outer) {
-            if (rn <= o.tol * bn || bn == 0.0) { conv = true; break; }
-            double eta = std::max(0.9 * o.tol * bn / rn, o.inner_tol);
+            if (have && (rn <= o.tol * bn || bn == 0.0)) { conv = true; break; }
+            double eta = have ? std::max(0.9 * o.tol * bn / rn, o.inner_tol) : std::max(0.9 * o.tol, o.inner_tol);
             hipLaunchKernelGGL((k_copy_cast<float, double>), dim3(gN), dim3(kThreads), 0, s, N, rf, r64);
             SGIPC_LAUNCH_CHECK();
-            double rz0 = E.init(eta, max_iter - total);
-            if (outer == 0) st.rz0 = rz0;
-            if (rz0 == 0.0) { conv = true; break; }
+            E.init(eta, max_iter - total);
             bool c2 = false;
             int it = E.run(w.graph_mixed, o.check_every, c2);
+            if (outer == 0) st.rz0 = E.rz0_h;
+            if (E.rz0_h == 0.0) { conv = true; break; }  // the residual is zero: x is exact
             total += it;
             hipLaunchKernelGGL(k_axpy, dim3(gN), dim3(kThreads), 0, s, N, x, 1.0, (const double*)d64);
             SGIPC_LAUNCH_CHECK();
@@ -836,9 +861,17 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
             hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, r64, b);
             SGIPC_LAUNCH_CHECK();
             spmv64(c, -1.0, x, 1.0, r64);
-            rr = device_norm2(c, r64, N, part_o, sc, S_NUM);
+            device_norm2_to(c, r64, N, part_o, sc, S_NUM);
+            read_scalars(c, sc, S_NUM, S_B2, rr, bb);
+            have = true;
             rn = std::sqrt(rr);
+            bn = std::sqrt(bb);
             st.refinements = outer + 1;
+        }
+        if (!have) {  // no residual computed (zero right-hand side or no budget): x = 0, r = b
+            read_scalars(c, sc, S_B2, S_B2, bb, bb);
+            rr = bb;
+            rn = bn = std::sqrt(bb);
         }
         if (rn <= o.tol * bn) conv = true;
         st.iterations = total;
@@ -852,8 +885,10 @@ void pcg_solve(Ctx& c, const double* b, double* x, const SolveOpts& o, SolveStat
     hipLaunchKernelGGL((k_copy_cast<double, double>), dim3(gN), dim3(kThreads), 0, s, N, tmp, b);
     SGIPC_LAUNCH_CHECK();
     spmv64(c, -1.0, x, 1.0, tmp);
-    double rr = device_norm2(c, tmp, N, parts + 5 * kPart, sc, S_NUM);
-    double bb = device_norm2(c, b, N, parts + 5 * kPart, sc, S_NUM);
+    // ||b||^2 is the init's S_BB (r held b); one read for both
+    device_norm2_to(c, tmp, N, parts + 5 * kPart, sc, S_NUM);
+    double rr = 0.0, bb = 0.0;
+    read_scalars(c, sc, S_NUM, S_BB, rr, bb);
     st.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 

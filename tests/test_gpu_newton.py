@@ -262,3 +262,29 @@ def test_convert_long_rows_bitexact(torch_cuda, oracle, cube4):
     assert K == len(orow)
     assert np.array_equal(kr, orow) and np.array_equal(kc, ocol)
     assert np.array_equal(kv, oval)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32_apply", "mixed", "diag"])
+def test_zero_rhs_returns_zero(torch_cuda, cube10, mode):
+    """A zero right-hand side stops the loop before its first iteration (the
+    device sees rz0 = 0 itself): x = 0, no iterations, converged, and a
+    following solve of a real right-hand side is unaffected."""
+    from sgipc import capi
+    print("case: zero rhs", mode, flush=True)
+    kind = {"fp64": capi.PRECOND_MAS, "fp32_apply": capi.PRECOND_MAS_FP32, "mixed": capi.PRECOND_MAS,
+            "diag": capi.PRECOND_DIAG}[mode]
+    prec = capi.PRECISION_MIXED if mode == "mixed" else capi.PRECISION_FP64
+    sys_ = make_system(cube10, kind)
+    sys_.assemble(); sys_.convert(); sys_.build_precond()
+    b0 = sys_.b.clone()
+    ref = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000, precision=prec)
+    x_ref = sys_.dx.clone()
+    sys_.dx.fill_(1.0)
+    sys_.b.zero_()
+    st = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000, precision=prec)
+    assert st.iterations == 0 and st.converged and st.rz0 == 0.0
+    assert not torch_cuda.any(sys_.dx != 0)
+    sys_.b.copy_(b0)
+    st2 = sys_.solve(tol=1e-6, stop_rule=1, max_iter=5000, precision=prec)
+    assert st2.converged and st2.iterations == ref.iterations
+    assert torch_cuda.equal(sys_.dx, x_ref)
