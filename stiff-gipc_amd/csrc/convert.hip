@@ -402,12 +402,24 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
     int* ducol = c.dyn_ucol.reserve((size_t)s0 + 1);
     int* durun = c.dyn_urun.reserve((size_t)s0 + 1);
     sort_segment(c, 0, s0, nbr, dstart, dsorted, dus, ducol, durun, dint + 2, dint);
+    // merged block count per row and K (into dint[1]) from the cached plan,
+    // before the check of that plan is read, so one host read serves both
+    // (redone below when the plan had to be rebuilt)
+    auto merge_count = [&]() {
+        int* mcnt = c.row_count.get();  // the sort's row counts are no longer needed
+        hipLaunchKernelGGL(k_merge_count, dim3(div_up(nbr, 256)), dim3(256), 0, s, nbr, (const int*)P.ustart.p,
+                           (const int*)P.ucol.p, (const int*)dus, (const int*)ducol, mcnt);
+        SGIPC_LAUNCH_CHECK();
+        exclusive_scan(c, mcnt, ustart, nbr, dint + 1, s);
+    };
+    if (candidate) merge_count();
     int hk[8] = {0}, serr = 0;
     SGIPC_HIP(hipMemcpyAsync(hk, dint, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
     if (c.sort_err.p) SGIPC_HIP(hipMemcpyAsync(&serr, c.sort_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
     if (serr) throw std::runtime_error("convert: radix sort look-back failed");
     if (hk[2]) throw arg_error("convert: triplet row/col index out of range [0, n_block_rows)");
+    int K = hk[1];
     if (!candidate || hk[5]) {
         // (re)build the plan of the static segment
         P.valid = false;
@@ -431,15 +443,10 @@ int bcoo_convert(Ctx& c, long long nt, int nbr) {
         P.ns = ns;
         P.nbr = nbr;
         P.valid = true;
+        merge_count();
+        SGIPC_HIP(hipMemcpyAsync(&K, dint + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
     }
-    int* mcnt = c.row_count.get();  // counts no longer needed
-    hipLaunchKernelGGL(k_merge_count, dim3(div_up(nbr, 256)), dim3(256), 0, s, nbr, (const int*)P.ustart.p,
-                       (const int*)P.ucol.p, (const int*)dus, (const int*)ducol, mcnt);
-    SGIPC_LAUNCH_CHECK();
-    exclusive_scan(c, mcnt, ustart, nbr, dint + 1, s);
-    int K = 0;
-    SGIPC_HIP(hipMemcpyAsync(&K, dint + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
     int* orow = c.merge_row.reserve((size_t)K + 1);
     int* ocol = c.bcoo_col.reserve((size_t)K + 1);
     int4* desc = c.merge_desc.reserve((size_t)K + 1);
