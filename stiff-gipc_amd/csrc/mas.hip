@@ -984,55 +984,34 @@ __global__ void __launch_bounds__(kTB) k_update_restrict(
     __shared__ double sh[2 * (kTB / 64)];
     __shared__ T sv[3][kTB];
     __shared__ T l1v[3][32];
-    // this workgroup's first bank indices, and the x, r, p, Ap of this
-    // thread's first slot, loaded while the dots are reduced
+    // this workgroup's first bank indices, loaded while the dots are reduced
     BankIdx pre{0, 0, -1, -1, 0u, false};
     if ((int)blockIdx.x < g1 && (int)blockIdx.x < nb1)
         pre = bank_index(blockIdx.x, B, L, ls[2], ls[3], part_map, going_next, comp_mask, l1lo, l1hi);
-    double fx[3] = {0.0, 0.0, 0.0};
-    V fr[3] = {0, 0, 0}, fp[3] = {0, 0, 0}, fa[3] = {0, 0, 0};
-    if (pre.v0 >= 0) {
-        const size_t o = 3 * ((size_t)sub_off + pre.v0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { fr[k] = r[o + k]; fa[k] = Ap[o + k]; fp[k] = p[o + k]; fx[k] = x[o + k]; }
-    }
     double rz, pap;
     sum_partials2(part_rz, np_rz, part_pap, np_pap, sh, rz, pap);
     const double al = rz / pap;
     double acc = 0.0;
     if ((int)blockIdx.x < g1) {
         const int cnt1 = ls[2], beg1 = ls[3];
-        const int first_sl = pre.lo + (int)threadIdx.x;  // the slot whose operands are loaded above
-        bool first_bank = true;
         // only the vertex's owner reads and updates r in place (see restrict_bank)
-        auto value = [&](int v, int sl, bool own, T& v0, T& v1, T& v2) {
+        auto value = [&](int v, int, bool own, T& v0, T& v1, T& v2) {
             if (!own) return;
             const size_t o = 3 * ((size_t)sub_off + v);
-            V rv[3], av[3], pv[3];
-            double xv[3];
-            if (first_bank && sl == first_sl) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) { rv[k] = fr[k]; av[k] = fa[k]; pv[k] = fp[k]; xv[k] = fx[k]; }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) { rv[k] = r[o + k]; av[k] = Ap[o + k]; pv[k] = p[o + k]; xv[k] = x[o + k]; }
-            }
-            V r0 = (V)((double)rv[0] - al * (double)av[0]);
-            V r1 = (V)((double)rv[1] - al * (double)av[1]);
-            V r2 = (V)((double)rv[2] - al * (double)av[2]);
-            x[o] = xv[0] + al * (double)pv[0]; x[o + 1] = xv[1] + al * (double)pv[1]; x[o + 2] = xv[2] + al * (double)pv[2];
+            V r0 = (V)((double)r[o] - al * (double)Ap[o]);
+            V r1 = (V)((double)r[o + 1] - al * (double)Ap[o + 1]);
+            V r2 = (V)((double)r[o + 2] - al * (double)Ap[o + 2]);
+            x[o] += al * (double)p[o]; x[o + 1] += al * (double)p[o + 1]; x[o + 2] += al * (double)p[o + 2];
             r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
             acc += (double)r0 * (double)r0 + (double)r1 * (double)r1 + (double)r2 * (double)r2;
             v0 = (T)r0; v1 = (T)r1; v2 = (T)r2;
         };
-        for (int b = blockIdx.x; b < nb1; b += g1) {
-            first_bank = b == (int)blockIdx.x;
+        for (int b = blockIdx.x; b < nb1; b += g1)
             restrict_bank<T>(b, B, L, cnt1, beg1, part_map, going_next, fine_mask, comp_mask,
-                             first_bank ? pre
-                                        : bank_index(b, B, L, cnt1, beg1, part_map, going_next, comp_mask, l1lo,
-                                                     l1hi),
+                             b == (int)blockIdx.x ? pre
+                                                  : bank_index(b, B, L, cnt1, beg1, part_map, going_next, comp_mask,
+                                                               l1lo, l1hi),
                              R, sv, l1v, value);
-        }
     } else {
         const long long m0 = 3LL * sub_off, m1 = 3LL * ((long long)sub_off + n_mas), nout = N - (m1 - m0);
         for (long long q = (long long)(blockIdx.x - g1) * kTB + threadIdx.x; q < nout;
