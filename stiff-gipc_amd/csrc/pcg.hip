@@ -228,9 +228,10 @@ __global__ void __launch_bounds__(kThreads) k_simple_precond_dot(long long N, co
 // The same with the row-contiguous FP32 copy of both triangles: the G lanes
 // of a row read consecutive entries (40 bytes each: two 16-byte loads at
 // 8-byte alignment and one 8-byte load that also brings the column), no
-// block index, no transposition in registers, no column stream (measured
-// 44 -> 40 bytes per entry: 19.7 -> 19.2 us per launch at 137 k block
-// rows); the column's three z values come in one 12-byte load; same
+// block index, no transposition in registers, no column stream (44 -> 40
+// bytes per entry: counted traffic 96.7 -> 80.4 MB per launch at 137 k
+// block rows, the launch time within box-to-box noise of 2-3 %); the
+// column's three z values come in one 12-byte load; same
 // entry order and the same products, so the same sums.  (Measured and kept
 // out: 9-float entries read at 4-byte alignment, 30 % slower; the row's own
 // z, p, Ap loaded ahead of the row sums, 35 % slower.)
