@@ -413,6 +413,21 @@ int sgipc_get_batch_mode(sgipc_ctx* ctx, int* h_persistent, int* h_threads);
 #define SGIPC_BANK_INVERSE_MFMA64 1
 #define SGIPC_BANK_INVERSE_MFMA32_REFINED 2
 int sgipc_set_bank_inverse(sgipc_ctx* ctx, int mode);
+/* Storage of the inverses that the FP32 bank solves of the PCG iteration read
+ * (the fused top-restriction + bank-solve launch, 16-vertex banks).  No
+ * reference counterpart: the reference stores FP32 (MASPreconditioner.cu:
+ * 630-726).  Applies to the next sgipc_precond_build.
+ *  F32 (default): the FP32 inverses, 4704 bytes a bank;
+ *  F16: inv = D S D with D = sqrt(diag inv) in FP32 and S (unit diagonal,
+ *    |S_ij| <= 1) in FP16, 2544 bytes a bank; the arithmetic stays FP32.
+ *    It halves the bytes of the bank solves.  The preconditioner changes
+ *    by the FP16 rounding of S (relative 2^-11 an entry), so iteration
+ *    counts can differ slightly; the stop test and the answer's accuracy do
+ *    not depend on it.  FP64 applies and the other bank-solve paths keep
+ *    reading FP64/FP32. */
+#define SGIPC_BANK_STORE_F32 0
+#define SGIPC_BANK_STORE_F16 1
+int sgipc_set_bank_storage(sgipc_ctx* ctx, int store);
 
 /* ---- FP64 energies (line search) ------------------------------------------ */
 /* Fixed-order FP64 reductions, result on the host (synchronises the stream);
@@ -485,6 +500,9 @@ int sgipc_mas_tables(sgipc_ctx* ctx, int* h_coarse_space, int* h_going_next, int
 int sgipc_mas_bank_inverse(sgipc_ctx* ctx, int bank_index, double* h_inv);
 /* the same from the FP32 inverses the FP32 MAS apply reads */
 int sgipc_mas_bank_inverse_f32(sgipc_ctx* ctx, int bank_index, float* h_inv);
+/* the same rebuilt from the half storage (D S D, FP32), after a build with
+ * SGIPC_BANK_STORE_F16 (SGIPC_ERR_STATE otherwise) */
+int sgipc_mas_bank_inverse_f16(sgipc_ctx* ctx, int bank_index, float* h_inv);
 
 /* ---- device memory helpers for callers without a framework ---------------- */
 int sgipc_malloc(sgipc_ctx* ctx, void** d_ptr, size_t bytes);

@@ -255,10 +255,13 @@ __device__ __forceinline__ void gj16(T (&S)[16], int i, std::integer_sequence<in
 // pivot column broadcast through scalar registers.
 template <typename T>
 __global__ void __launch_bounds__(64) k_bank_inverse_mfma48(int nbanks, const double* __restrict__ bank_upper,
-                                                            double* __restrict__ inv64, float* __restrict__ inv32) {
+                                                            double* __restrict__ inv64, float* __restrict__ inv32,
+                                                            unsigned short* __restrict__ inv16,
+                                                            float* __restrict__ inv16_d) {
     typedef typename Mf<T>::acc acc;
     constexpr int D = 48, B = 16, npack = B * (B + 1) / 2, NP = D * (D + 1) / 2, LDW = 17;
     __shared__ T sw[5][16 * LDW];  // pivot inverse, row tiles (2), column tiles (2)
+    __shared__ double dg[D];       // half storage: sqrt of the inverse's diagonal
     const Tiles<T, LDW> W{sw[0]}, R0{sw[1]}, R1{sw[2]}, C0{sw[3]}, C1{sw[4]};
     const int b = blockIdx.x, l = threadIdx.x, q = l >> 4, j = l & 15;
     const double* src = bank_upper + (size_t)b * npack * 9;
@@ -349,6 +352,38 @@ __global__ void __launch_bounds__(64) k_bank_inverse_mfma48(int nbanks, const do
                     o32[e] = (float)A[ti][tj][r];
                 }
             }
+    if (inv16 == nullptr) return;
+    // the half-storage form D S D of the same inverse (the arithmetic of
+    // k_pack_inv_half: D = sqrt of the diagonal, 1 where it is not positive;
+    // S = D^-1 inv D^-1 rounded to FP16 within its range)
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * t + Mf<T>::row(q, r);
+            if (row == 16 * t + j) {
+                const double a = (double)A[t][t][r];
+                const double di = a > 0.0 ? sqrt(a) : 1.0;
+                dg[row] = di;
+                inv16_d[(size_t)b * D + row] = (float)di;
+            }
+        }
+    __syncthreads();
+    unsigned short* o16 = inv16 + (size_t)b * NP;
+#pragma unroll
+    for (int ti = 0; ti < 3; ++ti)
+#pragma unroll
+        for (int tj = ti; tj < 3; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ti + Mf<T>::row(q, r), c = 16 * tj + j;
+                if (row <= c) {
+                    const int e = row * (2 * D - row + 1) / 2 + (c - row);
+                    const float v = (float)((double)A[ti][tj][r] / (dg[row] * dg[c]));
+                    const _Float16 hv = (_Float16)fminf(fmaxf(v, -65504.f), 65504.f);
+                    o16[e] = __builtin_bit_cast(unsigned short, hv);
+                }
+            }
 }
 
 // One refinement step of the FP32 inverse X0 (inv32, packed upper):
@@ -425,14 +460,14 @@ __global__ void __launch_bounds__(64) k_bank_refine(int nbanks, const double* __
 }  // namespace
 
 void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, double* inv64, float* inv32,
-                       int mode) {
+                       int mode, unsigned short* inv16, float* inv16_d) {
     if (nbanks <= 0) return;
     hipStream_t s = c.stream;
     if (bank != 16 && bank != 32) throw arg_error("bank inverse: bank size must be 16 or 32");
     if (mode == BANK_INV_MFMA64) {
         if (bank == 16)
             hipLaunchKernelGGL((k_bank_inverse_mfma48<double>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
-                               inv64, inv32);
+                               inv64, inv32, inv16, inv16_d);
         else
             hipLaunchKernelGGL((k_bank_inverse_mfma<double, 6>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
                                inv64, inv32);
@@ -440,7 +475,7 @@ void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, d
     } else if (mode == BANK_INV_MFMA32_REFINED) {
         if (bank == 16) {
             hipLaunchKernelGGL((k_bank_inverse_mfma48<float>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper,
-                               inv64, inv32);
+                               inv64, inv32, (unsigned short*)nullptr, (float*)nullptr);  // refined below
             SGIPC_LAUNCH_CHECK();
             hipLaunchKernelGGL((k_bank_refine<3>), dim3(nbanks), dim3(64), 0, s, nbanks, bank_upper, inv32);
         } else {

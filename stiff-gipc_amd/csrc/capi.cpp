@@ -1,5 +1,6 @@
 // extern "C" boundary of libsgipc (declarations: include/sgipc.h).
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 #include "../../include/sgipc.h"
 #include "context.hpp"
@@ -792,6 +793,39 @@ int sgipc_mas_bank_inverse_f32(sgipc_ctx* c, int b, float* inv) {
         size_t e = 0;
         for (size_t r = 0; r < D; ++r)
             for (size_t q = r; q < D; ++q, ++e) inv[r * D + q] = inv[q * D + r] = up[e];
+    });
+}
+
+int sgipc_mas_bank_inverse_f16(sgipc_ctx* c, int b, float* inv) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        sgipc::MasState& m = c->mas;
+        if (!m.ready) throw sgipc::state_error("mas_bank_inverse_f16: MAS not built");
+        if (!m.inv16_ok) throw sgipc::state_error("mas_bank_inverse_f16: no half-storage inverses were built");
+        if (b < 0 || b >= m.nbanks || !inv) throw sgipc::arg_error("mas_bank_inverse_f16: bad bank");
+        set_device(c);
+        SGIPC_HIP(hipStreamSynchronize(c->stream));
+        const size_t D = 3 * (size_t)m.bank, NP = D * (D + 1) / 2;
+        std::vector<unsigned short> up(NP);
+        std::vector<float> d(D);
+        SGIPC_HIP(hipMemcpy(up.data(), m.inv16.p + (size_t)b * NP, sizeof(unsigned short) * NP, hipMemcpyDeviceToHost));
+        SGIPC_HIP(hipMemcpy(d.data(), m.inv16_d.p + (size_t)b * D, sizeof(float) * D, hipMemcpyDeviceToHost));
+        size_t e = 0;
+        for (size_t r = 0; r < D; ++r)
+            for (size_t q = r; q < D; ++q, ++e) {
+                _Float16 h;
+                std::memcpy(&h, &up[e], sizeof h);
+                inv[r * D + q] = inv[q * D + r] = d[r] * (float)h * d[q];
+            }
+    });
+}
+
+int sgipc_set_bank_storage(sgipc_ctx* c, int store) {
+    CHECK_CTX(c);
+    return guard(c, [&] {
+        if (store != SGIPC_BANK_STORE_F32 && store != SGIPC_BANK_STORE_F16)
+            throw sgipc::arg_error("set_bank_storage: unknown storage");
+        c->bank_store = store;
     });
 }
 

@@ -74,6 +74,9 @@ struct MasState {
     DBuf<double> inv64;      // nbanks * D(D+1)/2 symmetric inverse, upper triangle packed by rows (FP64)
     DBuf<float> inv32;       // same, FP32
     bool inv64_ok = true;    // inv64 holds the FP64 inverses (not after an FP32-only build)
+    DBuf<unsigned short> inv16;  // half storage: S = D^-1 inv D^-1 as FP16 bits, packed like inv64
+    DBuf<float> inv16_d;     // its D: sqrt of the inverse's diagonal, nbanks * 48
+    bool inv16_ok = false;   // inv16 built (16-vertex banks, half storage chosen)
     DBuf<int> l1lo, l1hi;    // per level-1 bank: range [lo, hi) of level-0 banks holding its children
     DBuf<int> own_lo, own_hi;  // per level-1 bank: owned vertex range (slot-ordered vertices)
     DBuf<double> proj_scratch; // per-workgroup partials of the top-level bank matrices
@@ -227,6 +230,7 @@ struct Ctx {
     // preconditioner
     int precond_kind = 0;
     int bank_inverse = 1;   // MAS bank inverses: BANK_INV_* (default: FP64 MFMA)
+    int bank_store = 0;     // storage the fused FP32 bank solves read: BANK_STORE_* (default: FP32)
     int mas_fused = 1;      // top restriction beside the bank solves (16-vertex banks; env SGIPC_MAS_FUSED=0: own launch)
     int sub_offset = 0, sub_nb = 0;
     MasState mas;
@@ -327,8 +331,11 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
 // FP64 Gauss-Jordan on MFMA, or blocked FP32 on MFMA + one refinement step
 // with the residual accumulated in FP64 (FP32 inverses only)
 enum { BANK_INV_VALU64 = 0, BANK_INV_MFMA64 = 1, BANK_INV_MFMA32_REFINED = 2 };
+enum { BANK_STORE_F32 = 0, BANK_STORE_F16 = 1 };
+// inv16/inv16_d (optional, 16-vertex banks with MFMA64): also write the
+// half-storage form of the inverses in the same launch
 void bank_inverse_mfma(Ctx& c, int bank, int nbanks, const double* bank_upper, double* inv64, float* inv32,
-                       int mode);
+                       int mode, unsigned short* inv16 = nullptr, float* inv16_d = nullptr);
 void mas_apply64(Ctx& c, const double* r, double* z);
 void mas_apply32(Ctx& c, const double* r, double* z);
 // fused multilevel restriction of r[sub3 .. sub3 + 3n) into R_l (FP64 or FP32

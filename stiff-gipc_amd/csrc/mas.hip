@@ -746,6 +746,38 @@ __global__ void __launch_bounds__(64) k_bank_inverse_reg48(int nbanks, const dou
     }
 }
 
+// Half-storage form of the packed 48 x 48 inverses for the fused FP32 bank
+// solves: D = sqrt of the inverse's diagonal (1 where it is not positive),
+// S = D^-1 inv D^-1 rounded to FP16 (unit diagonal; |S_ij| <= 1 for an SPD
+// inverse, clamped to the FP16 range otherwise).  One wavefront per bank,
+// FP64 arithmetic from the FP64 inverses when they were built.
+template <typename S>
+__global__ void __launch_bounds__(64) k_pack_inv_half(int nbanks, const S* __restrict__ inv,
+                                                      unsigned short* __restrict__ h, float* __restrict__ dsc) {
+    constexpr int D = 48, NP = D * (D + 1) / 2;
+    __shared__ double d[D];
+    const int b = blockIdx.x, i = threadIdx.x;
+    if (b >= nbanks) return;
+    const S* src = inv + (size_t)b * NP;
+    if (i < D) {
+        const double a = (double)src[pack_index(D, i, i)];
+        const double di = a > 0.0 ? sqrt(a) : 1.0;
+        d[i] = di;
+        dsc[(size_t)b * D + i] = (float)di;
+    }
+    __syncthreads();
+    unsigned short* dst = h + (size_t)b * NP;
+    for (int r = 0; r < D; ++r) {
+        const int c = r + i;
+        if (c < D) {
+            const int q = pack_index(D, r, c);
+            const float v = (float)((double)src[q] / (d[r] * d[c]));
+            const _Float16 hv = (_Float16)fminf(fmaxf(v, -65504.f), 65504.f);
+            dst[q] = __builtin_bit_cast(unsigned short, hv);
+        }
+    }
+}
+
 // ---- apply -----------------------------------------------------------------
 // value of v in lane j of the wavefront (j uniform)
 __device__ inline float lane_bcast(float v, int j) {
@@ -1115,12 +1147,59 @@ __device__ inline T wave_bank_solve(const T* __restrict__ inv, const T* __restri
     return acc;
 }
 
-template <typename T>
+// The same solve from the half-storage form of an inverse: inv = D S D with
+// D = diag(sqrt(inv_ii)) (FP32, one per bank row) and S (unit diagonal,
+// |S_ij| <= 1 for an SPD inverse) packed like inv as FP16, so the bank reads
+// 2352 + 192 bytes instead of 4704.  z_i = d_i sum_j S_ij (d_j r_j), FP32
+// arithmetic.
+__device__ inline float wave_bank_solve_h(const unsigned short* __restrict__ inv, const float* __restrict__ dsc,
+                                          const float* __restrict__ R, int b, void* lds) {
+    constexpr int D = 48, NP = D * (D + 1) / 2;
+    constexpr int NV = NP * 2 / 16;  // 147 16-byte pieces per bank (2352 bytes)
+    constexpr int PER = (NV + 63) / 64;
+    float4* As = static_cast<float4*>(lds);
+    const int i = threadIdx.x & 63;
+    const float4* src = reinterpret_cast<const float4*>(inv + (size_t)b * NP);
+    const float di = dsc[(size_t)b * D + min(i, D - 1)];
+    const float rv = R[(size_t)b * D + min(i, D - 1)] * di;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q0 = k * 64;
+        __builtin_amdgcn_global_load_lds((gptr_t)(src + min(q0 + i, NV - 1)), (lptr_t)(As + q0), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    const _Float16* A = reinterpret_cast<const _Float16*>(As);
+    float acc = 0.f;
+    if (i < D) {
+        int idx = i;
+#pragma unroll 8
+        for (int j = 0; j < D; ++j) {
+            acc += (float)A[idx] * lane_bcast(rv, j);
+            idx += (j < i) ? (D - 1 - j) : 1;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return acc * di;
+}
+
+// one bank solve from either storage (H: the half form; inv is then the FP16
+// S and dsc its row scales)
+template <typename T, bool H>
+__device__ inline T bank_solve_any(const void* __restrict__ inv, const float* __restrict__ dsc,
+                                   const T* __restrict__ R, int b, void* lds) {
+    if constexpr (H) return wave_bank_solve_h(static_cast<const unsigned short*>(inv), dsc, R, b, lds);
+    else return wave_bank_solve<T>(static_cast<const T*>(inv), R, b, lds);
+}
+
+template <typename T, bool H>
 __global__ void __launch_bounds__(kCT) k_mas_solve_top(int L, const int* __restrict__ ls,
                                                      const int* __restrict__ going_next,
                                                      const unsigned* __restrict__ comp_mask, int top_bank0,
-                                                     int nbanks, const T* __restrict__ inv, T* __restrict__ R,
+                                                     int nbanks, const void* __restrict__ inv,
+                                                     const float* __restrict__ dsc, T* __restrict__ R,
                                                      T* __restrict__ Z, const int* __restrict__ ctrl, SceneSkip skip) {
+    static_assert(!H || sizeof(T) == 4, "half-storage inverses serve FP32 applies");
     if (ctrl[C_STOP]) return;
     constexpr int D = 48;
     using V4 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
@@ -1167,14 +1246,14 @@ __global__ void __launch_bounds__(kCT) k_mas_solve_top(int L, const int* __restr
         __syncthreads();
         for (int b = top_bank0 + wv; b < nbanks; b += kCW) {
             if (skip.stopped(b)) continue;
-            const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
+            const T zb = bank_solve_any<T, H>(inv, dsc, R, b, &lds[wv][0]);
             if (lane < D) Z[(size_t)b * D + lane] = zb;
         }
         return;
     }
     const int b = (blockIdx.x - 1) * kCW + wv;  // levels 0 .. 2
     if (b < top_bank0 && !skip.stopped(b)) {
-        const T zb = wave_bank_solve<T>(inv, R, b, &lds[wv][0]);
+        const T zb = bank_solve_any<T, H>(inv, dsc, R, b, &lds[wv][0]);
         if (lane < D) Z[(size_t)b * D + lane] = zb;
     }
 }
@@ -1476,8 +1555,18 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
     int inv_mode = c.bank_inverse;
     if (inv_mode == BANK_INV_MFMA32_REFINED && c.precond_kind != 3) inv_mode = BANK_INV_MFMA64;
     m.inv64_ok = inv_mode != BANK_INV_MFMA32_REFINED;
+    // half storage for the fused FP32 bank solves (16-vertex banks only):
+    // written by the FP64 MFMA inversion itself, else packed afterwards
+    m.inv16_ok = false;
+    const bool half = c.bank_store == BANK_STORE_F16 && B == 16 && m.nbanks > 0;
+    if (half) {
+        m.inv16.reserve((size_t)m.nbanks * (D * (D + 1) / 2) + 8);  // 2352 bytes a bank: 16-byte aligned banks
+        m.inv16_d.reserve((size_t)m.nbanks * D);
+    }
+    const bool half_fused = half && inv_mode == BANK_INV_MFMA64;
     if (inv_mode != BANK_INV_VALU64) {
-        bank_inverse_mfma(c, B, m.nbanks, m.bank_upper.p, m.inv64.p, m.inv32.p, inv_mode);
+        bank_inverse_mfma(c, B, m.nbanks, m.bank_upper.p, m.inv64.p, m.inv32.p, inv_mode,
+                          half_fused ? m.inv16.p : nullptr, half_fused ? m.inv16_d.p : nullptr);
     } else {
         if (B == 16)
             hipLaunchKernelGGL(k_bank_inverse_reg48, dim3(m.nbanks), dim3(64), 0, s, m.nbanks, m.bank_upper.p,
@@ -1487,6 +1576,16 @@ void mas_build(Ctx& c, int sub_offset, int sub_nb, const int* pairs, int npairs)
                                m.inv64.p, m.inv32.p);
         SGIPC_LAUNCH_CHECK();
     }
+    if (half && !half_fused) {
+        if (m.inv64_ok)
+            hipLaunchKernelGGL(k_pack_inv_half<double>, dim3(m.nbanks), dim3(64), 0, s, m.nbanks,
+                               (const double*)m.inv64.p, m.inv16.p, m.inv16_d.p);
+        else
+            hipLaunchKernelGGL(k_pack_inv_half<float>, dim3(m.nbanks), dim3(64), 0, s, m.nbanks,
+                               (const float*)m.inv32.p, m.inv16.p, m.inv16_d.p);
+        SGIPC_LAUNCH_CHECK();
+    }
+    m.inv16_ok = half;
     // apply work vectors: allocated here so their addresses are stable for
     // captured PCG graphs; padding entries stay zero (never written)
     m.R64.reserve(3 * (size_t)m.total + 3);
@@ -1572,14 +1671,14 @@ void mas_update_restrict(Ctx& c, double* x, float* r, const float* p, const floa
     update_restrict_t<float>(c, x, r, p, Ap, part_pap, np_pap, part_rz, np_rz, part_rr, ctrl, fp32, top);
 }
 
-template <typename T>
-static void solve_top_t(Ctx& c, const T* inv, T* R, T* Z, const int* ctrl, SceneSkip skip) {
+template <typename T, bool H>
+static void solve_top_t(Ctx& c, const void* inv, const float* dsc, T* R, T* Z, const int* ctrl, SceneSkip skip) {
     MasState& m = c.mas;
     const int top_bank0 = m.L >= 4 ? m.level_size_h[7] / m.bank : m.nbanks;
     prof_begin(c, PROF_MAS_SOLVE);
-    hipLaunchKernelGGL((k_mas_solve_top<T>), dim3(1 + div_up(top_bank0, kCW)), dim3(kCT), 0, c.stream, m.L,
+    hipLaunchKernelGGL((k_mas_solve_top<T, H>), dim3(1 + div_up(top_bank0, kCW)), dim3(kCT), 0, c.stream, m.L,
                        (const int*)m.level_size.p, (const int*)m.going_next.p, (const unsigned*)m.comp_mask.p,
-                       top_bank0, m.nbanks, inv, R, Z, ctrl, skip);
+                       top_bank0, m.nbanks, inv, dsc, R, Z, ctrl, skip);
     SGIPC_LAUNCH_CHECK();
     prof_end(c, PROF_MAS_SOLVE);
 }
@@ -1587,10 +1686,12 @@ static void solve_top_t(Ctx& c, const T* inv, T* R, T* Z, const int* ctrl, Scene
 bool mas_fused(const Ctx& c) { return c.mas_fused && c.mas.bank == 16 && c.mas.ready; }
 
 void mas_solve_with_top(Ctx& c, bool fp32, const int* ctrl, SceneSkip skip) {
-    if (fp32) solve_top_t<float>(c, c.mas.inv32.p, c.mas.R32.p, c.mas.Z32.p, ctrl, skip);
+    MasState& m = c.mas;
+    if (fp32 && m.inv16_ok) solve_top_t<float, true>(c, m.inv16.p, m.inv16_d.p, m.R32.p, m.Z32.p, ctrl, skip);
+    else if (fp32) solve_top_t<float, false>(c, m.inv32.p, nullptr, m.R32.p, m.Z32.p, ctrl, skip);
     else {
-        if (!c.mas.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
-        solve_top_t<double>(c, c.mas.inv64.p, c.mas.R64.p, c.mas.Z64.p, ctrl, skip);
+        if (!m.inv64_ok) throw state_error("MAS: only FP32 inverses were built");
+        solve_top_t<double, false>(c, m.inv64.p, nullptr, m.R64.p, m.Z64.p, ctrl, skip);
     }
 }
 

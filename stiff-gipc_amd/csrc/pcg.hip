@@ -509,6 +509,7 @@ struct Engine {
             mix((size_t)m.comp_mask.p); mix((size_t)m.fine_mask.p);
             mix((size_t)m.inv64.p); mix((size_t)m.inv32.p); mix((size_t)m.R64.p); mix((size_t)m.R32.p);
             mix((size_t)m.Z64.p); mix((size_t)m.Z32.p);
+            mix((size_t)m.inv16_ok); mix((size_t)m.inv16.p); mix((size_t)m.inv16_d.p);
         }
         return h;
     }

@@ -1,6 +1,7 @@
-// Device-wide exclusive prefix sum of int32 counts (three passes: tile sums,
-// one-block scan of the tile sums, tile rescan).  Used by the BCOO convert,
-// the full-BSR build and the MAS level numbering.
+// Device-wide exclusive prefix sum of int32 counts: one workgroup for up to
+// 16384 entries (the MAS level numbering scans one flag per bank), else three
+// passes (tile sums, one-block scan of the tile sums, tile rescan).  Used by
+// the BCOO convert, the full-BSR build and the MAS level numbering.
 #include "context.hpp"
 
 namespace sgipc {
@@ -8,9 +9,13 @@ namespace sgipc {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallItems = 16;
+constexpr int kSmallScan = kSmallThreads * kSmallItems;  // 16384
 
+template <int NT>
 __device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
-    // sh: kScanThreads/64 + 1 ints
+    // sh: NT/64 + 1 ints
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int incl = v;
 #pragma unroll
@@ -22,15 +27,15 @@ __device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int acc = 0;
-        for (int w = 0; w < kScanThreads / 64; ++w) {
+        for (int w = 0; w < NT / 64; ++w) {
             int t = sh[w];
             sh[w] = acc;
             acc += t;
         }
-        sh[kScanThreads / 64] = acc;
+        sh[NT / 64] = acc;
     }
     __syncthreads();
-    total = sh[kScanThreads / 64];
+    total = sh[NT / 64];
     int r = sh[wid] + incl - v;
     __syncthreads();
     return r;
@@ -45,7 +50,7 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* __restric
     for (int i = 0; i < kScanItems; ++i)
         if (base + i < n) s += in[base + i];
     int total;
-    (void)block_exclusive_scan(s, sh, total);
+    (void)block_exclusive_scan<kScanThreads>(s, sh, total);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
@@ -61,7 +66,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_sums(int* __restrict
         s += vals[i];
     }
     int total;
-    int run = block_exclusive_scan(s, sh, total);
+    int run = block_exclusive_scan<kScanThreads>(s, sh, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         if (base + i < ntiles) tile_sums[base + i] = run;
@@ -82,7 +87,7 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_apply(const int* __restri
         s += vals[i];
     }
     int total;
-    int run = block_exclusive_scan(s, sh, total) + tile_offsets[blockIdx.x];
+    int run = block_exclusive_scan<kScanThreads>(s, sh, total) + tile_offsets[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         if (base + i < n) out[base + i] = run;
@@ -90,6 +95,50 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_apply(const int* __restri
     }
     // out has n+1 entries: out[n] = grand total
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
+}
+
+// n <= kSmallScan: the whole scan in one workgroup.  The input goes through
+// LDS so that global loads and stores are coalesced (item k = i * 1024 + t);
+// thread t then scans its 16 consecutive items there.  The whole input is
+// read before anything is written, so in and out may alias.
+__device__ __forceinline__ int small_slot(int k) { return k + (k >> 5); }  // one skip word per 32: the
+                                                                            // 16-word lane stride spreads over the banks
+
+__global__ void __launch_bounds__(kSmallThreads) k_scan_small(const int* in, int n, int* out,
+                                                              int* __restrict__ total_out) {
+    __shared__ int buf[kSmallScan + kSmallScan / 32];
+    __shared__ int sh[kSmallThreads / 64 + 1];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        const int k = i * kSmallThreads + t;
+        buf[small_slot(k)] = k < n ? in[k] : 0;
+    }
+    __syncthreads();
+    const int base = t * kSmallItems;
+    int vals[kSmallItems], s = 0;
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        vals[i] = buf[small_slot(base + i)];
+        s += vals[i];
+    }
+    int total;
+    int run = block_exclusive_scan<kSmallThreads>(s, sh, total);  // ends with a barrier: every read of buf is done
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        buf[small_slot(base + i)] = run;
+        run += vals[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSmallItems; ++i) {
+        const int k = i * kSmallThreads + t;
+        if (k < n) out[k] = buf[small_slot(k)];
+    }
+    if (t == 0) {
+        out[n] = total;
+        if (total_out) *total_out = total;
+    }
 }
 
 // one level: tile sums of in, their exclusive scan (one block when they fit
@@ -114,6 +163,11 @@ void exclusive_scan(Ctx& c, const int* in, int* out, int n, int* total_dev, hipS
     if (n <= 0) {
         fill<int>(out, 0, 1, s);
         if (total_dev) fill<int>(total_dev, 0, 1, s);
+        return;
+    }
+    if (n <= kSmallScan) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallThreads), 0, s, in, n, out, total_dev);
+        SGIPC_LAUNCH_CHECK();
         return;
     }
     // scratch: the tile sums of every level (each level's scan writes one

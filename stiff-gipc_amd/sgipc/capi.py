@@ -15,6 +15,7 @@ PRECOND_NONE, PRECOND_DIAG, PRECOND_MAS, PRECOND_MAS_FP32 = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 STOP_REF_RZ, STOP_TRUE_REL = 0, 1
 BANK_INVERSE_VALU64, BANK_INVERSE_MFMA64, BANK_INVERSE_MFMA32_REFINED = 0, 1, 2
+BANK_STORE_F32, BANK_STORE_F16 = 0, 1
 SORT_OWN, SORT_ROCPRIM = 0, 1
 PROF_SPMV, PROF_MAS_SOLVE, PROF_FEM, PROF_CONVERT, PROF_PRECOND_BUILD, PROF_SOLVE, PROF_BARRIER = range(7)
 
@@ -108,6 +109,8 @@ SIGNATURES = {
     "sgipc_mas_bank_inverse": ([P, I, P], I),
     "sgipc_mas_bank_inverse_f32": ([P, I, P], I),
     "sgipc_set_bank_inverse": ([P, I], I),
+    "sgipc_mas_bank_inverse_f16": ([P, I, P], I),
+    "sgipc_set_bank_storage": ([P, I], I),
     "sgipc_malloc": ([P, C.POINTER(P), SZ], I),
     "sgipc_free": ([P, P], I),
     "sgipc_memcpy_h2d": ([P, P, P, SZ], I),

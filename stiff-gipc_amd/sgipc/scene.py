@@ -97,6 +97,16 @@ def renumber_by_part(part, *arrays_to_permute):
     return perm, inv, part[perm]
 
 
+def order_tets_by_vertex(tets):
+    """Stable sort of tets by their smallest vertex id.  With vertices in part
+    order this puts the tets around a vertex next to each other, so the
+    per-tet Hessian triplets that one block row sums are written close
+    together and the convert's merge gathers them from few cache lines
+    (in cell order the tets around a vertex are a z-slab of cells apart)."""
+    order = np.argsort(tets.min(axis=1), kind="stable")
+    return np.ascontiguousarray(tets[order])
+
+
 @dataclass
 class Body:
     """One deformable body of the FEM subsystem (already part-sorted)."""
@@ -163,7 +173,7 @@ def make_box(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), bank=16, pin_bottom=False):
     part = block_partition(ijk, bank)
     perm, inv, part_sorted = renumber_by_part(part)
     x = verts[perm]
-    tets = inv[tets.astype(np.int64)].astype(np.uint32)
+    tets = order_tets_by_vertex(inv[tets.astype(np.int64)].astype(np.uint32))
     ijk = ijk[perm]
     btype = np.zeros(x.shape[0], np.int32)
     if pin_bottom:

@@ -250,3 +250,11 @@ class NewtonSystem:
         out = np.zeros(D * D, np.float32)
         self.ctx.call("sgipc_mas_bank_inverse_f32", b, out.ctypes.data)
         return out.reshape(D, D)
+
+    def bank_inverse_f16(self, b):
+        """D S D rebuilt from the half storage (after a build with
+        sgipc_set_bank_storage(SGIPC_BANK_STORE_F16))."""
+        D = 3 * self.sc.bank
+        out = np.zeros(D * D, np.float32)
+        self.ctx.call("sgipc_mas_bank_inverse_f16", b, out.ctypes.data)
+        return out.reshape(D, D)
