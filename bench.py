@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-half-store", action="store_true",
                     help="skip the side record of the mixed form with FP16 bank-inverse storage")
+    ap.add_argument("--bank-store", choices=["f32", "f16"], default="f32",
+                    help="batched: storage of the MAS bank inverses the FP32 bank solves of the launch sequence "
+                         "read (f16: D S D with S in FP16, opt-in)")
     ap.add_argument("--workload", choices=["contact", "cantilever_adjoint", "batched", "hybrid"], default="contact",
                     help="contact: the Newton-step benchmark (default); cantilever_adjoint: the differentiable "
                          "rollout (60x12x12-cell cantilever over a ground barrier, 20 steps) with its adjoint "
@@ -835,6 +838,8 @@ def batched(a):
     print("batched: generating %d scenes ..." % a.scenes, file=sys.stderr, flush=True)
     scs = scene.batch_scenes(a.scenes, n_min=a.scene_vertices[0], n_max=a.scene_vertices[1])
     bn = BatchedNewton(scs, "cuda:0", precond=capi.PRECOND_MAS_FP32)
+    if a.bank_store == "f16":
+        bn.sys.ctx.call("sgipc_set_bank_storage", capi.BANK_STORE_F16)
     print("batched: %d vertices, %d tets; warm-up + autotune ..." % (bn.sc.n, bn.sc.T), file=sys.stderr, flush=True)
     bn.newton_step(precision=capi.PRECISION_MIXED)
     tuned = bn.autotune(capi.PRECISION_MIXED)
@@ -917,6 +922,8 @@ def batched(a):
                                "vertices), E log-U[1e4,1e7], nu U[0.3,0.48], dt=0.01" %
                                (bn.S, bn.sc.n, bn.sc.T, min(s.n for s in scs), max(s.n for s in scs)),
                    "precision": "FP32 inner PCG, per-scene FP64 refinement", "preconditioner": "MAS, 16-vertex banks",
+                   "bank_inverse_storage": ("FP16 S with FP32 row scales (opt-in; read by the launch sequence, the "
+                                            "persistent form reads FP32)") if a.bank_store == "f16" else "FP32",
                    "schedule": {"spmv_lanes_per_row": tuned[0], "iterations_per_graph": tuned[1],
                                 "form": ("persistent launch, %d threads per workgroup" % form[1]) if form[0]
                                 else "graph-captured launch sequence",

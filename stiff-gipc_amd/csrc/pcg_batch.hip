@@ -458,6 +458,7 @@ struct BEngine {
         mix((size_t)m.nb1); mix((size_t)m.node_of.p); mix((size_t)m.inv64.p); mix((size_t)m.inv32.p);
         mix((size_t)m.R64.p); mix((size_t)m.R32.p); mix((size_t)m.Z64.p); mix((size_t)m.Z32.p);
         mix((size_t)m.going_next.p); mix((size_t)m.level_size.p); mix((size_t)m.l1lo.p);
+        mix((size_t)m.inv16_ok); mix((size_t)m.inv16.p); mix((size_t)m.inv16_d.p);
         return h;
     }
 

@@ -158,6 +158,43 @@ def test_batched_mixed_reaches_tolerance_per_scene(torch_cuda, five_scenes, form
     bn.sys.build_precond(capi.PRECOND_MAS)
 
 
+def test_batched_half_bank_storage(torch_cuda, five_scenes):
+    """The launch sequence's FP32 bank solves reading the FP16 storage
+    (sgipc_set_bank_storage): every scene reaches the 1e-6 true residual
+    (checked against scipy's matrix), with iteration counts within a few of
+    the FP32 storage's; back in FP32 storage after a rebuild the iteration
+    counts are the FP32 ones again (the solve's cached graph follows the
+    storage)."""
+    from sgipc import capi
+    scs, bn = five_scenes
+    print("case: batched S=5 mixed, FP16 bank storage (sequence)", flush=True)
+    bn.sys.build_precond(capi.PRECOND_MAS_FP32)
+    st32, _ = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_MIXED, 1e-6, 1)
+    its32 = [st32[s].iterations for s in range(bn.S)]
+    kr, kc, kv = bn.sys.bcoo()
+    A = _csr(kr, kc, kv, bn.sc.n)
+    b = bn.sys.b.cpu().numpy()
+    try:
+        bn.sys.ctx.call("sgipc_set_bank_storage", capi.BANK_STORE_F16)
+        bn.sys.build_precond(capi.PRECOND_MAS_FP32)
+        st, x = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_MIXED, 1e-6, 1)
+        its16 = [st[s].iterations for s in range(bn.S)]
+        print("  iterations FP32 storage %s, FP16 storage %s" % (its32, its16), flush=True)
+        for s in range(bn.S):
+            sl = bn.scene_slice(s)
+            rel = np.linalg.norm(b[sl] - A[sl, sl] @ x[sl]) / np.linalg.norm(b[sl])
+            assert st[s].converged and st[s].rel_residual <= 1e-6 and rel <= 1.5e-6, (s, rel)
+            assert abs(its16[s] - its32[s]) <= 3, (s, its16[s], its32[s])
+    finally:
+        bn.sys.ctx.call("sgipc_set_bank_storage", capi.BANK_STORE_F32)
+    bn.sys.build_precond(capi.PRECOND_MAS_FP32)
+    st2, _ = _solve_batched(bn.sys, bn.row_start, capi.PRECISION_MIXED, 1e-6, 1)
+    # (the banked projection's atomic sums may reorder between builds: one
+    # iteration of slack)
+    assert all(abs(st2[s].iterations - its32[s]) <= 1 for s in range(bn.S))
+    bn.sys.build_precond(capi.PRECOND_MAS)
+
+
 @pytest.mark.parametrize("precond", ["MAS", "MAS_FP32"])
 @pytest.mark.parametrize("precision", ["FP64", "MIXED"])
 def test_persistent_matches_launch_sequence(torch_cuda, five_scenes, precond, precision):
