@@ -414,8 +414,9 @@ int sgipc_get_batch_mode(sgipc_ctx* ctx, int* h_persistent, int* h_threads);
 #define SGIPC_BANK_INVERSE_MFMA32_REFINED 2
 int sgipc_set_bank_inverse(sgipc_ctx* ctx, int mode);
 /* Storage of the inverses that the FP32 bank solves of the PCG iteration read
- * (the fused top-restriction + bank-solve launch, 16-vertex banks).  No
- * reference counterpart: the reference stores FP32 (MASPreconditioner.cu:
+ * (the fused top-restriction + bank-solve launch, 16-vertex banks: the
+ * single-system solve and the batched launch sequence).  No reference
+ * counterpart: the reference stores FP32 (MASPreconditioner.cu:
  * 630-726).  Applies to the next sgipc_precond_build.
  *  F32 (default): the FP32 inverses, 4704 bytes a bank;
  *  F16: inv = D S D with D = sqrt(diag inv) in FP32 and S (unit diagonal,
@@ -423,8 +424,8 @@ int sgipc_set_bank_inverse(sgipc_ctx* ctx, int mode);
  *    It halves the bytes of the bank solves.  The preconditioner changes
  *    by the FP16 rounding of S (relative 2^-11 an entry), so iteration
  *    counts can differ slightly; the stop test and the answer's accuracy do
- *    not depend on it.  FP64 applies and the other bank-solve paths keep
- *    reading FP64/FP32. */
+ *    not depend on it.  FP64 applies, the persistent batched form and the
+ *    other bank-solve paths keep reading FP64/FP32. */
 #define SGIPC_BANK_STORE_F32 0
 #define SGIPC_BANK_STORE_F16 1
 int sgipc_set_bank_storage(sgipc_ctx* ctx, int store);
